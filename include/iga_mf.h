@@ -1,0 +1,280 @@
+/*
+ * iga_mf.h -- C ABI of the MI355X (gfx950) matrix-free isogeometric Poisson
+ * operator: Bezier-extracted (NURBS) basis, Gauss quadrature, element action
+ * and global scatter/gather on one GPU, plus the solver around it.
+ *
+ * Plain C types only: device buffers are passed as pointers the caller owns,
+ * streams as `void*` (a hipStream_t; NULL = the null stream), sizes as ints.
+ * Every call returns an iga_status; nothing prints or exits.
+ *
+ * Conventions (those of the reference, SeaVegetable/Parallel-Computing):
+ *   - functions (control points) numbered x-fastest, A = j*nfx + i,
+ *     nfx = nelem_x + p, nfy = nelem_y + q; vectors have nfx*nfy entries,
+ *     Dirichlet functions included (matrix_free_gpu/matrixfreegpu.cu:68-78);
+ *   - elements numbered x-fastest, local basis a = jloc*(p+1) + iloc,
+ *     quadrature points y-outer, x-inner (matrix_free/src/ElementMF.cpp:
+ *     270-276);
+ *   - the operator is the reference's, sign included:
+ *       y = P_int * sum_e P_e^T K_e x[IEN_e],
+ *       K_e = -sum_q kappa_q w_q det(J_q) grad R_a . grad R_b,
+ *     x gathered for all functions, rows of Dirichlet functions zeroed
+ *     (matrix_free/src/GlobalAssemblyMF.cpp:141-210).
+ */
+#ifndef IGA_MF_H
+#define IGA_MF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status -------------------------------------------------------------
+ * Replaces the reference's print-and-exit error handling
+ * (matrix_free_gpu/include/memory.cuh:9-12, matrix_free_gpu/src/mult.cu:
+ * 510-515, 528-532, 550-557; matrix_free/src/BernsteinBasis.cpp:67-70;
+ * QuadraturePoint.cpp:48-51; ControlPointGenerator.cpp:11-15). */
+typedef enum {
+  IGA_OK = 0,
+  IGA_ERR_INVALID_ARGUMENT = 1,   /* NULL where a buffer is required, bad enum */
+  IGA_ERR_INVALID_DEGREE = 2,     /* p or q outside [1,8] */
+  IGA_ERR_INVALID_QUADRATURE = 3, /* quadrature points outside [1,10] */
+  IGA_ERR_TOO_FEW_ELEMENTS = 4,   /* nelem < p + 2 (nfunc < 2p + 2) */
+  IGA_ERR_BAD_MESH = 5,           /* IEN / ID / Dir / extraction inconsistent */
+  IGA_ERR_UNSUPPORTED = 6,        /* valid input with no kernel built for it */
+  IGA_ERR_HIP = 7,                /* a HIP runtime call failed */
+  IGA_ERR_OUT_OF_MEMORY = 8,
+  IGA_ERR_NOT_FINITE = 9,         /* non-finite value in an input vector */
+  IGA_ERR_DIVERGED = 10,          /* residual grew past divtol */
+  IGA_ERR_NOT_CONVERGED = 11,     /* maxit reached */
+  IGA_ERR_SIZE_LIMIT = 12         /* request larger than a fixed capacity */
+} iga_status;
+
+const char *iga_status_string(int status);
+
+/* ---- mesh description ---------------------------------------------------
+ * Host arrays, read during iga_op_create only.  The fields are what the
+ * reference's MatMulMF / MatrixFreeMatMultCUDA receive
+ * (matrix_free_gpu/include/GlobalAssemblyMF.cuh:36-48,
+ * matrix_free_gpu/include/mult.cuh:16-24); the Bernstein tables and Gauss
+ * rule the reference also passes are rebuilt from p and nqp. */
+typedef struct {
+  int p, q;                    /* degrees */
+  int nelem_x, nelem_y;        /* elements per direction */
+  int nqp_x, nqp_y;            /* Gauss points per direction (reference: p+1) */
+  const double *cp;            /* [nfunc][2] control points (x,y) */
+  const double *weights;       /* [nfunc] NURBS weights, NULL = all one */
+  const int *ien;              /* [E][(p+1)(q+1)], NULL = tensor-product IEN */
+  const int *id;               /* [nfunc] equation numbers, -1 = Dirichlet */
+  const int *dir;              /* [ndir] Dirichlet functions */
+  int ndir;
+  const double *extraction_x;  /* [nelem_x][(p+1)^2], row = B-spline */
+  const double *extraction_y;  /* [nelem_y][(q+1)^2] */
+  const double *elem_size_x;   /* [nelem_x] parametric element sizes */
+  const double *elem_size_y;   /* [nelem_y] */
+  const double *kappa;         /* [E][nqp_x*nqp_y] conductivity, NULL = 1 */
+} iga_mesh_desc;
+
+/* ---- mesh generation (the reference's preprocessing, in memory) ---------
+ * Replaces matrix_free/preprocess.cpp:16-98 (knots, IEN, ID, control points,
+ * extraction) and matrix_free_gpu/src/Partition.cpp:238-281 (serial
+ * renumbering); files are not written. */
+typedef struct iga_mesh iga_mesh;
+
+typedef enum {
+  IGA_GEOM_UNIT_SQUARE = 0,     /* [0,L]^2, identity map, Greville points */
+  IGA_GEOM_QUARTER_ANNULUS = 1, /* r in [r0,r1], theta in [0,pi/2], exact NURBS */
+  IGA_GEOM_PERTURBED_SQUARE = 2 /* interior points moved U[-a h, a h], weights U[w0,w1] */
+} iga_geometry;
+
+typedef struct {
+  iga_geometry kind;
+  int p;                  /* degree in both directions */
+  int nelem_x, nelem_y;
+  double L;               /* unit square side */
+  double r0, r1;          /* annulus radii */
+  double perturb;         /* perturbation amplitude as a fraction of h */
+  double w_lo, w_hi;      /* weight range of the perturbed square */
+  uint64_t seed;          /* splitmix64 seed of the perturbed square */
+} iga_mesh_params;
+
+int iga_mesh_build(const iga_mesh_params *params, iga_mesh **out);
+/* Fill a descriptor that points into the mesh's arrays (valid until
+ * iga_mesh_destroy). */
+int iga_mesh_describe(const iga_mesh *m, iga_mesh_desc *desc);
+int iga_mesh_destroy(iga_mesh *m);
+
+/* splitmix64 -> Box-Muller, the generator of every synthetic input */
+int iga_rng_normal(uint64_t seed, int64_t n, double *out);
+int iga_rng_uniform(uint64_t seed, int64_t n, double lo, double hi, double *out);
+
+/* ---- the operator -------------------------------------------------------
+ * Replaces GlobalAssemblyMF::MatMulMF (.cu) and MatrixFreeMatMultCUDA
+ * (matrix_free_gpu/src/GlobalAssemblyMF.cu:141-242,
+ * matrix_free_gpu/src/mult.cu:518-558), which re-upload every mesh array on
+ * every apply: here the mesh is uploaded once and the per-quadrature-point
+ * factors G_q = kappa_q w_q det(J_q) J_q^-1 J_q^-T are precomputed in FP64. */
+typedef struct iga_op iga_op;
+
+typedef enum { IGA_F64 = 0, IGA_F32 = 1 } iga_dtype;
+
+typedef enum {
+  IGA_KERNEL_SF = 0,  /* sum-factorised element action */
+  IGA_KERNEL_EBE = 1  /* element-by-element dense action */
+} iga_kernel;
+
+typedef enum {
+  IGA_ASSEMBLY_IEN_ATOMIC = 0,  /* scatter-add through IEN
+                                   (matrix_free_gpu/src/mult.cu:454-461) */
+  IGA_ASSEMBLY_INVLM_GATHER = 1 /* atomics-free gather over the inverse LM
+                                   (matrix_free_gpu_dbd/src/mult.cu:341-450);
+                                   bitwise reproducible */
+} iga_assembly;
+
+typedef struct {
+  int device;       /* HIP device ordinal */
+  int want_f64;     /* keep FP64 G_q (default 1) */
+  int want_f32;     /* keep FP32 G_q (default 1) */
+  int tile_x_f64, tile_y_f64; /* element tile of the FP64 kernels, 0 = default */
+  int tile_x_f32, tile_y_f32; /* element tile of the FP32 kernels, 0 = default */
+} iga_op_opts;
+
+typedef struct {
+  int nfunc, nfx, nfy, nelem, nqp, nloc, ndir;
+  int rational;               /* 0 unit weights, 1 tensor-product weights */
+  int tile_x_f64, tile_y_f64, tile_x_f32, tile_y_f32;
+  int64_t device_bytes;       /* device memory held by the operator */
+  double setup_ms;            /* wall time of iga_op_create */
+} iga_op_info;
+
+void iga_op_opts_default(iga_op_opts *o);
+int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o, iga_op **out);
+int iga_op_destroy(iga_op *op);
+int iga_op_get_info(const iga_op *op, iga_op_info *info);
+
+/* y = A x on the device, y overwritten (the caller's VecSet(y,0) of
+ * MyMatMult, matrix_free_gpu/matrixfreegpu.cu:52-66, is folded in).
+ * x, y: nfunc entries of the dtype, device memory, not aliased.
+ * Asynchronous on `stream`; allocates nothing and never synchronises, so it
+ * can be captured in a HIP graph.  Calls on one op must not overlap when
+ * they use the gather assembly (the op owns its halo buffer). */
+int iga_op_apply(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb,
+                 const void *x, void *y, void *stream);
+
+/* Kernel timing of the element-action kernel (k_apply) with HIP events on
+ * the stream it runs on.  No counterpart in the reference, which times whole
+ * solves only (PetscTime, matrix_free_gpu/matrixfreegpu.cu:299-306).
+ * enable = 1 starts recording (up to max_calls applies, earlier records
+ * dropped), 0 stops; read synchronises on the recorded events. */
+int iga_op_profile(iga_op *op, int enable, int max_calls);
+int iga_op_profile_read(iga_op *op, int *count, double *total_ms, double *min_ms,
+                        double *max_ms);
+
+/* Load vector F (FP64, device) with the reference's F[Dir] = 0
+ * (matrix_free_gpu/src/mult.cu:233-336, matrix_free/src/LocalAssemblyMF.cpp:
+ * 3-29).  kind: 0 f = x(1-x)y(1-y) (the reference's), 1 f = value,
+ * 2 f = value*sin(pi x)sin(pi y). */
+int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream);
+
+/* ---- per-qp conductivity and its gradient (extension; kappa = 1 is the
+ * reference).  kappa: [E][nqp] FP64 device; G_q is rebuilt on the device. */
+int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream);
+/* For y = A(kappa) x and an output cotangent g: grad_x = A^T g
+ * (= K~ P_int g; equals A g on interior rows when g vanishes on Dirichlet
+ * rows) and grad_kappa[e][q] = -w_q det(J_q) grad x_q . grad (P_int g)_q.
+ * Either output may be NULL.  FP64 device buffers. */
+int iga_op_apply_backward(iga_op *op, const double *x, const double *g,
+                          double *grad_x, double *grad_kappa, void *stream);
+
+/* ---- multi right-hand side ----------------------------------------------
+ * Y[:,j] = A X[:,j] for k right-hand sides stored column-major with leading
+ * dimension ld >= nfunc (FP32, device).  The 1D contractions run on MFMA. */
+int iga_op_apply_block(iga_op *op, int k, int ld, const float *X, float *Y,
+                       void *stream);
+
+/* ---- FEM preconditioner (elem2coo -> coo2aij) ---------------------------
+ * Q1 stiffness on the nfx x nfy node grid `nodes` ([nfx*nfy][2] host),
+ * assembled on the device by a deterministic sort-reduce into CSR:
+ * replaces Elem2COOGenerator + AssembleStiffnessKernel + DirichletBCKKernel +
+ * MatSetPreallocationCOO/MatSetValuesCOO (matrix_free_gpu/src/
+ * Elem2COOGenerator.cpp:3-69, assembly.cu:41-99, GlobalAssembly.cu:3-108).
+ * Row/column indices are those of the reference's sorted COO pattern. */
+typedef struct iga_fem iga_fem;
+typedef struct {
+  int nrows;
+  int64_t nnz;
+  const int *rowptr;      /* device [nrows+1] */
+  const int *col;         /* device [nnz] */
+  const double *val;      /* device [nnz] */
+  const int *coo_row;     /* device [nnz] COO row of each entry */
+  const int *elem2coo;    /* device [E_fem][4][4], -1 = dropped */
+  const int *dir2coo;     /* device [ndir] */
+  int ndir;
+} iga_csr_view;
+/* Reference FEM-PC node grid (matrix_free_gpu/matrixfreegpu.cu:104-122) for a
+ * p x q patch with nelem_x x nelem_y IGA elements on [0,Lx]x[0,Ly]; nodes is
+ * [(nelem_x+p)(nelem_y+q)][2] host memory. */
+int iga_fem_reference_nodes(int p, int q, int nelem_x, int nelem_y, double Lx,
+                            double Ly, double *nodes);
+int iga_fem_assemble(int nfx, int nfy, const double *nodes, int device,
+                     iga_fem **out);
+int iga_fem_view(const iga_fem *f, iga_csr_view *v);
+int iga_fem_destroy(iga_fem *f);
+
+/* ---- solver -------------------------------------------------------------
+ * Replaces the KSP + MATSHELL + PCSHELL set-up of
+ * matrix_free_gpu/matrixfreegpu.cu:257-306 (zero initial guess, rtol/abstol
+ * convergence test on the unpreconditioned residual). */
+typedef enum {
+  IGA_PC_NONE = 0,
+  IGA_PC_JACOBI = 1,     /* diagonal of A */
+  IGA_PC_FEM_CG = 2,     /* inner CG + Jacobi on the Q1 FEM matrix
+                            (matrix_free_gpu/matrixfreegpu.cu:18-24,273-289) */
+  IGA_PC_FEM_CHEB = 3    /* fixed-degree Chebyshev-Jacobi on the Q1 matrix */
+} iga_pc_kind;
+
+typedef enum {
+  IGA_SOLVE_PCG_F64 = 0,        /* plain FP64 preconditioned CG */
+  IGA_SOLVE_PCG_F32_REFINE = 1  /* FP32 inner PCG under FP64 defect correction */
+} iga_solve_kind;
+
+typedef struct {
+  iga_solve_kind kind;
+  iga_pc_kind pc;
+  iga_kernel kernel;
+  iga_assembly assembly;
+  double rtol, abstol, divtol;  /* on ||f - A u||_2 (FP64) */
+  int maxit;                    /* total inner iterations cap */
+  double inner_rtol;            /* FP32 inner solve reduction per outer step */
+  int inner_maxit;
+  int pc_inner_maxit;           /* FEM_CG: inner CG iteration cap */
+  double pc_inner_rtol;         /* FEM_CG: inner CG tolerance */
+  int pc_cheb_degree;           /* FEM_CHEB: polynomial degree */
+  int use_graph;                /* capture the iteration body in a HIP graph */
+} iga_solver_opts;
+
+typedef struct {
+  int iterations;        /* total (inner) CG iterations */
+  int outer_iterations;  /* defect-correction steps (1 for plain FP64) */
+  double rel_residual;   /* final ||f - A u|| / ||f|| in FP64 */
+  double solve_ms;       /* wall time of the solve on the stream */
+  int status;
+} iga_solve_stats;
+
+typedef struct iga_solver iga_solver;
+void iga_solver_opts_default(iga_solver_opts *o);
+int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o,
+                      iga_solver **out);
+/* Solve A u = f; f, u FP64 device (u overwritten, zero initial guess). */
+int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
+                     iga_solve_stats *stats);
+int iga_solver_destroy(iga_solver *s);
+
+/* ---- device vector helpers (FP64 / FP32, used by drivers and tests) ----- */
+int iga_dot_f64(const double *a, const double *b, int64_t n, double *out_host,
+                void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IGA_MF_H */

@@ -1,0 +1,391 @@
+"""ctypes front end of the CPU oracle (liboracle.so) and of the reference's own
+CPU units (_ref/libref_mf.so, _ref/libref_gpu.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product package never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class _Mesh(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("p", "q", "nex", "ney", "nq1", "nq2", "nfunc")] + [
+        ("CP", C.c_void_p), ("W", C.c_void_p), ("IEN", C.c_void_p), ("ID", C.c_void_p),
+        ("Dir", C.c_void_p), ("ndir", C.c_int), ("C1", C.c_void_p), ("C2", C.c_void_p),
+        ("h1", C.c_void_p), ("h2", C.c_void_p), ("kappa", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        _lib = C.CDLL(path)
+        L = _lib
+        L.orc_rng_normal.argtypes = [C.c_uint64, C.c_int64, _F64P]
+        L.orc_rng_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, _F64P]
+        L.orc_open_uniform_knots.argtypes = [C.c_int, C.c_int, C.c_double, _F64P]
+        L.orc_bernstein.argtypes = [C.c_int, C.c_double, _F64P, _F64P]
+        L.orc_gauss_legendre01.argtypes = [C.c_int, _F64P, _F64P]
+        L.orc_bspline_basis.argtypes = [C.c_int, _F64P, C.c_int, C.c_double, _F64P,
+                                        C.POINTER(C.c_int)]
+        L.orc_extraction_1d.argtypes = [C.c_int, _F64P, C.c_int, _F64P]
+        L.orc_greville.argtypes = [C.c_int, _F64P, C.c_int, _F64P]
+        L.orc_ien_2d.argtypes = [C.c_int] * 4 + [_I32P]
+        L.orc_id_2d.argtypes = [C.c_int, C.c_int, _I32P, _I32P]
+        L.orc_geometry_unit_square.argtypes = [C.c_int] * 4 + [C.c_double] * 2 + [_F64P, _F64P]
+        L.orc_geometry_quarter_annulus.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [_F64P, _F64P]
+        L.orc_element_qp.argtypes = [C.c_int] * 4 + [_F64P, _F64P, C.c_double, C.c_double,
+                                                     _F64P, C.c_void_p, _F64P, _F64P, _F64P,
+                                                     _F64P, C.c_void_p]
+        L.orc_apply_mf.argtypes = [C.POINTER(_Mesh), _F64P, _F64P]
+        L.orc_apply_mf_colored.argtypes = [C.POINTER(_Mesh), _F64P, _F64P, C.c_int]
+        L.orc_apply_sf_ref.argtypes = [C.POINTER(_Mesh), _F64P, _F64P]
+        L.orc_load.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_double, _F64P]
+        L.orc_kappa_grad.argtypes = [C.POINTER(_Mesh), _F64P, _F64P, _F64P]
+        L.orc_element_matrices.argtypes = [C.POINTER(_Mesh), C.c_int64, _F64P]
+        L.orc_element_matrices.restype = C.c_int64
+        L.orc_fem_nodes.argtypes = [C.c_int] * 4 + [C.c_double] * 2 + [_F64P]
+        L.orc_fem_pattern.argtypes = [C.c_int, C.c_int, C.c_int64, _I32P, _I32P]
+        L.orc_fem_pattern.restype = C.c_int64
+        L.orc_fem_elem2coo.argtypes = [C.c_int, C.c_int, _I32P, _I32P, C.c_int64, _I32P]
+        L.orc_fem_dir2coo.argtypes = [C.c_int, C.c_int, _I32P, C.c_int64, _I32P]
+        L.orc_fem_values.argtypes = [C.c_int, C.c_int, _F64P, _I32P, _I32P, C.c_int,
+                                     C.c_int64, _F64P]
+        L.orc_invlm.argtypes = [C.c_int, C.c_int, C.c_int, _I32P, _I32P, _I32P, _I32P, _I32P]
+        L.orc_invlm.restype = C.c_int64
+    return _lib
+
+
+# --------------------------------------------------------------------------- inputs
+def rng_normal(seed, n):
+    out = np.empty(n, np.float64)
+    lib().orc_rng_normal(seed, n, out)
+    return out
+
+
+def rng_uniform(seed, n, lo, hi):
+    out = np.empty(n, np.float64)
+    lib().orc_rng_uniform(seed, n, lo, hi, out)
+    return out
+
+
+def open_knots(p, nelem, L=1.0):
+    S = np.empty(nelem + 2 * p + 1)
+    lib().orc_open_uniform_knots(p, nelem, L, S)
+    return S
+
+
+def bernstein(p, xi):
+    B = np.empty(p + 1); dB = np.empty(p + 1)
+    lib().orc_bernstein(p, xi, B, dB)
+    return B, dB
+
+
+def gauss01(n):
+    x = np.empty(n); w = np.empty(n)
+    if lib().orc_gauss_legendre01(n, x, w) != 0:
+        raise ValueError("n out of range")
+    return x, w
+
+
+def bspline_basis(p, S, u):
+    N = np.empty(p + 1)
+    span = C.c_int()
+    lib().orc_bspline_basis(p, np.ascontiguousarray(S, np.float64), len(S), u, N,
+                            C.byref(span))
+    return N, span.value
+
+
+def extraction(p, S):
+    S = np.ascontiguousarray(S, np.float64)
+    nelem = len(S) - 2 * p - 1
+    Cm = np.empty(nelem * (p + 1) ** 2)
+    lib().orc_extraction_1d(p, S, len(S), Cm)
+    return Cm.reshape(nelem, p + 1, p + 1)
+
+
+def greville(p, S):
+    S = np.ascontiguousarray(S, np.float64)
+    g = np.empty(len(S) - p - 1)
+    lib().orc_greville(p, S, len(S), g)
+    return g
+
+
+class Mesh:
+    """Arrays of one tensor-product patch in the reference's layout."""
+
+    def __init__(self, p, q, nex, ney, CP, W, IEN, ID, Dir, C1, C2, h1, h2, nq1=None,
+                 nq2=None, kappa=None):
+        self.p, self.q, self.nex, self.ney = p, q, nex, ney
+        self.nq1 = nq1 or p + 1
+        self.nq2 = nq2 or q + 1
+        self.nfx, self.nfy = nex + p, ney + q
+        self.nfunc = self.nfx * self.nfy
+        self.CP = np.ascontiguousarray(CP, np.float64).reshape(-1)
+        self.W = None if W is None else np.ascontiguousarray(W, np.float64)
+        self.IEN = np.ascontiguousarray(IEN, np.int32).reshape(-1)
+        self.ID = np.ascontiguousarray(ID, np.int32)
+        self.Dir = np.ascontiguousarray(Dir, np.int32)
+        self.C1 = np.ascontiguousarray(C1, np.float64).reshape(-1)
+        self.C2 = np.ascontiguousarray(C2, np.float64).reshape(-1)
+        self.h1 = np.ascontiguousarray(h1, np.float64)
+        self.h2 = np.ascontiguousarray(h2, np.float64)
+        self.kappa = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+
+    @property
+    def nelem(self):
+        return self.nex * self.ney
+
+    @property
+    def nloc(self):
+        return (self.p + 1) * (self.q + 1)
+
+    def with_kappa(self, kappa):
+        m = Mesh(self.p, self.q, self.nex, self.ney, self.CP, self.W, self.IEN, self.ID,
+                 self.Dir, self.C1, self.C2, self.h1, self.h2, self.nq1, self.nq2, kappa)
+        return m
+
+    def struct(self):
+        def ptr(a):
+            return None if a is None else a.ctypes.data
+        s = _Mesh(self.p, self.q, self.nex, self.ney, self.nq1, self.nq2, self.nfunc,
+                  ptr(self.CP), ptr(self.W), ptr(self.IEN), ptr(self.ID), ptr(self.Dir),
+                  len(self.Dir), ptr(self.C1), ptr(self.C2), ptr(self.h1), ptr(self.h2),
+                  ptr(self.kappa))
+        s._keep = self  # keep arrays alive
+        return s
+
+
+def topology(p, q, nex, ney):
+    nfx, nfy = nex + p, ney + q
+    IEN = np.empty(nex * ney * (p + 1) * (q + 1), np.int32)
+    lib().orc_ien_2d(p, q, nex, ney, IEN)
+    ID = np.empty(nfx * nfy, np.int32)
+    Dir = np.empty(nfx * nfy, np.int32)
+    nd = lib().orc_id_2d(nfx, nfy, ID, Dir)
+    return IEN, ID, Dir[:nd].copy()
+
+
+def mesh_unit_square(p, nex, ney=None, L=1.0, q=None):
+    q = p if q is None else q
+    ney = nex if ney is None else ney
+    nfx, nfy = nex + p, ney + q
+    CP = np.empty(2 * nfx * nfy)
+    W = np.empty(nfx * nfy)
+    lib().orc_geometry_unit_square(p, q, nex, ney, L, L, CP, W)
+    IEN, ID, Dir = topology(p, q, nex, ney)
+    S, T = open_knots(p, nex, L), open_knots(q, ney, L)
+    C1, C2 = extraction(p, S), extraction(q, T)
+    h1 = np.full(nex, L / nex)
+    h2 = np.full(ney, L / ney)
+    return Mesh(p, q, nex, ney, CP, None, IEN, ID, Dir, C1, C2, h1, h2)
+
+
+def mesh_quarter_annulus(p, nex, ney=None, r0=1.0, r1=2.0):
+    ney = nex if ney is None else ney
+    nfx, nfy = nex + p, ney + p
+    CP = np.empty(2 * nfx * nfy)
+    W = np.empty(nfx * nfy)
+    rc = lib().orc_geometry_quarter_annulus(p, nex, ney, r0, r1, CP, W)
+    if rc != 0:
+        raise RuntimeError("annulus construction failed: %d" % rc)
+    IEN, ID, Dir = topology(p, p, nex, ney)
+    S, T = open_knots(p, nex), open_knots(p, ney)
+    C1, C2 = extraction(p, S), extraction(p, T)
+    return Mesh(p, p, nex, ney, CP, W, IEN, ID, Dir, C1, C2, np.full(nex, 1.0 / nex),
+                np.full(ney, 1.0 / ney))
+
+
+# --------------------------------------------------------------------------- operators
+def element_qp(m, ex, ey):
+    nloc = m.nloc
+    nqp = m.nq1 * m.nq2
+    e = ey * m.nex + ex
+    ien = m.IEN[e * nloc:(e + 1) * nloc]
+    eCP = np.ascontiguousarray(m.CP.reshape(-1, 2)[ien].reshape(-1))
+    eW = None if m.W is None else np.ascontiguousarray(m.W[ien])
+    R = np.empty(nqp * nloc); dx = np.empty(nqp * nloc); dy = np.empty(nqp * nloc)
+    JxW = np.empty(nqp); xq = np.empty(2 * nqp)
+    C1e = np.ascontiguousarray(m.C1.reshape(m.nex, -1)[ex])
+    C2e = np.ascontiguousarray(m.C2.reshape(m.ney, -1)[ey])
+    lib().orc_element_qp(m.p, m.q, m.nq1, m.nq2, C1e, C2e, float(m.h1[ex]), float(m.h2[ey]),
+                         eCP, None if eW is None else eW.ctypes.data, R, dx, dy, JxW,
+                         xq.ctypes.data)
+    return (R.reshape(nqp, nloc), dx.reshape(nqp, nloc), dy.reshape(nqp, nloc), JxW,
+            xq.reshape(nqp, 2))
+
+
+def apply_mf(m, x):
+    y = np.empty(m.nfunc)
+    s = m.struct()
+    lib().orc_apply_mf(C.byref(s), np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def apply_mf_colored(m, x, nthreads=0):
+    y = np.empty(m.nfunc)
+    s = m.struct()
+    used = lib().orc_apply_mf_colored(C.byref(s), np.ascontiguousarray(x, np.float64), y,
+                                      nthreads)
+    return y, used
+
+
+def apply_sf_ref(m, x):
+    y = np.empty(m.nfunc)
+    s = m.struct()
+    if lib().orc_apply_sf_ref(C.byref(s), np.ascontiguousarray(x, np.float64), y) != 0:
+        raise ValueError("mult_sf semantics need p == q and nq1 == nq2")
+    return y
+
+
+LOAD_REF_POLY, LOAD_CONST, LOAD_SINSIN = 0, 1, 2
+
+
+def load(m, kind=LOAD_REF_POLY, value=0.0):
+    F = np.empty(m.nfunc)
+    s = m.struct()
+    lib().orc_load(C.byref(s), kind, value, F)
+    return F
+
+
+def kappa_grad(m, u, g):
+    out = np.empty(m.nelem * m.nq1 * m.nq2)
+    s = m.struct()
+    lib().orc_kappa_grad(C.byref(s), np.ascontiguousarray(u, np.float64),
+                         np.ascontiguousarray(g, np.float64), out)
+    return out
+
+
+def element_matrices(m):
+    n = m.nelem * m.nloc * m.nloc
+    Ke = np.empty(n)
+    s = m.struct()
+    lib().orc_element_matrices(C.byref(s), n, Ke)
+    return Ke.reshape(m.nelem, m.nloc, m.nloc)
+
+
+def assembled_matrix(m):
+    """Assembled A = P_int * K~ (rows of Dirichlet functions empty) as scipy CSR."""
+    import scipy.sparse as sp
+    Ke = element_matrices(m)
+    ien = m.IEN.reshape(m.nelem, m.nloc)
+    rows = np.repeat(ien, m.nloc, axis=1).reshape(-1)
+    cols = np.tile(ien, (1, m.nloc)).reshape(-1)
+    rid = m.ID[rows]
+    keep = rid >= 0
+    A = sp.coo_matrix((Ke.reshape(-1)[keep], (rid[keep], cols[keep])),
+                      shape=(m.nfunc, m.nfunc)).tocsr()
+    A.sum_duplicates()
+    return A
+
+
+# --------------------------------------------------------------------------- FEM-PC
+def fem_nodes(p, q, nex, ney, Lx=1.0, Ly=1.0):
+    nfx, nfy = nex + p, ney + q
+    CPf = np.empty(2 * nfx * nfy)
+    if lib().orc_fem_nodes(p, q, nex, ney, Lx, Ly, CPf) < 0:
+        raise ValueError("FEM preconditioner needs p, q >= 2")
+    return CPf
+
+
+def fem_assemble(nfx, nfy, CPf):
+    cap = (3 * nfx) * (3 * nfy) + nfx * nfy
+    rows = np.empty(cap, np.int32); cols = np.empty(cap, np.int32)
+    nnz = lib().orc_fem_pattern(nfx, nfy, cap, rows, cols)
+    rows, cols = rows[:nnz].copy(), cols[:nnz].copy()
+    e2c = np.empty((nfx - 1) * (nfy - 1) * 16, np.int32)
+    lib().orc_fem_elem2coo(nfx, nfy, rows, cols, nnz, e2c)
+    d2c = np.empty(nfx * nfy, np.int32)
+    nd = lib().orc_fem_dir2coo(nfx, nfy, rows, nnz, d2c)
+    d2c = d2c[:nd].copy()
+    val = np.empty(nnz)
+    lib().orc_fem_values(nfx, nfy, np.ascontiguousarray(CPf, np.float64), e2c, d2c, nd,
+                         nnz, val)
+    return rows, cols, e2c, d2c, val
+
+
+def invlm(nloc, nelem, nfunc, IEN):
+    en = np.empty(nfunc, np.int32); off = np.empty(nfunc, np.int32)
+    tot = nelem * nloc
+    ei = np.empty(tot, np.int32); bi = np.empty(tot, np.int32)
+    lib().orc_invlm(nloc, nelem, nfunc, np.ascontiguousarray(IEN, np.int32), en, off, ei, bi)
+    return en, off, ei, bi
+
+
+# --------------------------------------------------------------------------- reference units
+_ref_mf = None
+_ref_gpu = None
+
+
+def ref_available():
+    return (os.path.exists(os.path.join(HERE, "_ref", "libref_mf.so")) and
+            os.path.exists(os.path.join(HERE, "_ref", "libref_gpu.so")))
+
+
+def ref_mf():
+    global _ref_mf
+    if _ref_mf is None:
+        _ref_mf = C.CDLL(os.path.join(HERE, "_ref", "libref_mf.so"))
+        L = _ref_mf
+        L.ref_quadrature.argtypes = [C.c_int, _F64P, _F64P]
+        L.ref_bernstein_table.argtypes = [C.c_int, C.c_int, _F64P, _F64P]
+        L.ref_open_knots.argtypes = [C.c_int, C.c_int, C.c_double, _F64P]
+        L.ref_extraction.argtypes = [C.c_int, C.c_int, C.c_double, _F64P]
+        L.ref_bspline_basis.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _F64P]
+        L.ref_greville.argtypes = [C.c_int, C.c_int, C.c_double, _F64P]
+        L.ref_topology.argtypes = [C.c_int] * 4 + [_I32P] * 3
+        L.ref_element_qp.argtypes = [C.c_int, C.c_int, _F64P, _F64P, C.c_double, C.c_double,
+                                     _F64P, _F64P, _F64P, _F64P, _F64P]
+        mesh_args = [C.c_int] * 5 + [_F64P, _I32P, _I32P, _I32P, C.c_int] + [_F64P] * 4
+        L.ref_apply_mf.argtypes = mesh_args + [_F64P, _F64P]
+        L.ref_apply_sf.argtypes = mesh_args + [_F64P, _F64P]
+        L.ref_load.argtypes = mesh_args + [_F64P]
+    return _ref_mf
+
+
+def ref_gpu():
+    global _ref_gpu
+    if _ref_gpu is None:
+        _ref_gpu = C.CDLL(os.path.join(HERE, "_ref", "libref_gpu.so"))
+        L = _ref_gpu
+        L.ref_fem_nodes.argtypes = [C.c_int] * 4 + [C.c_double] * 2 + [_F64P]
+        L.ref_fem_assemble.argtypes = [C.c_int] * 4 + [C.c_double] * 2 + [
+            C.c_long, _I32P, _I32P, _I32P, _I32P, _F64P, C.POINTER(C.c_int)]
+        L.ref_fem_assemble.restype = C.c_long
+        L.ref_invlm.argtypes = [C.c_int] * 3 + [_I32P] * 5
+        L.ref_invlm.restype = C.c_long
+    return _ref_gpu
+
+
+def _mesh_args(m):
+    return (m.p, m.q, m.nex, m.ney, m.nfunc, m.CP, m.IEN, m.ID, m.Dir, len(m.Dir), m.C1,
+            m.C2, m.h1, m.h2)
+
+
+def ref_apply_mf(m, x):
+    y = np.empty(m.nfunc)
+    ref_mf().ref_apply_mf(*_mesh_args(m), np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def ref_apply_sf(m, x):
+    y = np.empty(m.nfunc)
+    ref_mf().ref_apply_sf(*_mesh_args(m), np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def ref_load(m):
+    F = np.empty(m.nfunc)
+    ref_mf().ref_load(*_mesh_args(m), F)
+    return F

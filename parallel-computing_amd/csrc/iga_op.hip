@@ -1,0 +1,590 @@
+// iga_op.hip -- operator set-up and the C entry points of the apply.
+//
+// Replaces GlobalAssemblyMF (.cu) / MatrixFreeMatMultCUDA of the reference
+// (matrix_free_gpu/src/GlobalAssemblyMF.cu:141-242, matrix_free_gpu/src/
+// mult.cu:338-558): the mesh is validated and uploaded once, the 1D basis
+// tables (C_e . Bernstein, with tensor-product NURBS weights folded in) and
+// the per-quadrature-point factors G_q are built once, and each apply is two
+// asynchronous launches with no allocation and no synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "op_internal.hpp"
+
+namespace igak {
+const Ops *ops_p1();
+const Ops *ops_p2();
+const Ops *ops_p3();
+const Ops *ops_p4();
+const Ops *ops_p5();
+const Ops *ops_p6();
+const Ops *ops_for_degree(int p) {
+  switch (p) {
+    case 1: return ops_p1();
+    case 2: return ops_p2();
+    case 3: return ops_p3();
+    case 4: return ops_p4();
+    case 5: return ops_p5();
+    case 6: return ops_p6();
+  }
+  return nullptr;
+}
+}  // namespace igak
+
+namespace {
+
+// Gauss-Legendre on [0,1] by Newton iteration on P_n (weights sum to one).
+void gauss01(int n, double *x, double *w) {
+  const double pi = 3.14159265358979323846264338327950288;
+  for (int i = 0; i < n; ++i) {
+    double t = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        const double pk = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = pk;
+      }
+      if (n == 1) { p0 = 1.0; p1 = t; }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+      const double dt = p1 / dp;
+      t -= dt;
+      if (std::fabs(dt) < 1e-17) break;
+    }
+    double p0 = 1.0, p1 = t;
+    for (int k = 2; k <= n; ++k) {
+      const double pk = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = pk;
+    }
+    if (n == 1) { p0 = 1.0; p1 = t; }
+    dp = n * (t * p1 - p0) / (t * t - 1.0);
+    x[n - 1 - i] = 0.5 * t + 0.5;
+    w[n - 1 - i] = 1.0 / ((1.0 - t * t) * dp * dp);
+  }
+}
+
+// Bernstein values and derivatives of degree p at xi (de Casteljau-style
+// recurrence on the degree).
+void bernstein(int p, double xi, double *B, double *dB) {
+  double lo[16] = {1.0};
+  for (int d = 1; d < p; ++d) {
+    double nx[16];
+    for (int k = 0; k <= d; ++k)
+      nx[k] = (k < d ? (1.0 - xi) * lo[k] : 0.0) + (k > 0 ? xi * lo[k - 1] : 0.0);
+    for (int k = 0; k <= d; ++k) lo[k] = nx[k];
+  }
+  // lo = degree p-1 basis (p >= 1)
+  for (int k = 0; k <= p; ++k) {
+    const double a = (k < p) ? lo[k] : 0.0, b = (k > 0) ? lo[k - 1] : 0.0;
+    B[k] = (1.0 - xi) * a + xi * b;
+    dB[k] = p * (b - a);
+  }
+}
+
+// 1D tables of one direction: for each element, N[i][a] and dN/dxi[i][a] at
+// the Gauss points (reference coordinate on [0,1]), rational when w1 != null
+// (w1 = the direction's factor of tensor-product weights).
+std::vector<double> basis_tables(int p, int nq, int nelem, const double *C,
+                                 const double *w1, const double *gx) {
+  const int nb = p + 1, tab = nq * nb;
+  std::vector<double> out((size_t)nelem * 2 * tab);
+  double B[16], dB[16];
+  for (int e = 0; e < nelem; ++e) {
+    const double *Ce = C + (size_t)e * nb * nb;
+    double *o = &out[(size_t)e * 2 * tab];
+    for (int i = 0; i < nq; ++i) {
+      bernstein(p, gx[i], B, dB);
+      double N[16], D[16], W = 0.0, dW = 0.0;
+      for (int a = 0; a < nb; ++a) {
+        double s = 0.0, d = 0.0;
+        for (int k = 0; k < nb; ++k) {
+          s += Ce[a * nb + k] * B[k];
+          d += Ce[a * nb + k] * dB[k];
+        }
+        const double wa = w1 ? w1[e + a] : 1.0;
+        N[a] = wa * s;
+        D[a] = wa * d;
+        W += N[a];
+        dW += D[a];
+      }
+      for (int a = 0; a < nb; ++a) {
+        double R = N[a], dR = D[a];
+        if (w1) {
+          R = N[a] / W;
+          dR = (D[a] - dW * R) / W;
+        }
+        o[i * nb + a] = R;
+        o[tab + i * nb + a] = dR;
+      }
+    }
+  }
+  return out;
+}
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+namespace igai {
+
+int set_device(int dev) {
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) return IGA_ERR_HIP;
+  if (cur != dev && hipSetDevice(dev) != hipSuccess) return IGA_ERR_HIP;
+  return IGA_OK;
+}
+
+int dev_alloc(iga_op *op, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return e == hipErrorOutOfMemory ? IGA_ERR_OUT_OF_MEMORY : IGA_ERR_HIP;
+  }
+  op->allocs.push_back(*p);
+  op->bytes += (int64_t)bytes;
+  return IGA_OK;
+}
+
+int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
+              int mask_in, int mask_out, hipStream_t s, const int *skip) {
+  const igak::Layout &L = f64 ? op->L64 : op->L32;
+  igak::ApplyArgs a;
+  a.x = x;
+  a.y = y;
+  a.G = f64 ? (const void *)op->G64 : (const void *)op->G32;
+  a.tabx = f64 ? (const void *)op->tabx64 : (const void *)op->tabx32;
+  a.taby = f64 ? (const void *)op->taby64 : (const void *)op->taby32;
+  a.dirmask = op->dirmask;
+  a.halo = f64 ? (void *)op->halo64 : (void *)op->halo32;
+  a.nex = op->nex;
+  a.ney = op->ney;
+  a.nfx = op->nfx;
+  a.nfy = op->nfy;
+  a.ntx = L.ntx;
+  a.nty = L.nty;
+  a.mode = gather ? 1 : 0;
+  a.mask_in = mask_in;
+  a.mask_out = mask_out;
+  a.skip = skip;
+  if (!a.G) return IGA_ERR_UNSUPPORTED;
+  const int ntiles = L.ntx * L.nty;
+  const int tile = f64 ? op->tile64 : op->tile32;
+  if (!gather) {
+    if (f64)
+      IGA_HIP_TRY(op->ops->seam64((double *)y, nullptr, op->dirmask, op->nex, op->ney, op->nfx,
+                                  op->nfy, L, 0, mask_out, skip, s));
+    else
+      IGA_HIP_TRY(op->ops->seam32((float *)y, nullptr, op->dirmask, op->nex, op->ney, op->nfx,
+                                  op->nfy, L, 0, mask_out, skip, s));
+  }
+  const bool prof = op->prof_on && op->prof_used < op->prof_cap;
+  if (prof) IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used], s));
+  IGA_HIP_TRY(op->ops->apply(a, f64, ebe, tile, ntiles, s));
+  if (prof) {
+    IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used + 1], s));
+    ++op->prof_used;
+  }
+  if (gather) {
+    if (f64)
+      IGA_HIP_TRY(op->ops->seam64((double *)y, op->halo64, op->dirmask, op->nex, op->ney,
+                                  op->nfx, op->nfy, L, 1, mask_out, skip, s));
+    else
+      IGA_HIP_TRY(op->ops->seam32((float *)y, op->halo32, op->dirmask, op->nex, op->ney,
+                                  op->nfx, op->nfy, L, 1, mask_out, skip, s));
+  }
+  return IGA_OK;
+}
+
+}  // namespace igai
+
+extern "C" {
+
+const char *iga_status_string(int s) {
+  switch (s) {
+    case IGA_OK: return "ok";
+    case IGA_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case IGA_ERR_INVALID_DEGREE: return "invalid degree (p, q must lie in [1,8])";
+    case IGA_ERR_INVALID_QUADRATURE: return "invalid quadrature (1..10 points per direction)";
+    case IGA_ERR_TOO_FEW_ELEMENTS: return "too few elements (need nelem >= p + 2)";
+    case IGA_ERR_BAD_MESH: return "inconsistent mesh (IEN / ID / Dir / extraction)";
+    case IGA_ERR_UNSUPPORTED: return "no kernel for this configuration";
+    case IGA_ERR_HIP: return "HIP runtime error";
+    case IGA_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case IGA_ERR_NOT_FINITE: return "non-finite value in an input vector";
+    case IGA_ERR_DIVERGED: return "solver diverged";
+    case IGA_ERR_NOT_CONVERGED: return "solver reached its iteration limit";
+    case IGA_ERR_SIZE_LIMIT: return "request exceeds a fixed capacity";
+  }
+  return "unknown status";
+}
+
+void iga_op_opts_default(iga_op_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->want_f64 = 1;
+  o->want_f32 = 1;
+}
+
+int iga_op_destroy(iga_op *op) {
+  if (!op) return IGA_OK;
+  int cur = -1;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(op->device);
+  for (void *p : op->allocs) (void)hipFree(p);
+  for (hipEvent_t e : op->prof_ev) (void)hipEventDestroy(e);
+  if (cur >= 0) (void)hipSetDevice(cur);
+  delete op;
+  return IGA_OK;
+}
+
+int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out) {
+  if (!m || !out) return IGA_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  iga_op_opts o;
+  if (o_in) o = *o_in; else iga_op_opts_default(&o);
+  const int p = m->p;
+  if (p < 1 || p > 8 || m->q < 1 || m->q > 8) return IGA_ERR_INVALID_DEGREE;
+  if (m->nqp_x < 1 || m->nqp_x > 10 || m->nqp_y < 1 || m->nqp_y > 10)
+    return IGA_ERR_INVALID_QUADRATURE;
+  if (m->nelem_x < p + 2 || m->nelem_y < m->q + 2) return IGA_ERR_TOO_FEW_ELEMENTS;
+  if (!m->cp || !m->id || !m->extraction_x || !m->extraction_y || !m->elem_size_x ||
+      !m->elem_size_y || (m->ndir > 0 && !m->dir) || m->ndir < 0)
+    return IGA_ERR_INVALID_ARGUMENT;
+  if (m->q != p || m->nqp_x != p + 1 || m->nqp_y != p + 1) return IGA_ERR_UNSUPPORTED;
+  const igak::Ops *ops = igak::ops_for_degree(p);
+  if (!ops) return IGA_ERR_UNSUPPORTED;
+
+  const int nex = m->nelem_x, ney = m->nelem_y, nfx = nex + p, nfy = ney + p;
+  const int nfunc = nfx * nfy, nloc = (p + 1) * (p + 1), nq = p + 1;
+  // connectivity: the tensor-product IEN of matrix_free/src/IENGenerator.cpp:68-103
+  if (m->ien) {
+    for (int ey = 0; ey < ney; ++ey)
+      for (int ex = 0; ex < nex; ++ex) {
+        const int *row = m->ien + ((size_t)ey * nex + ex) * nloc;
+        for (int b = 0; b <= p; ++b)
+          for (int a = 0; a <= p; ++a) {
+            const int v = row[b * (p + 1) + a];
+            if (v < 0 || v >= nfunc) return IGA_ERR_BAD_MESH;
+            if (v != (ey + b) * nfx + ex + a) return IGA_ERR_UNSUPPORTED;
+          }
+      }
+  }
+  // equation numbers: the serial partition maps survivors to themselves
+  std::vector<uint8_t> mask(nfunc, 0);
+  for (int A = 0; A < nfunc; ++A) {
+    const int v = m->id[A];
+    if (v < -1 || v >= nfunc) return IGA_ERR_BAD_MESH;
+    if (v >= 0 && v != A) return IGA_ERR_UNSUPPORTED;
+    mask[A] = (v < 0);
+  }
+  {
+    std::vector<uint8_t> seen(nfunc, 0);
+    int cnt = 0;
+    for (int d = 0; d < m->ndir; ++d) {
+      const int A = m->dir[d];
+      if (A < 0 || A >= nfunc || !mask[A]) return IGA_ERR_BAD_MESH;
+      if (!seen[A]) { seen[A] = 1; ++cnt; }
+    }
+    int nmask = 0;
+    for (int A = 0; A < nfunc; ++A) nmask += mask[A];
+    if (cnt != nmask) return IGA_ERR_BAD_MESH;
+  }
+  bool ring = true;
+  for (int j = 0; j < nfy && ring; ++j)
+    for (int i = 0; i < nfx; ++i) {
+      const bool r = i == 0 || j == 0 || i == nfx - 1 || j == nfy - 1;
+      if (r != (mask[(size_t)j * nfx + i] != 0)) { ring = false; break; }
+    }
+  for (int e = 0; e < nex; ++e)
+    if (!(m->elem_size_x[e] > 0.0)) return IGA_ERR_BAD_MESH;
+  for (int e = 0; e < ney; ++e)
+    if (!(m->elem_size_y[e] > 0.0)) return IGA_ERR_BAD_MESH;
+  for (size_t k = 0; k < (size_t)2 * nfunc; ++k)
+    if (!std::isfinite(m->cp[k])) return IGA_ERR_BAD_MESH;
+  // weights: unit, or tensor-product w[j*nfx+i] = wx[i]*wy[j]
+  std::vector<double> wx, wy;
+  int rational = 0;
+  if (m->weights) {
+    bool unit = true;
+    for (int A = 0; A < nfunc; ++A) {
+      if (!(m->weights[A] > 0.0)) return IGA_ERR_BAD_MESH;
+      if (m->weights[A] != 1.0) unit = false;
+    }
+    if (!unit) {
+      wx.resize(nfx);
+      wy.resize(nfy);
+      const double w00 = m->weights[0];
+      for (int i = 0; i < nfx; ++i) wx[i] = m->weights[i];
+      for (int j = 0; j < nfy; ++j) wy[j] = m->weights[(size_t)j * nfx] / w00;
+      for (int j = 0; j < nfy; ++j)
+        for (int i = 0; i < nfx; ++i) {
+          const double w = m->weights[(size_t)j * nfx + i];
+          if (std::fabs(w - wx[i] * wy[j]) > 1e-13 * w) return IGA_ERR_UNSUPPORTED;
+        }
+      rational = 1;
+    }
+  }
+
+  iga_op *op = new (std::nothrow) iga_op;
+  if (!op) return IGA_ERR_OUT_OF_MEMORY;
+  op->device = o.device;
+  op->p = p;
+  op->nq = nq;
+  op->nex = nex;
+  op->ney = ney;
+  op->nfx = nfx;
+  op->nfy = nfy;
+  op->nfunc = nfunc;
+  op->nelem = nex * ney;
+  op->nqp = nq * nq;
+  op->nloc = nloc;
+  op->ndir = m->ndir;
+  op->rational = rational;
+  op->ring = ring;
+  op->ops = ops;
+  op->want64 = o.want_f64 != 0;
+  op->want32 = o.want_f32 != 0;
+  auto fail = [&](int rc) {
+    iga_op_destroy(op);
+    return rc;
+  };
+  int rc = igai::set_device(op->device);
+  if (rc) return fail(rc);
+
+  // tiles
+  auto pick = [&](int tx, int ty, int dflt, int &id, igak::Layout &L) {
+    id = (tx || ty) ? igak::tile_id(tx, ty) : dflt;
+    if (id < 0) return false;
+    igak::tile_shape(id, L.tx, L.ty);
+    if (L.tx < p + 1 || L.ty < p + 1) return false;
+    L.ntx = ceil_div(nex, L.tx);
+    L.nty = ceil_div(ney, L.ty);
+    return true;
+  };
+  if (!pick(o.tile_x_f64, o.tile_y_f64, igak::TILE_32x16, op->tile64, op->L64) ||
+      !pick(o.tile_x_f32, o.tile_y_f32, igak::TILE_32x16, op->tile32, op->L32))
+    return fail(IGA_ERR_UNSUPPORTED);
+
+  // tables and Gauss rule (host FP64)
+  std::vector<double> gx(nq), gw(nq);
+  gauss01(nq, gx.data(), gw.data());
+  op->h_wq = gw;
+  op->h_tabx = basis_tables(p, nq, nex, m->extraction_x, rational ? wx.data() : nullptr,
+                            gx.data());
+  op->h_taby = basis_tables(p, nq, ney, m->extraction_y, rational ? wy.data() : nullptr,
+                            gx.data());
+  const size_t tabn_x = op->h_tabx.size(), tabn_y = op->h_taby.size();
+  std::vector<float> tx32(op->h_tabx.begin(), op->h_tabx.end());
+  std::vector<float> ty32(op->h_taby.begin(), op->h_taby.end());
+
+  hipStream_t s = nullptr;
+#define ALLOC(ptr, bytes)                                          \
+  do {                                                             \
+    int r_ = igai::dev_alloc(op, (void **)&(ptr), (bytes));        \
+    if (r_) return fail(r_);                                       \
+  } while (0)
+#define H2D(dst, src, bytes)                                                   \
+  do {                                                                         \
+    if (hipMemcpy((dst), (src), (bytes), hipMemcpyHostToDevice) != hipSuccess) \
+      return fail(IGA_ERR_HIP);                                                \
+  } while (0)
+  ALLOC(op->tabx64, tabn_x * 8);
+  ALLOC(op->taby64, tabn_y * 8);
+  ALLOC(op->tabx32, tabn_x * 4);
+  ALLOC(op->taby32, tabn_y * 4);
+  ALLOC(op->cp, (size_t)2 * nfunc * 8);
+  ALLOC(op->wq, (size_t)nq * 8);
+  H2D(op->tabx64, op->h_tabx.data(), tabn_x * 8);
+  H2D(op->taby64, op->h_taby.data(), tabn_y * 8);
+  H2D(op->tabx32, tx32.data(), tabn_x * 4);
+  H2D(op->taby32, ty32.data(), tabn_y * 4);
+  H2D(op->cp, m->cp, (size_t)2 * nfunc * 8);
+  H2D(op->wq, gw.data(), (size_t)nq * 8);
+  if (!ring) {
+    ALLOC(op->dirmask, (size_t)nfunc);
+    H2D(op->dirmask, mask.data(), (size_t)nfunc);
+  }
+  if (m->kappa) {
+    ALLOC(op->kappa, (size_t)op->nelem * op->nqp * 8);
+    H2D(op->kappa, m->kappa, (size_t)op->nelem * op->nqp * 8);
+  }
+  const int nqp = op->nqp;
+  // G_q: FP64 always (the backward pass and the refinement residual need it)
+  {
+    const igak::Layout &L = op->L64;
+    const size_t n = (size_t)L.ntx * L.nty * 3 * nqp * L.tx * L.ty;
+    ALLOC(op->G64, n * 8);
+    if (hipSuccess != ops->geom64(op->G64, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
+                                  nex, ney, nfx, L, s))
+      return fail(IGA_ERR_HIP);
+    if (op->kappa) {
+      ALLOC(op->G0, n * 8);
+      if (hipSuccess != ops->geom64(op->G0, op->tabx64, op->taby64, op->cp, op->wq, nullptr,
+                                    nex, ney, nfx, L, s))
+        return fail(IGA_ERR_HIP);
+    } else {
+      op->G0 = op->G64;
+    }
+    ALLOC(op->halo64, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 8);
+  }
+  if (op->want32) {
+    const igak::Layout &L = op->L32;
+    const size_t n = (size_t)L.ntx * L.nty * 3 * nqp * L.tx * L.ty;
+    ALLOC(op->G32, n * 4);
+    if (hipSuccess != ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
+                                  nex, ney, nfx, L, s))
+      return fail(IGA_ERR_HIP);
+    ALLOC(op->halo32, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 4);
+  }
+  if (hipDeviceSynchronize() != hipSuccess) return fail(IGA_ERR_HIP);
+#undef ALLOC
+#undef H2D
+  op->setup_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = op;
+  return IGA_OK;
+}
+
+int iga_op_get_info(const iga_op *op, iga_op_info *info) {
+  if (!op || !info) return IGA_ERR_INVALID_ARGUMENT;
+  info->nfunc = op->nfunc;
+  info->nfx = op->nfx;
+  info->nfy = op->nfy;
+  info->nelem = op->nelem;
+  info->nqp = op->nqp;
+  info->nloc = op->nloc;
+  info->ndir = op->ndir;
+  info->rational = op->rational;
+  info->tile_x_f64 = op->L64.tx;
+  info->tile_y_f64 = op->L64.ty;
+  info->tile_x_f32 = op->L32.tx;
+  info->tile_y_f32 = op->L32.ty;
+  info->device_bytes = op->bytes;
+  info->setup_ms = op->setup_ms;
+  return IGA_OK;
+}
+
+int iga_op_apply(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb, const void *x,
+                 void *y, void *stream) {
+  if (!op || !x || !y || x == y) return IGA_ERR_INVALID_ARGUMENT;
+  if ((dt != IGA_F64 && dt != IGA_F32) || (kern != IGA_KERNEL_SF && kern != IGA_KERNEL_EBE) ||
+      (asmb != IGA_ASSEMBLY_IEN_ATOMIC && asmb != IGA_ASSEMBLY_INVLM_GATHER))
+    return IGA_ERR_INVALID_ARGUMENT;
+  if (dt == IGA_F32 && !op->G32) return IGA_ERR_UNSUPPORTED;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  return igai::apply_raw(op, dt == IGA_F64, kern == IGA_KERNEL_EBE,
+                         asmb == IGA_ASSEMBLY_INVLM_GATHER, x, y, 0, 1, (hipStream_t)stream,
+                         nullptr);
+}
+
+int iga_op_profile(iga_op *op, int enable, int max_calls) {
+  if (!op || (enable && max_calls < 1)) return IGA_ERR_INVALID_ARGUMENT;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  if (!enable) {
+    op->prof_on = false;
+    return IGA_OK;
+  }
+  while ((int)op->prof_ev.size() < 2 * max_calls) {
+    hipEvent_t e;
+    IGA_HIP_TRY(hipEventCreate(&e));
+    op->prof_ev.push_back(e);
+  }
+  op->prof_cap = max_calls;
+  op->prof_used = 0;
+  op->prof_on = true;
+  return IGA_OK;
+}
+
+int iga_op_profile_read(iga_op *op, int *count, double *total_ms, double *min_ms,
+                        double *max_ms) {
+  if (!op || !count || !total_ms) return IGA_ERR_INVALID_ARGUMENT;
+  double tot = 0.0, mn = 1e300, mx = 0.0;
+  for (int i = 0; i < op->prof_used; ++i) {
+    IGA_HIP_TRY(hipEventSynchronize(op->prof_ev[2 * i + 1]));
+    float ms = 0.f;
+    IGA_HIP_TRY(hipEventElapsedTime(&ms, op->prof_ev[2 * i], op->prof_ev[2 * i + 1]));
+    tot += ms;
+    mn = ms < mn ? ms : mn;
+    mx = ms > mx ? ms : mx;
+  }
+  *count = op->prof_used;
+  *total_ms = tot;
+  if (min_ms) *min_ms = op->prof_used ? mn : 0.0;
+  if (max_ms) *max_ms = mx;
+  return IGA_OK;
+}
+
+int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream) {
+  if (!op || !F || kind < 0 || kind > 2) return IGA_ERR_INVALID_ARGUMENT;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  if (!op->fJ) {
+    rc = igai::dev_alloc(op, (void **)&op->fJ, (size_t)op->nelem * op->nqp * 8);
+    if (rc) return rc;
+  }
+  IGA_HIP_TRY(op->ops->load(F, op->fJ, op->tabx64, op->taby64, op->cp, op->wq, op->dirmask,
+                            op->nex, op->ney, op->nfx, op->nfy, kind, value,
+                            (hipStream_t)stream));
+  return IGA_OK;
+}
+
+int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream) {
+  if (!op || !kappa) return IGA_ERR_INVALID_ARGUMENT;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const igak::Layout &L = op->L64;
+  const size_t n = (size_t)L.ntx * L.nty * 3 * op->nqp * L.tx * L.ty;
+  if (!op->kappa) {
+    rc = igai::dev_alloc(op, (void **)&op->kappa, (size_t)op->nelem * op->nqp * 8);
+    if (rc) return rc;
+  }
+  if (op->G0 == op->G64) {
+    // keep the geometry-only factors for the backward pass
+    rc = igai::dev_alloc(op, (void **)&op->G0, n * 8);
+    if (rc) return rc;
+    IGA_HIP_TRY(hipMemcpyAsync(op->G0, op->G64, n * 8, hipMemcpyDeviceToDevice, s));
+  }
+  IGA_HIP_TRY(hipMemcpyAsync(op->kappa, kappa, (size_t)op->nelem * op->nqp * 8,
+                             hipMemcpyDeviceToDevice, s));
+  IGA_HIP_TRY(op->ops->geom64(op->G64, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
+                              op->nex, op->ney, op->nfx, op->L64, s));
+  if (op->G32)
+    IGA_HIP_TRY(op->ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
+                                op->nex, op->ney, op->nfx, op->L32, s));
+  ++op->gen;
+  return IGA_OK;
+}
+
+int iga_op_apply_backward(iga_op *op, const double *x, const double *g, double *grad_x,
+                          double *grad_kappa, void *stream) {
+  if (!op || !g || (!grad_x && !grad_kappa) || (grad_kappa && !x)) return IGA_ERR_INVALID_ARGUMENT;
+  if (grad_x == g) return IGA_ERR_INVALID_ARGUMENT;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (grad_x) {
+    // A^T g = K~ P_int g: gather P_int g, keep every row
+    rc = igai::apply_raw(op, true, false, true, g, grad_x, 1, 0, s, nullptr);
+    if (rc) return rc;
+  }
+  if (grad_kappa)
+    IGA_HIP_TRY(op->ops->kappa_grad(grad_kappa, x, g, op->G0, op->tabx64, op->taby64,
+                                    op->dirmask, op->nex, op->ney, op->nfx, op->nfy, op->L64,
+                                    s));
+  return IGA_OK;
+}
+
+}  // extern "C"

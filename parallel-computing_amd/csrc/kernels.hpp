@@ -1,0 +1,602 @@
+// kernels.hpp -- gfx950 kernels of the matrix-free operator, templated on the
+// scalar type T (float / double), the degree P, the Gauss points per
+// direction NQ and the element tile TX x TY.
+//
+// Work decomposition (one workgroup = one tile of TX x TY elements, one
+// element per thread):
+//   1. the tile's footprint of x ((TX+P) x (TY+P) functions) and the 1D basis
+//      tables of its element columns / rows are staged in LDS;
+//   2. each thread streams its element's G_q (3 x NQ^2 values, tile-major SoA,
+//      coalesced) and applies the element operator, sum-factorised (SF) or
+//      dense (EBE);
+//   3. the element vectors are reduced over the footprint in LDS in a fixed
+//      order (P+1 passes, one element-row offset per pass);
+//   4. functions touched by this tile only are stored; functions on a tile
+//      seam are either atomically added (IEN scatter-add, y pre-zeroed on the
+//      seams) or parked in a per-tile halo slot and summed by k_seam in a
+//      fixed tile order (inverse-LM gather, bitwise reproducible).
+// Dirichlet rows are written as zero; x is gathered for every function
+// (matrix_free/src/GlobalAssemblyMF.cpp:166-205).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace igak {
+
+struct Layout {
+  int tx, ty;    // tile shape (elements)
+  int ntx, nty;  // tiles per direction
+};
+
+struct ApplyArgs {
+  const void *x;
+  void *y;
+  const void *G;          // [tile][3][NQ*NQ][TX*TY]
+  const void *tabx;       // [nelem_x][2][NQ][P+1] : N, dN/dxi (reference coords)
+  const void *taby;       // [nelem_y][2][NQ][P+1]
+  const uint8_t *dirmask; // nullptr = Dirichlet on the boundary ring
+  void *halo;             // [tile][(TX+P)*(TY+P)]
+  int nex, ney, nfx, nfy, ntx, nty;
+  int mode;               // 0 scatter-add, 1 gather
+  int mask_in;            // treat x as P_int x
+  int mask_out;           // zero Dirichlet rows of y
+  const int *skip;        // device flag: nonzero = do nothing (converged solve)
+};
+
+__device__ __forceinline__ bool is_dirichlet(const uint8_t *mask, int A, int gx,
+                                             int gy, int nfx, int nfy) {
+  if (mask) return mask[A] != 0;
+  return gx == 0 || gy == 0 || gx == nfx - 1 || gy == nfy - 1;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so that neighbouring tiles (which share footprint
+// rows of x) hit the same L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = bid & 7, li = bid >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + li;
+}
+
+// Tiles touching function g along one direction: elements [g-P, g] clipped.
+template <int P>
+__device__ __forceinline__ void touching_tiles(int g, int nel, int tile, int &lo,
+                                               int &hi) {
+  int e0 = g - P;
+  if (e0 < 0) e0 = 0;
+  int e1 = g;
+  if (e1 > nel - 1) e1 = nel - 1;
+  lo = e0 / tile;
+  hi = e1 / tile;
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T *p, T v) {
+  atomicAdd(p, v);
+}
+
+// --------------------------------------------------------------------------
+// element operators
+// --------------------------------------------------------------------------
+
+// Sum-factorised element action: ye = -sum_q gradR^T G_q gradR xe, with the
+// 1D tables N (values) and D (reference derivatives) of the element column
+// (1) and row (2).  X[b][a], ye[b][a]: b = row offset (eta), a = column (xi).
+template <typename T, int P, int NQ>
+__device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
+                                           const T (&N1)[NQ][P + 1],
+                                           const T (&D1)[NQ][P + 1],
+                                           const T (&N2)[NQ][P + 1],
+                                           const T (&D2)[NQ][P + 1],
+                                           const T (&g11)[NQ * NQ],
+                                           const T (&g12)[NQ * NQ],
+                                           const T (&g22)[NQ * NQ],
+                                           T (&ye)[P + 1][P + 1]) {
+  constexpr int NB = P + 1;
+  T t0[NB][NQ], t1[NB][NQ];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      T s0 = T(0), s1 = T(0);
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        s0 += N1[i][a] * X[b][a];
+        s1 += D1[i][a] * X[b][a];
+      }
+      t0[b][i] = s0;
+      t1[b][i] = s1;
+    }
+  T fx[NQ][NQ], fy[NQ][NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      T ux = T(0), uy = T(0);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        ux += N2[j][b] * t1[b][i];
+        uy += D2[j][b] * t0[b][i];
+      }
+      const int q = j * NQ + i;
+      fx[j][i] = -(g11[q] * ux + g12[q] * uy);
+      fy[j][i] = -(g12[q] * ux + g22[q] * uy);
+    }
+  T s0[NB][NQ], s1[NB][NQ];
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      T a0 = T(0), a1 = T(0);
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        a0 += N2[j][b] * fx[j][i];
+        a1 += D2[j][b] * fy[j][i];
+      }
+      s0[b][i] = a0;
+      s1[b][i] = a1;
+    }
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      T acc = T(0);
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) acc += D1[i][a] * s0[b][i] + N1[i][a] * s1[b][i];
+      ye[b][a] = acc;
+    }
+}
+
+// Element-by-element action: per quadrature point the full reference
+// gradient of every local function (LocalAssemblyMF.cpp:31-67 shape).
+template <typename T, int P, int NQ>
+__device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
+                                            const T (&N1)[NQ][P + 1],
+                                            const T (&D1)[NQ][P + 1],
+                                            const T (&N2)[NQ][P + 1],
+                                            const T (&D2)[NQ][P + 1],
+                                            const T (&g11)[NQ * NQ],
+                                            const T (&g12)[NQ * NQ],
+                                            const T (&g22)[NQ * NQ],
+                                            T (&ye)[P + 1][P + 1]) {
+  constexpr int NB = P + 1;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
+#pragma unroll
+  for (int j = 0; j < NQ; ++j)
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      T ux = T(0), uy = T(0);
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          ux += (D1[i][a] * N2[j][b]) * X[b][a];
+          uy += (N1[i][a] * D2[j][b]) * X[b][a];
+        }
+      const int q = j * NQ + i;
+      const T fx = -(g11[q] * ux + g12[q] * uy);
+      const T fy = -(g12[q] * ux + g22[q] * uy);
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+          ye[b][a] += (D1[i][a] * N2[j][b]) * fx + (N1[i][a] * D2[j][b]) * fy;
+    }
+}
+
+// --------------------------------------------------------------------------
+// the apply kernel
+// --------------------------------------------------------------------------
+template <typename T, int P, int NQ, int TX, int TY, bool EBE>
+__global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
+  constexpr int NB = P + 1, NT = TX * TY, FX = TX + P, FY = TY + P;
+  constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY;
+  __shared__ T xs[NFP];
+  __shared__ T tbx[2 * TAB][TX];
+  __shared__ T tby[2 * TAB][TY];
+  __shared__ T buf[NB][NT];
+
+  if (A.skip && *A.skip) return;
+  const int t = threadIdx.x, lx = t % TX, ly = t / TX;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % A.ntx, ty = tile / A.ntx;
+  const int ex0 = tx * TX, ey0 = ty * TY;
+  const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
+
+  // geometry factors of this element: issued first, consumed last
+  T g11[NQ2], g12[NQ2], g22[NQ2];
+  {
+    const T *G = static_cast<const T *>(A.G) + (size_t)tile * 3 * NQ2 * NT + t;
+#pragma unroll
+    for (int q = 0; q < NQ2; ++q) {
+      g11[q] = G[(size_t)(0 * NQ2 + q) * NT];
+      g12[q] = G[(size_t)(1 * NQ2 + q) * NT];
+      g22[q] = G[(size_t)(2 * NQ2 + q) * NT];
+    }
+  }
+  // x footprint
+  {
+    const T *x = static_cast<const T *>(A.x);
+    for (int k = t; k < NFP; k += NT) {
+      const int fx = k % FX, fy = k / FX;
+      const int gx = ex0 + fx, gy = ey0 + fy;
+      T v = T(0);
+      if (gx < A.nfx && gy < A.nfy) {
+        const int Ag = gy * A.nfx + gx;
+        v = x[Ag];
+        if (A.mask_in && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) v = T(0);
+      }
+      xs[k] = v;
+    }
+  }
+  // 1D tables of the tile's columns and rows
+  {
+    const T *tx_ = static_cast<const T *>(A.tabx);
+    const T *ty_ = static_cast<const T *>(A.taby);
+    for (int k = t; k < 2 * TAB * TX; k += NT) {
+      const int l = k % TX, c = k / TX;
+      int e = ex0 + l;
+      if (e > A.nex - 1) e = A.nex - 1;
+      tbx[c][l] = tx_[(size_t)e * 2 * TAB + c];
+    }
+    for (int k = t; k < 2 * TAB * TY; k += NT) {
+      const int l = k % TY, c = k / TY;
+      int e = ey0 + l;
+      if (e > A.ney - 1) e = A.ney - 1;
+      tby[c][l] = ty_[(size_t)e * 2 * TAB + c];
+    }
+  }
+  __syncthreads();
+
+  T ye[NB][NB];
+  {
+    T X[NB][NB], N1[NQ][NB], D1[NQ][NB], N2[NQ][NB], D2[NQ][NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        N1[i][a] = tbx[i * NB + a][lx];
+        D1[i][a] = tbx[TAB + i * NB + a][lx];
+        N2[i][a] = tby[i * NB + a][ly];
+        D2[i][a] = tby[TAB + i * NB + a][ly];
+      }
+    if (EBE)
+      element_ebe<T, P, NQ>(X, N1, D1, N2, D2, g11, g12, g22, ye);
+    else
+      element_sf<T, P, NQ>(X, N1, D1, N2, D2, g11, g12, g22, ye);
+    if (!valid) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
+    }
+  }
+
+  // fixed-order reduction over the footprint: pass b adds element row offset b
+  constexpr int NOWN = (NFP + NT - 1) / NT;
+  T acc[NOWN];
+#pragma unroll
+  for (int o = 0; o < NOWN; ++o) acc[o] = T(0);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b > 0) __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NB; ++a) buf[a][t] = ye[b][a];
+    __syncthreads();
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int k = t + o * NT;
+      if (k < NFP) {
+        const int fx = k % FX, fy = k / FX;
+        const int el_y = fy - b;
+        if (el_y >= 0 && el_y < TY) {
+#pragma unroll
+          for (int a = 0; a < NB; ++a) {
+            const int el_x = fx - a;
+            if (el_x >= 0 && el_x < TX) acc[o] += buf[a][el_y * TX + el_x];
+          }
+        }
+      }
+    }
+  }
+
+  // output
+  T *y = static_cast<T *>(A.y);
+  T *halo = static_cast<T *>(A.halo);
+#pragma unroll
+  for (int o = 0; o < NOWN; ++o) {
+    const int k = t + o * NT;
+    if (k >= NFP) continue;
+    const int fx = k % FX, fy = k / FX;
+    const int gx = ex0 + fx, gy = ey0 + fy;
+    if (gx >= A.nfx || gy >= A.nfy) continue;
+    int xlo, xhi, ylo, yhi;
+    touching_tiles<P>(gx, A.nex, TX, xlo, xhi);
+    touching_tiles<P>(gy, A.ney, TY, ylo, yhi);
+    const bool shared = (xlo != xhi) || (ylo != yhi);
+    const int Ag = gy * A.nfx + gx;
+    if (A.mask_out && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) {
+      if (!shared || A.mode == 0) y[Ag] = T(0);
+      continue;
+    }
+    if (!shared)
+      y[Ag] = acc[o];
+    else if (A.mode == 0)
+      atomic_add(&y[Ag], acc[o]);
+    else
+      halo[(size_t)tile * NFP + k] = acc[o];
+  }
+}
+
+// Seam functions (touched by more than one tile).  mode 0: zero them before
+// a scatter-add apply; mode 1: sum the halo slots of the touching tiles in
+// (tile row, tile column) order.
+template <typename T, int P>
+__global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex,
+                       int ney, int nfx, int nfy, int TX, int TY, int ntx,
+                       int nty, int mode, int mask_out, const int *skip) {
+  if (skip && *skip) return;
+  const int64_t nv = (int64_t)(ntx - 1) * P * nfy;
+  const int64_t nh = (int64_t)(nty - 1) * P * nfx;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv + nh) return;
+  int gx, gy;
+  if (i < nv) {
+    const int band = (int)(i / ((int64_t)P * nfy));
+    const int r = (int)(i % ((int64_t)P * nfy));
+    gx = (band + 1) * TX + r % P;
+    gy = r / P;
+  } else {
+    const int64_t j = i - nv;
+    const int band = (int)(j / ((int64_t)P * nfx));
+    const int r = (int)(j % ((int64_t)P * nfx));
+    gy = (band + 1) * TY + r / nfx;
+    gx = r % nfx;
+    int xlo, xhi;
+    touching_tiles<P>(gx, nex, TX, xlo, xhi);
+    if (xlo != xhi) return;  // owned by the vertical-band pass
+  }
+  const int Ag = gy * nfx + gx;
+  if (mode == 0 || (mask_out && is_dirichlet(dirmask, Ag, gx, gy, nfx, nfy))) {
+    y[Ag] = T(0);
+    return;
+  }
+  const int FX = TX + P, FY = TY + P;
+  int xlo, xhi, ylo, yhi;
+  touching_tiles<P>(gx, nex, TX, xlo, xhi);
+  touching_tiles<P>(gy, ney, TY, ylo, yhi);
+  T acc = T(0);
+  for (int ty = ylo; ty <= yhi; ++ty)
+    for (int tx = xlo; tx <= xhi; ++tx) {
+      const int k = (gy - ty * TY) * FX + (gx - tx * TX);
+      acc += halo[(size_t)(ty * ntx + tx) * FX * FY + k];
+    }
+  y[Ag] = acc;
+}
+
+// --------------------------------------------------------------------------
+// geometry precompute (FP64 arithmetic, stored as T)
+// --------------------------------------------------------------------------
+// G_q = kappa_q w_q det(J_q) J_q^-1 J_q^-T with J the Jacobian of the map
+// from the element's reference square [0,1]^2 (the reference's parametric
+// Jacobian times diag(hx,hy), matrix_free/src/ElementMF.cpp:159-206).
+// One thread per tile slot; slots past the mesh edge get zeros.
+template <typename T, int P, int NQ>
+__global__ void k_geometry(T *G, const double *tabx, const double *taby,
+                           const double *cp, const double *wq,
+                           const double *kappa, int nex, int ney, int nfx,
+                           int TX, int TY, int ntx) {
+  constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
+  const int NT = TX * TY;
+  const int tile = blockIdx.x;
+  for (int t = threadIdx.x; t < NT; t += blockDim.x) {
+    const int tx = tile % ntx, ty = tile / ntx;
+    const int ex = tx * TX + t % TX, ey = ty * TY + t / TX;
+    T *g = G + (size_t)tile * 3 * NQ2 * NT + t;
+    if (ex >= nex || ey >= ney) {
+      for (int q = 0; q < NQ2; ++q) {
+        g[(size_t)(0 * NQ2 + q) * NT] = T(0);
+        g[(size_t)(1 * NQ2 + q) * NT] = T(0);
+        g[(size_t)(2 * NQ2 + q) * NT] = T(0);
+      }
+      continue;
+    }
+    double X[NB][NB], Y[NB][NB];
+    for (int b = 0; b < NB; ++b)
+      for (int a = 0; a < NB; ++a) {
+        const size_t A = (size_t)(ey + b) * nfx + ex + a;
+        X[b][a] = cp[2 * A];
+        Y[b][a] = cp[2 * A + 1];
+      }
+    const double *t1 = tabx + (size_t)ex * 2 * TAB;
+    const double *t2 = taby + (size_t)ey * 2 * TAB;
+    for (int j = 0; j < NQ; ++j)
+      for (int i = 0; i < NQ; ++i) {
+        double xxi = 0, xeta = 0, yxi = 0, yeta = 0;
+        for (int b = 0; b < NB; ++b)
+          for (int a = 0; a < NB; ++a) {
+            const double dxi = t1[TAB + i * NB + a] * t2[j * NB + b];
+            const double deta = t1[i * NB + a] * t2[TAB + j * NB + b];
+            xxi += X[b][a] * dxi;
+            xeta += X[b][a] * deta;
+            yxi += Y[b][a] * dxi;
+            yeta += Y[b][a] * deta;
+          }
+        const double det = xxi * yeta - xeta * yxi;
+        const int q = j * NQ + i;
+        double s = wq[i] * wq[j] / det;
+        if (kappa) s *= kappa[((size_t)ey * nex + ex) * NQ2 + q];
+        g[(size_t)(0 * NQ2 + q) * NT] = (T)(s * (xeta * xeta + yeta * yeta));
+        g[(size_t)(1 * NQ2 + q) * NT] = (T)(-s * (xxi * xeta + yxi * yeta));
+        g[(size_t)(2 * NQ2 + q) * NT] = (T)(s * (xxi * xxi + yxi * yxi));
+      }
+  }
+}
+
+// --------------------------------------------------------------------------
+// load vector (FP64): fJ[e][q] = f(x_q) w_q det(J_q), then a per-function
+// gather over the adjacent elements (matrix_free_gpu_dbd/src/mult.cu:233-339
+// is the per-function form of the reference's load kernel).
+// --------------------------------------------------------------------------
+template <int P, int NQ>
+__global__ void k_load_qp(double *fJ, const double *tabx, const double *taby,
+                          const double *cp, const double *wq, int nex, int ney,
+                          int nfx, int kind, double value) {
+  constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nex * ney) return;
+  const int ex = e % nex, ey = e / nex;
+  const double *t1 = tabx + (size_t)ex * 2 * TAB;
+  const double *t2 = taby + (size_t)ey * 2 * TAB;
+  const double pi = 3.14159265358979323846264338327950288;
+  for (int j = 0; j < NQ; ++j)
+    for (int i = 0; i < NQ; ++i) {
+      double x = 0, y = 0, xxi = 0, xeta = 0, yxi = 0, yeta = 0;
+      for (int b = 0; b < NB; ++b)
+        for (int a = 0; a < NB; ++a) {
+          const size_t A = (size_t)(ey + b) * nfx + ex + a;
+          const double R = t1[i * NB + a] * t2[j * NB + b];
+          const double dxi = t1[TAB + i * NB + a] * t2[j * NB + b];
+          const double deta = t1[i * NB + a] * t2[TAB + j * NB + b];
+          x += cp[2 * A] * R;
+          y += cp[2 * A + 1] * R;
+          xxi += cp[2 * A] * dxi;
+          xeta += cp[2 * A] * deta;
+          yxi += cp[2 * A + 1] * dxi;
+          yeta += cp[2 * A + 1] * deta;
+        }
+      double f;
+      if (kind == 0) f = x * (1.0 - x) * y * (1.0 - y);
+      else if (kind == 1) f = value;
+      else f = value * sin(pi * x) * sin(pi * y);
+      fJ[(size_t)e * NQ2 + j * NQ + i] = f * wq[i] * wq[j] * (xxi * yeta - xeta * yxi);
+    }
+}
+
+template <int P, int NQ>
+__global__ void k_load_gather(double *F, const double *fJ, const double *tabx,
+                              const double *taby, const uint8_t *dirmask,
+                              int nex, int ney, int nfx, int nfy) {
+  constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nfx * nfy) return;
+  const int gx = A % nfx, gy = A / nfx;
+  if (is_dirichlet(dirmask, A, gx, gy, nfx, nfy)) {
+    F[A] = 0.0;
+    return;
+  }
+  double acc = 0.0;
+  for (int b = P; b >= 0; --b) {
+    const int ey = gy - b;
+    if (ey < 0 || ey >= ney) continue;
+    for (int a = P; a >= 0; --a) {
+      const int ex = gx - a;
+      if (ex < 0 || ex >= nex) continue;
+      const double *t1 = tabx + (size_t)ex * 2 * TAB;
+      const double *t2 = taby + (size_t)ey * 2 * TAB;
+      const double *f = fJ + ((size_t)ey * nex + ex) * NQ2;
+      double s = 0.0;
+      for (int j = 0; j < NQ; ++j)
+        for (int i = 0; i < NQ; ++i) s += t1[i * NB + a] * t2[j * NB + b] * f[j * NQ + i];
+      acc += s;
+    }
+  }
+  F[A] = acc;
+}
+
+// --------------------------------------------------------------------------
+// grad_kappa[e][q] = -grad x_q^T G0_q grad (P_int g)_q   (FP64)
+// --------------------------------------------------------------------------
+template <int P, int NQ>
+__global__ void k_kappa_grad(double *out, const double *x, const double *g,
+                             const double *G0, const double *tabx,
+                             const double *taby, const uint8_t *dirmask,
+                             int nex, int ney, int nfx, int nfy, int TX, int TY,
+                             int ntx) {
+  constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nex * ney) return;
+  const int ex = e % nex, ey = e / nex;
+  const int NT = TX * TY;
+  const int tile = (ey / TY) * ntx + ex / TX, t = (ey % TY) * TX + ex % TX;
+  const double *Gt = G0 + (size_t)tile * 3 * NQ2 * NT + t;
+  const double *t1 = tabx + (size_t)ex * 2 * TAB;
+  const double *t2 = taby + (size_t)ey * 2 * TAB;
+  double xe[NB][NB], ge[NB][NB];
+  for (int b = 0; b < NB; ++b)
+    for (int a = 0; a < NB; ++a) {
+      const int gx = ex + a, gy = ey + b, A = gy * nfx + gx;
+      xe[b][a] = x[A];
+      ge[b][a] = is_dirichlet(dirmask, A, gx, gy, nfx, nfy) ? 0.0 : g[A];
+    }
+  for (int j = 0; j < NQ; ++j)
+    for (int i = 0; i < NQ; ++i) {
+      double ux = 0, uy = 0, vx = 0, vy = 0;
+      for (int b = 0; b < NB; ++b)
+        for (int a = 0; a < NB; ++a) {
+          const double dxi = t1[TAB + i * NB + a] * t2[j * NB + b];
+          const double deta = t1[i * NB + a] * t2[TAB + j * NB + b];
+          ux += dxi * xe[b][a];
+          uy += deta * xe[b][a];
+          vx += dxi * ge[b][a];
+          vy += deta * ge[b][a];
+        }
+      const int q = j * NQ + i;
+      const double a11 = Gt[(size_t)(0 * NQ2 + q) * NT];
+      const double a12 = Gt[(size_t)(1 * NQ2 + q) * NT];
+      const double a22 = Gt[(size_t)(2 * NQ2 + q) * NT];
+      out[(size_t)e * NQ2 + q] = -(ux * (a11 * vx + a12 * vy) + uy * (a12 * vx + a22 * vy));
+    }
+}
+
+// --------------------------------------------------------------------------
+// diagonal of A (FP64): diag[A] = -sum_{e,a} sum_q gradR_a^T G_q gradR_a,
+// gathered per function in a fixed element order; Dirichlet rows get 1.
+// --------------------------------------------------------------------------
+template <int P, int NQ>
+__global__ void k_diag(double *diag, const double *G, const double *tabx,
+                       const double *taby, const uint8_t *dirmask, int nex,
+                       int ney, int nfx, int nfy, int TX, int TY, int ntx) {
+  constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nfx * nfy) return;
+  const int gx = A % nfx, gy = A / nfx;
+  if (is_dirichlet(dirmask, A, gx, gy, nfx, nfy)) {
+    diag[A] = 1.0;
+    return;
+  }
+  const int NT = TX * TY;
+  double acc = 0.0;
+  for (int b = P; b >= 0; --b) {
+    const int ey = gy - b;
+    if (ey < 0 || ey >= ney) continue;
+    for (int a = P; a >= 0; --a) {
+      const int ex = gx - a;
+      if (ex < 0 || ex >= nex) continue;
+      const double *t1 = tabx + (size_t)ex * 2 * TAB;
+      const double *t2 = taby + (size_t)ey * 2 * TAB;
+      const int tile = (ey / TY) * ntx + ex / TX, t = (ey % TY) * TX + ex % TX;
+      const double *Gt = G + (size_t)tile * 3 * NQ2 * NT + t;
+      for (int j = 0; j < NQ; ++j)
+        for (int i = 0; i < NQ; ++i) {
+          const int q = j * NQ + i;
+          const double dx = t1[TAB + i * NB + a] * t2[j * NB + b];
+          const double dy = t1[i * NB + a] * t2[TAB + j * NB + b];
+          acc -= Gt[(size_t)(0 * NQ2 + q) * NT] * dx * dx +
+                 2.0 * Gt[(size_t)(1 * NQ2 + q) * NT] * dx * dy +
+                 Gt[(size_t)(2 * NQ2 + q) * NT] * dy * dy;
+        }
+    }
+  }
+  diag[A] = acc;
+}
+
+}  // namespace igak

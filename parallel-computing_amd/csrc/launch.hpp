@@ -1,0 +1,160 @@
+// launch.hpp -- host-side launchers of the templated kernels, one set per
+// degree P (instantiated in kern_p<P>.hip so the degrees compile in
+// parallel).  Quadrature is the reference's p+1 Gauss points per direction.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace igak {
+
+// element tiles with compiled kernels
+enum TileId { TILE_32x32 = 0, TILE_32x16 = 1, TILE_16x16 = 2, TILE_64x8 = 3, NTILE = 4 };
+inline void tile_shape(int id, int &tx, int &ty) {
+  static const int s[NTILE][2] = {{32, 32}, {32, 16}, {16, 16}, {64, 8}};
+  tx = s[id][0];
+  ty = s[id][1];
+}
+inline int tile_id(int tx, int ty) {
+  for (int i = 0; i < NTILE; ++i) {
+    int a, b;
+    tile_shape(i, a, b);
+    if (a == tx && b == ty) return i;
+  }
+  return -1;
+}
+
+template <int P>
+struct Launch {
+  static constexpr int NQ = P + 1;
+
+  template <typename T, int TX, int TY, bool EBE>
+  static hipError_t apply_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
+    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE>), dim3(ntiles), dim3(TX * TY), 0, s, a);
+    return hipGetLastError();
+  }
+  template <typename T, bool EBE>
+  static hipError_t apply_tile(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {
+    switch (tile) {
+      case TILE_32x32: return apply_t<T, 32, 32, EBE>(a, ntiles, s);
+      case TILE_32x16: return apply_t<T, 32, 16, EBE>(a, ntiles, s);
+      case TILE_16x16: return apply_t<T, 16, 16, EBE>(a, ntiles, s);
+      case TILE_64x8: return apply_t<T, 64, 8, EBE>(a, ntiles, s);
+    }
+    return hipErrorInvalidValue;
+  }
+  static hipError_t apply(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles,
+                          hipStream_t s) {
+    if (f64) return ebe ? apply_tile<double, true>(a, tile, ntiles, s)
+                        : apply_tile<double, false>(a, tile, ntiles, s);
+    return ebe ? apply_tile<float, true>(a, tile, ntiles, s)
+               : apply_tile<float, false>(a, tile, ntiles, s);
+  }
+
+  template <typename T>
+  static hipError_t seam(T *y, const T *halo, const uint8_t *mask, int nex, int ney, int nfx,
+                         int nfy, const Layout &L, int mode, int mask_out, const int *skip,
+                         hipStream_t s) {
+    const int64_t n = (int64_t)(L.ntx - 1) * P * nfy + (int64_t)(L.nty - 1) * P * nfx;
+    if (n == 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL((k_seam<T, P>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, y,
+                       halo, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx, L.nty, mode, mask_out,
+                       skip);
+    return hipGetLastError();
+  }
+
+  static hipError_t diag(double *d, const double *G, const double *tabx, const double *taby,
+                         const uint8_t *mask, int nex, int ney, int nfx, int nfy,
+                         const Layout &L, hipStream_t s) {
+    const int bs = 256, nf = nfx * nfy;
+    hipLaunchKernelGGL((k_diag<P, NQ>), dim3((nf + bs - 1) / bs), dim3(bs), 0, s, d, G, tabx,
+                       taby, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx);
+    return hipGetLastError();
+  }
+
+  template <typename T>
+  static hipError_t geometry(T *G, const double *tabx, const double *taby, const double *cp,
+                             const double *wq, const double *kappa, int nex, int ney, int nfx,
+                             const Layout &L, hipStream_t s) {
+    const int nt = L.ntx * L.nty;
+    int bs = L.tx * L.ty;
+    if (bs > 256) bs = 256;
+    hipLaunchKernelGGL((k_geometry<T, P, NQ>), dim3(nt), dim3(bs), 0, s, G, tabx, taby, cp, wq,
+                       kappa, nex, ney, nfx, L.tx, L.ty, L.ntx);
+    return hipGetLastError();
+  }
+
+  static hipError_t load(double *F, double *fJ, const double *tabx, const double *taby,
+                         const double *cp, const double *wq, const uint8_t *mask, int nex,
+                         int ney, int nfx, int nfy, int kind, double value, hipStream_t s) {
+    const int bs = 256;
+    const int ne = nex * ney, nf = nfx * nfy;
+    hipLaunchKernelGGL((k_load_qp<P, NQ>), dim3((ne + bs - 1) / bs), dim3(bs), 0, s, fJ, tabx,
+                       taby, cp, wq, nex, ney, nfx, kind, value);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_load_gather<P, NQ>), dim3((nf + bs - 1) / bs), dim3(bs), 0, s, F, fJ,
+                       tabx, taby, mask, nex, ney, nfx, nfy);
+    return hipGetLastError();
+  }
+
+  static hipError_t kappa_grad(double *out, const double *x, const double *g, const double *G0,
+                               const double *tabx, const double *taby, const uint8_t *mask,
+                               int nex, int ney, int nfx, int nfy, const Layout &L,
+                               hipStream_t s) {
+    const int bs = 256, ne = nex * ney;
+    hipLaunchKernelGGL((k_kappa_grad<P, NQ>), dim3((ne + bs - 1) / bs), dim3(bs), 0, s, out, x,
+                       g, G0, tabx, taby, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx);
+    return hipGetLastError();
+  }
+};
+
+// per-degree entry points (defined in kern_p<P>.hip)
+struct Ops {
+  hipError_t (*apply)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles, hipStream_t);
+  hipError_t (*seam64)(double *, const double *, const uint8_t *, int, int, int, int,
+                       const Layout &, int, int, const int *, hipStream_t);
+  hipError_t (*seam32)(float *, const float *, const uint8_t *, int, int, int, int,
+                       const Layout &, int, int, const int *, hipStream_t);
+  hipError_t (*diag)(double *, const double *, const double *, const double *, const uint8_t *,
+                     int, int, int, int, const Layout &, hipStream_t);
+  hipError_t (*geom64)(double *, const double *, const double *, const double *,
+                       const double *, const double *, int, int, int, const Layout &,
+                       hipStream_t);
+  hipError_t (*geom32)(float *, const double *, const double *, const double *,
+                       const double *, const double *, int, int, int, const Layout &,
+                       hipStream_t);
+  hipError_t (*load)(double *, double *, const double *, const double *, const double *,
+                     const double *, const uint8_t *, int, int, int, int, int, double,
+                     hipStream_t);
+  hipError_t (*kappa_grad)(double *, const double *, const double *, const double *,
+                           const double *, const double *, const uint8_t *, int, int, int,
+                           int, const Layout &, hipStream_t);
+};
+
+template <int P>
+Ops make_ops() {
+  Ops o;
+  o.apply = &Launch<P>::apply;
+  o.seam64 = &Launch<P>::template seam<double>;
+  o.seam32 = &Launch<P>::template seam<float>;
+  o.geom64 = &Launch<P>::template geometry<double>;
+  o.geom32 = &Launch<P>::template geometry<float>;
+  o.load = &Launch<P>::load;
+  o.diag = &Launch<P>::diag;
+  o.kappa_grad = &Launch<P>::kappa_grad;
+  return o;
+}
+
+// ops_for_degree(P) returns nullptr for degrees without compiled kernels
+const Ops *ops_for_degree(int p);
+
+}  // namespace igak
+
+#define IGAK_DEFINE_DEGREE(P)                      \
+  namespace igak {                                 \
+  const Ops *ops_p##P() {                          \
+    static const Ops o = make_ops<P>();            \
+    return &o;                                     \
+  }                                                \
+  }

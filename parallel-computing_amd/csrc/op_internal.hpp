@@ -1,0 +1,62 @@
+// op_internal.hpp -- the operator's device state, shared by the translation
+// units of the library (apply, solver, block apply).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "iga_mf.h"
+#include "launch.hpp"
+
+#define IGA_HIP_TRY(expr)                                  \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) {                                \
+      (void)hipGetLastError();                             \
+      return e_ == hipErrorOutOfMemory ? IGA_ERR_OUT_OF_MEMORY : IGA_ERR_HIP; \
+    }                                                      \
+  } while (0)
+
+struct iga_op {
+  int device = 0;
+  int p = 0, nq = 0, nex = 0, ney = 0, nfx = 0, nfy = 0;
+  int nfunc = 0, nelem = 0, nqp = 0, nloc = 0, ndir = 0;
+  int rational = 0;
+  bool ring = true;                 // Dirichlet set is the boundary ring
+  const igak::Ops *ops = nullptr;
+  uint8_t *dirmask = nullptr;       // device, only when !ring
+  double *tabx64 = nullptr, *taby64 = nullptr;
+  float *tabx32 = nullptr, *taby32 = nullptr;
+  double *cp = nullptr;             // device [nfunc][2]
+  double *wq = nullptr;             // device Gauss weights on [0,1]
+  igak::Layout L64{}, L32{};
+  int tile64 = 0, tile32 = 0;       // igak::TileId
+  double *G64 = nullptr;            // kappa-weighted
+  double *G0 = nullptr;             // geometry only (== G64 when kappa unset)
+  float *G32 = nullptr;
+  double *kappa = nullptr;          // device [E][nqp] or nullptr
+  double *halo64 = nullptr;
+  float *halo32 = nullptr;
+  double *fJ = nullptr;             // load-vector scratch [E][nqp]
+  bool want64 = true, want32 = true;
+  int gen = 0;                      // bumped whenever G_q changes (set_kappa)
+  int64_t bytes = 0;
+  double setup_ms = 0.0;
+  std::vector<void *> allocs;
+  // k_apply timing (iga_op_profile)
+  bool prof_on = false;
+  int prof_cap = 0, prof_used = 0;
+  std::vector<hipEvent_t> prof_ev;
+  // host copies used by the solver set-up
+  std::vector<double> h_tabx, h_taby, h_wq;
+};
+
+namespace igai {
+int dev_alloc(iga_op *op, void **p, size_t bytes);
+int set_device(int dev);
+// y = A x with an explicit layout / precision (used by the solver)
+int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
+              int mask_in, int mask_out, hipStream_t s, const int *skip);
+}  // namespace igai

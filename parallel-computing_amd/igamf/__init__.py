@@ -1,0 +1,416 @@
+"""Python binding of libigamf (the MI355X matrix-free IGA Poisson operator).
+
+A thin ctypes layer over the C ABI declared in include/iga_mf.h; PyTorch
+tensors provide device memory and streams.  There is no CPU fallback: every
+call goes to the HIP library, and loading fails loudly when it is missing.
+
+Mirrors the reference's operator interface: ``Operator.mult(x, y)`` is
+``MyMatMult`` (matrix_free_gpu/matrixfreegpu.cu:52-66) and
+``Operator.create_vecs()`` is ``MyMatCreateVecs`` (:68-78); ``Solver`` is the
+KSP/PCSHELL set-up of :257-306.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libigamf.so")
+
+F64, F32 = 0, 1
+SF, EBE = 0, 1
+IEN_ATOMIC, INVLM_GATHER = 0, 1
+GEOM_UNIT_SQUARE, GEOM_QUARTER_ANNULUS, GEOM_PERTURBED_SQUARE = 0, 1, 2
+PC_NONE, PC_JACOBI, PC_FEM_CG, PC_FEM_CHEB = 0, 1, 2, 3
+SOLVE_PCG_F64, SOLVE_PCG_F32_REFINE = 0, 1
+LOAD_REF_POLY, LOAD_CONST, LOAD_SINSIN = 0, 1, 2
+
+
+class IgaError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = _lib().iga_status_string(status).decode() if _LIB is not None else str(status)
+        super().__init__(f"{what}: {msg} (status {status})" if what else msg)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("p", "q", "nelem_x", "nelem_y", "nqp_x", "nqp_y")] + [
+        ("cp", C.c_void_p), ("weights", C.c_void_p), ("ien", C.c_void_p), ("id", C.c_void_p),
+        ("dir", C.c_void_p), ("ndir", C.c_int), ("extraction_x", C.c_void_p),
+        ("extraction_y", C.c_void_p), ("elem_size_x", C.c_void_p),
+        ("elem_size_y", C.c_void_p), ("kappa", C.c_void_p)]
+
+
+class MeshParams(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_int), ("nelem_x", C.c_int), ("nelem_y", C.c_int),
+                ("L", C.c_double), ("r0", C.c_double), ("r1", C.c_double),
+                ("perturb", C.c_double), ("w_lo", C.c_double), ("w_hi", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class OpOpts(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("device", "want_f64", "want_f32", "tile_x_f64",
+                                       "tile_y_f64", "tile_x_f32", "tile_y_f32")]
+
+
+class OpInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nfunc", "nfx", "nfy", "nelem", "nqp", "nloc", "ndir",
+                                       "rational", "tile_x_f64", "tile_y_f64", "tile_x_f32",
+                                       "tile_y_f32")] + [("device_bytes", C.c_int64),
+                                                         ("setup_ms", C.c_double)]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("kind", C.c_int), ("pc", C.c_int), ("kernel", C.c_int), ("assembly", C.c_int),
+                ("rtol", C.c_double), ("abstol", C.c_double), ("divtol", C.c_double),
+                ("maxit", C.c_int), ("inner_rtol", C.c_double), ("inner_maxit", C.c_int),
+                ("pc_inner_maxit", C.c_int), ("pc_inner_rtol", C.c_double),
+                ("pc_cheb_degree", C.c_int), ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int),
+                ("rel_residual", C.c_double), ("solve_ms", C.c_double), ("status", C.c_int)]
+
+
+class CsrView(C.Structure):
+    _fields_ = [("nrows", C.c_int), ("nnz", C.c_int64), ("rowptr", C.c_void_p),
+                ("col", C.c_void_p), ("val", C.c_void_p), ("coo_row", C.c_void_p),
+                ("elem2coo", C.c_void_p), ("dir2coo", C.c_void_p), ("ndir", C.c_int)]
+
+
+_LIB = None
+VP = C.c_void_p
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libigamf.so not built at {LIB_PATH}; run "
+                              "python parallel-computing_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.iga_status_string.restype = C.c_char_p
+        L.iga_status_string.argtypes = [C.c_int]
+        L.iga_mesh_build.argtypes = [C.POINTER(MeshParams), C.POINTER(VP)]
+        L.iga_mesh_describe.argtypes = [VP, C.POINTER(MeshDesc)]
+        L.iga_mesh_destroy.argtypes = [VP]
+        L.iga_rng_normal.argtypes = [C.c_uint64, C.c_int64, VP]
+        L.iga_rng_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, VP]
+        L.iga_op_opts_default.argtypes = [C.POINTER(OpOpts)]
+        L.iga_op_opts_default.restype = None
+        L.iga_op_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(OpOpts), C.POINTER(VP)]
+        L.iga_op_destroy.argtypes = [VP]
+        L.iga_op_get_info.argtypes = [VP, C.POINTER(OpInfo)]
+        L.iga_op_apply.argtypes = [VP, C.c_int, C.c_int, C.c_int, VP, VP, VP]
+        L.iga_op_load.argtypes = [VP, C.c_int, C.c_double, VP, VP]
+        L.iga_op_profile.argtypes = [VP, C.c_int, C.c_int]
+        L.iga_op_profile_read.argtypes = [VP, C.POINTER(C.c_int)] + [C.POINTER(C.c_double)] * 3
+        L.iga_op_set_kappa.argtypes = [VP, VP, VP]
+        L.iga_op_apply_backward.argtypes = [VP, VP, VP, VP, VP, VP]
+        L.iga_op_apply_block.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP]
+        L.iga_fem_reference_nodes.argtypes = [C.c_int] * 4 + [C.c_double] * 2 + [VP]
+        L.iga_fem_assemble.argtypes = [C.c_int, C.c_int, VP, C.c_int, C.POINTER(VP)]
+        L.iga_fem_view.argtypes = [VP, C.POINTER(CsrView)]
+        L.iga_fem_destroy.argtypes = [VP]
+        L.iga_solver_opts_default.argtypes = [C.POINTER(SolverOpts)]
+        L.iga_solver_opts_default.restype = None
+        L.iga_solver_create.argtypes = [VP, VP, C.POINTER(SolverOpts), C.POINTER(VP)]
+        L.iga_solver_solve.argtypes = [VP, VP, VP, VP, C.POINTER(SolveStats)]
+        L.iga_solver_destroy.argtypes = [VP]
+        L.iga_dot_f64.argtypes = [VP, VP, C.c_int64, C.POINTER(C.c_double), VP]
+        _LIB = L
+    return _LIB
+
+
+def lib():
+    return _lib()
+
+
+def check(status, what=""):
+    if status != 0:
+        raise IgaError(status, what)
+
+
+def rng_normal(seed, n):
+    out = np.empty(n, np.float64)
+    check(_lib().iga_rng_normal(seed, n, out.ctypes.data), "rng_normal")
+    return out
+
+
+def rng_uniform(seed, n, lo, hi):
+    out = np.empty(n, np.float64)
+    check(_lib().iga_rng_uniform(seed, n, lo, hi, out.ctypes.data), "rng_uniform")
+    return out
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+# --------------------------------------------------------------------------- mesh
+class Mesh:
+    """Host mesh of one patch, built by the library (iga_mesh_build) or wrapped
+    from caller arrays (``Mesh.from_arrays``)."""
+
+    def __init__(self, handle=None, arrays=None):
+        self._h = handle
+        self._arrays = arrays
+        self.desc = MeshDesc()
+        if handle is not None:
+            check(_lib().iga_mesh_describe(handle, C.byref(self.desc)), "mesh_describe")
+        else:
+            a = arrays
+            d = self.desc
+            d.p, d.q, d.nelem_x, d.nelem_y = a["p"], a["q"], a["nex"], a["ney"]
+            d.nqp_x, d.nqp_y = a.get("nq1", a["p"] + 1), a.get("nq2", a["q"] + 1)
+            for fld, key in (("cp", "CP"), ("weights", "W"), ("ien", "IEN"), ("id", "ID"),
+                             ("dir", "Dir"), ("extraction_x", "C1"), ("extraction_y", "C2"),
+                             ("elem_size_x", "h1"), ("elem_size_y", "h2"), ("kappa", "kappa")):
+                v = a.get(key)
+                setattr(d, fld, None if v is None else v.ctypes.data)
+            d.ndir = len(a["Dir"])
+
+    @classmethod
+    def build(cls, kind, p, nex, ney=None, L=1.0, r0=1.0, r1=2.0, perturb=0.0, w_lo=1.0,
+              w_hi=1.0, seed=0):
+        prm = MeshParams(kind, p, nex, nex if ney is None else ney, L, r0, r1, perturb, w_lo,
+                         w_hi, seed)
+        h = VP()
+        check(_lib().iga_mesh_build(C.byref(prm), C.byref(h)), "mesh_build")
+        return cls(handle=h)
+
+    @classmethod
+    def from_arrays(cls, p, q, nex, ney, CP, W, IEN, ID, Dir, C1, C2, h1, h2, kappa=None):
+        f64 = lambda v: None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+        i32 = lambda v: None if v is None else np.ascontiguousarray(v, np.int32).reshape(-1)
+        arrays = dict(p=p, q=q, nex=nex, ney=ney, CP=f64(CP), W=f64(W), IEN=i32(IEN),
+                      ID=i32(ID), Dir=i32(Dir), C1=f64(C1), C2=f64(C2), h1=f64(h1), h2=f64(h2),
+                      kappa=f64(kappa))
+        return cls(arrays=arrays)
+
+    def arrays(self):
+        """numpy copies of the mesh arrays (reference layout)."""
+        d = self.desc
+        nfx, nfy = d.nelem_x + d.p, d.nelem_y + d.q
+        nf, E, nloc = nfx * nfy, d.nelem_x * d.nelem_y, (d.p + 1) * (d.q + 1)
+
+        def grab(ptr, n, dt):
+            if not ptr:
+                return None
+            ct = C.c_double if dt == np.float64 else C.c_int
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(n,)).copy()
+        return dict(p=d.p, q=d.q, nex=d.nelem_x, ney=d.nelem_y,
+                    CP=grab(d.cp, 2 * nf, np.float64), W=grab(d.weights, nf, np.float64),
+                    IEN=grab(d.ien, E * nloc, np.int32), ID=grab(d.id, nf, np.int32),
+                    Dir=grab(d.dir, d.ndir, np.int32),
+                    C1=grab(d.extraction_x, d.nelem_x * (d.p + 1) ** 2, np.float64),
+                    C2=grab(d.extraction_y, d.nelem_y * (d.q + 1) ** 2, np.float64),
+                    h1=grab(d.elem_size_x, d.nelem_x, np.float64),
+                    h2=grab(d.elem_size_y, d.nelem_y, np.float64))
+
+    @property
+    def nfunc(self):
+        d = self.desc
+        return (d.nelem_x + d.p) * (d.nelem_y + d.q)
+
+    def __del__(self):
+        if self._h is not None and _LIB is not None:
+            _LIB.iga_mesh_destroy(self._h)
+            self._h = None
+
+
+# --------------------------------------------------------------------------- operator
+class Operator:
+    """The matrix-free operator y = A x on one GPU (iga_op)."""
+
+    def __init__(self, mesh, device=0, want_f64=True, want_f32=True, tile_f64=None,
+                 tile_f32=None, kappa=None):
+        import torch
+        self.mesh = mesh
+        o = OpOpts()
+        _lib().iga_op_opts_default(C.byref(o))
+        o.device, o.want_f64, o.want_f32 = device, int(want_f64), int(want_f32)
+        if tile_f64:
+            o.tile_x_f64, o.tile_y_f64 = tile_f64
+        if tile_f32:
+            o.tile_x_f32, o.tile_y_f32 = tile_f32
+        desc = mesh.desc
+        keep = None
+        if kappa is not None:
+            keep = np.ascontiguousarray(kappa, np.float64)
+            desc = MeshDesc.from_buffer_copy(mesh.desc)
+            desc.kappa = keep.ctypes.data
+        self.device = torch.device("cuda", device)
+        h = VP()
+        check(_lib().iga_op_create(C.byref(desc), C.byref(o), C.byref(h)), "op_create")
+        self._h = h
+        info = OpInfo()
+        check(_lib().iga_op_get_info(h, C.byref(info)), "op_get_info")
+        self.info = info
+        self.nfunc = info.nfunc
+        self.nelem = info.nelem
+        self.nqp = info.nqp
+
+    def create_vecs(self, dtype=None):
+        import torch
+        dtype = dtype or torch.float64
+        return (torch.zeros(self.nfunc, dtype=dtype, device=self.device),
+                torch.zeros(self.nfunc, dtype=dtype, device=self.device))
+
+    def apply(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None):
+        import torch
+        dt = {torch.float64: F64, torch.float32: F32}[x.dtype]
+        assert x.is_cuda and x.is_contiguous() and x.numel() == self.nfunc
+        if y is None:
+            y = torch.empty_like(x)
+        check(_lib().iga_op_apply(self._h, dt, kernel, assembly, x.data_ptr(), y.data_ptr(),
+                                  _stream_ptr(stream)), "op_apply")
+        return y
+
+    mult = apply  # MATOP_MULT
+
+    def profile(self, enable, max_calls=1024):
+        check(_lib().iga_op_profile(self._h, int(enable), max_calls), "op_profile")
+
+    def profile_read(self):
+        n = C.c_int()
+        tot, mn, mx = C.c_double(), C.c_double(), C.c_double()
+        check(_lib().iga_op_profile_read(self._h, C.byref(n), C.byref(tot), C.byref(mn),
+                                         C.byref(mx)), "op_profile_read")
+        return n.value, tot.value, mn.value, mx.value
+
+    def load(self, kind=LOAD_REF_POLY, value=0.0, stream=None):
+        import torch
+        F = torch.empty(self.nfunc, dtype=torch.float64, device=self.device)
+        check(_lib().iga_op_load(self._h, kind, value, F.data_ptr(), _stream_ptr(stream)),
+              "op_load")
+        return F
+
+    def set_kappa(self, kappa, stream=None):
+        assert kappa.dtype.is_floating_point and kappa.numel() == self.nelem * self.nqp
+        k = kappa.contiguous().double()
+        check(_lib().iga_op_set_kappa(self._h, k.data_ptr(), _stream_ptr(stream)), "set_kappa")
+
+    def backward(self, x, g, want_x=True, want_kappa=True, stream=None):
+        import torch
+        gx = torch.empty_like(g) if want_x else None
+        gk = (torch.empty(self.nelem * self.nqp, dtype=torch.float64, device=self.device)
+              if want_kappa else None)
+        check(_lib().iga_op_apply_backward(self._h, _ptr(x), g.data_ptr(), _ptr(gx), _ptr(gk),
+                                           _stream_ptr(stream)), "apply_backward")
+        return gx, gk
+
+    def apply_block(self, X, Y=None, stream=None):
+        import torch
+        assert X.dtype == torch.float32 and X.is_cuda
+        k, n = X.shape  # row-major [k][n] == column-major n x k with ld = n
+        assert n == self.nfunc
+        if Y is None:
+            Y = torch.empty_like(X)
+        check(_lib().iga_op_apply_block(self._h, k, n, X.data_ptr(), Y.data_ptr(),
+                                        _stream_ptr(stream)), "apply_block")
+        return Y
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.iga_op_destroy(self._h)
+            self._h = None
+
+
+# --------------------------------------------------------------------------- FEM-PC
+def fem_reference_nodes(p, q, nex, ney, Lx=1.0, Ly=1.0):
+    nodes = np.empty(2 * (nex + p) * (ney + q))
+    check(_lib().iga_fem_reference_nodes(p, q, nex, ney, Lx, Ly, nodes.ctypes.data),
+          "fem_reference_nodes")
+    return nodes
+
+
+class FemPC:
+    """Q1 FEM preconditioner matrix assembled on the device (iga_fem)."""
+
+    def __init__(self, nfx, nfy, nodes, device=0):
+        nodes = np.ascontiguousarray(nodes, np.float64)
+        h = VP()
+        check(_lib().iga_fem_assemble(nfx, nfy, nodes.ctypes.data, device, C.byref(h)),
+              "fem_assemble")
+        self._h = h
+        v = CsrView()
+        check(_lib().iga_fem_view(h, C.byref(v)), "fem_view")
+        self.view = v
+        self.nfx, self.nfy = nfx, nfy
+
+    def to_torch(self):
+        """Copies of the CSR (rowptr, col, val, coo_row, elem2coo, dir2coo)."""
+        import torch
+        v = self.view
+        ne = (self.nfx - 1) * (self.nfy - 1)
+
+        def wrap(ptr, n, dt):
+            out = torch.empty(n, dtype=dt, device="cuda")
+            torch.cuda.synchronize()
+            _copy_dev(out, ptr, n * out.element_size())
+            return out
+        return dict(rowptr=wrap(v.rowptr, v.nrows + 1, torch.int32),
+                    col=wrap(v.col, v.nnz, torch.int32),
+                    val=wrap(v.val, v.nnz, torch.float64),
+                    coo_row=wrap(v.coo_row, v.nnz, torch.int32),
+                    elem2coo=wrap(v.elem2coo, ne * 16, torch.int32),
+                    dir2coo=wrap(v.dir2coo, v.ndir, torch.int32))
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.iga_fem_destroy(self._h)
+            self._h = None
+
+
+_hip = None
+
+
+def _copy_dev(dst_tensor, src_ptr, nbytes):
+    """Device-to-device copy of nbytes from a raw device pointer."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [VP, VP, C.c_size_t, C.c_int]
+    rc = _hip.hipMemcpy(dst_tensor.data_ptr(), src_ptr, nbytes, 3)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+
+
+# --------------------------------------------------------------------------- solver
+class Solver:
+    def __init__(self, op, fem=None, **kw):
+        o = SolverOpts()
+        _lib().iga_solver_opts_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        h = VP()
+        check(_lib().iga_solver_create(op._h, None if fem is None else fem._h, C.byref(o),
+                                       C.byref(h)), "solver_create")
+        self._h = h
+        self.op = op
+        self.opts = o
+
+    def solve(self, f, u=None, stream=None):
+        import torch
+        if u is None:
+            u = torch.zeros_like(f)
+        st = SolveStats()
+        rc = _lib().iga_solver_solve(self._h, f.data_ptr(), u.data_ptr(), _stream_ptr(stream),
+                                     C.byref(st))
+        if rc not in (0,):
+            raise IgaError(rc, "solver_solve")
+        return u, st
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.iga_solver_destroy(self._h)
+            self._h = None

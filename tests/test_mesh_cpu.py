@@ -1,0 +1,109 @@
+"""CPU checks of the library's host side: its mesh generator against the
+oracle's (and, through the oracle, the reference's) preprocessing, its random
+numbers, and the status codes of invalid input (no GPU call is made)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import igamf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "iga_mf.h")).read()
+    names = set(re.findall(r"^\s*(?:int|void|const char \*)\s*(iga_\w+)\s*\(", hdr, re.M))
+    assert len(names) >= 24
+    L = igamf.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_rng_matches_oracle():
+    assert np.array_equal(igamf.rng_normal(7, 1001), O.rng_normal(7, 1001))
+    assert np.array_equal(igamf.rng_uniform(3, 999, -0.1, 0.1),
+                          O.rng_uniform(3, 999, -0.1, 0.1))
+
+
+@pytest.mark.parametrize("p,nex,ney", [(1, 5, 7), (2, 32, 32), (3, 16, 9), (4, 6, 11),
+                                       (5, 7, 7), (6, 8, 10), (8, 10, 10)])
+def test_square_mesh_matches_oracle(p, nex, ney):
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, nex, ney).arrays()
+    o = O.mesh_unit_square(p, nex, ney)
+    assert np.array_equal(m["IEN"], o.IEN)
+    assert np.array_equal(m["ID"], o.ID)
+    assert np.array_equal(m["Dir"], o.Dir)
+    assert np.abs(m["CP"] - o.CP).max() < 1e-15
+    assert np.abs(m["C1"] - o.C1).max() < 1e-13
+    assert np.abs(m["C2"] - o.C2).max() < 1e-13
+    assert m["W"] is None
+
+
+@pytest.mark.parametrize("p,nex,ney", [(2, 6, 9), (3, 16, 16), (3, 40, 7), (5, 9, 13)])
+def test_annulus_mesh_matches_oracle(p, nex, ney):
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, p, nex, ney, r0=1.0, r1=2.0).arrays()
+    o = O.mesh_quarter_annulus(p, nex, ney, 1.0, 2.0)
+    assert np.abs(m["CP"] - o.CP).max() < 1e-13
+    assert np.abs(m["W"] - o.W).max() < 1e-13
+    assert np.array_equal(m["IEN"], o.IEN)
+
+
+def test_perturbed_square_draws():
+    p, n = 3, 12
+    m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, p, n, perturb=0.1, w_lo=0.8, w_hi=1.2,
+                         seed=3).arrays()
+    base = O.mesh_unit_square(p, n).CP.reshape(-1, 2)
+    cp = m["CP"].reshape(-1, 2)
+    nf = n + p
+    ring = np.zeros((nf, nf), bool)
+    ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = True
+    ring = ring.reshape(-1)
+    assert np.array_equal(cp[ring], base[ring])
+    d = np.abs(cp[~ring] - base[~ring])
+    assert d.max() <= 0.1 / n + 1e-15 and d.max() > 0.05 / n
+    assert m["W"].min() >= 0.8 and m["W"].max() <= 1.2
+
+
+def _op_status(**over):
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 8)
+    d = igamf.MeshDesc.from_buffer_copy(m.desc)
+    keep = []
+    for k, v in over.items():
+        if isinstance(v, np.ndarray):
+            keep.append(v)
+            v = v.ctypes.data
+        setattr(d, k, v)
+    h = igamf.VP()
+    rc = igamf.lib().iga_op_create(C.byref(d), None, C.byref(h))
+    assert not h.value
+    return rc
+
+
+def test_create_rejects_invalid_input():
+    assert _op_status(p=0) == 2          # INVALID_DEGREE
+    assert _op_status(p=9) == 2
+    assert _op_status(nqp_x=11) == 3     # INVALID_QUADRATURE
+    assert _op_status(nqp_x=0) == 3
+    assert _op_status(nelem_x=4) == 4    # TOO_FEW_ELEMENTS (p = 3 needs 5)
+    assert _op_status(cp=None) == 1      # INVALID_ARGUMENT
+    bad_ien = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 8).arrays()["IEN"].copy()
+    bad_ien[5] = 10 ** 6
+    assert _op_status(ien=bad_ien) == 5  # BAD_MESH
+    bad_id = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 8).arrays()["ID"].copy()
+    bad_id[12] = -1                      # interior function marked Dirichlet, not in Dir
+    assert _op_status(id=bad_id) == 5
+    assert igamf.lib().iga_status_string(5) == b"inconsistent mesh (IEN / ID / Dir / extraction)"
+
+
+def test_mesh_build_rejects_invalid_input():
+    h = igamf.VP()
+    prm = igamf.MeshParams(0, 9, 20, 20, 1.0, 1.0, 2.0, 0.0, 1.0, 1.0, 0)
+    assert igamf.lib().iga_mesh_build(C.byref(prm), C.byref(h)) == 2
+    prm = igamf.MeshParams(0, 3, 4, 20, 1.0, 1.0, 2.0, 0.0, 1.0, 1.0, 0)
+    assert igamf.lib().iga_mesh_build(C.byref(prm), C.byref(h)) == 4
+    prm = igamf.MeshParams(1, 1, 20, 20, 1.0, 1.0, 2.0, 0.0, 1.0, 1.0, 0)
+    assert igamf.lib().iga_mesh_build(C.byref(prm), C.byref(h)) == 2

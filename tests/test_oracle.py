@@ -1,0 +1,364 @@
+"""Pins of the CPU oracle (oracle/iga_oracle.c) before it is trusted.
+
+Each oracle function is checked against (a) the golden vectors written by the
+reference's own compiled CPU units (tests/golden/*.npz), (b) oracle/_ref
+directly when it is built, and (c) what the mathematics fixes: library
+routines (numpy Gauss-Legendre, scipy B-splines), closed forms, invariants
+and brute force on small inputs.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+from scipy.interpolate import BSpline
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def golden_mesh(g):
+    p, n = int(g["p"]), int(g["nelem"])
+    h = np.full(n, 1.0 / n)
+    return O.Mesh(p, p, n, n, g["CP"], None, g["IEN"], g["ID"], g["Dir"], g["extraction"],
+                  g["extraction"], h, h)
+
+
+MF_GOLD = ["mf_p2_n32.npz", "mf_p3_n16.npz"]
+
+
+# ---------------------------------------------------------------- 1D pieces
+@pytest.mark.parametrize("p", range(1, 9))
+def test_bernstein_closed_form_and_invariants(p):
+    rng = np.random.default_rng(p)
+    from math import comb
+    for xi in rng.uniform(0, 1, 7):
+        B, dB = O.bernstein(p, xi)
+        ref = [comb(p, k) * xi ** k * (1 - xi) ** (p - k) for k in range(p + 1)]
+        assert np.allclose(B, ref, rtol=1e-14, atol=1e-15)
+        assert abs(B.sum() - 1.0) < 1e-14 and abs(dB.sum()) < 1e-12
+        h = 1e-6
+        Bp, _ = O.bernstein(p, xi + h)
+        Bm, _ = O.bernstein(p, xi - h)
+        assert np.allclose(dB, (Bp - Bm) / (2 * h), atol=1e-7 * p * p)
+
+
+@pytest.mark.parametrize("name", MF_GOLD)
+def test_bernstein_table_matches_reference(name):
+    g = gold(name)
+    p = int(g["p"])
+    x, _ = O.gauss01(p + 1)
+    B = np.array([O.bernstein(p, xi)[0] for xi in x]).reshape(-1)
+    dB = np.array([O.bernstein(p, xi)[1] for xi in x]).reshape(-1)
+    assert np.abs(B - g["bern_B"]).max() < 1e-14
+    assert np.abs(dB - g["bern_dB"]).max() < 1e-13
+
+
+@pytest.mark.parametrize("n", range(1, 11))
+def test_gauss_against_numpy_and_exactness(n):
+    x, w = O.gauss01(n)
+    t, v = np.polynomial.legendre.leggauss(n)
+    assert np.abs(x - (t + 1) / 2).max() < 1e-15
+    assert np.abs(w - v / 2).max() < 1e-15
+    for k in range(2 * n):
+        assert abs(np.dot(w, x ** k) - 1.0 / (k + 1)) < 1e-14
+
+
+def test_gauss_matches_reference_tables():
+    for name in MF_GOLD:
+        g = gold(name)
+        x, w = O.gauss01(int(g["p"]) + 1)
+        assert np.abs(x - g["quad_x"]).max() < 1e-15
+        assert np.abs(w - g["quad_w"]).max() < 3e-16
+    assert O.gauss01(3)[0].shape == (3,)
+    with pytest.raises(ValueError):
+        O.gauss01(11)
+
+
+@pytest.mark.parametrize("name", MF_GOLD)
+def test_extraction_matches_reference(name):
+    g = gold(name)
+    p = int(g["p"])
+    C = O.extraction(p, g["knots"])
+    assert np.array_equal(C.reshape(-1), g["extraction"])
+
+
+@pytest.mark.parametrize("p,n", [(1, 5), (2, 7), (3, 9), (4, 6), (5, 8), (7, 10), (8, 11)])
+def test_extraction_reproduces_bsplines(p, n):
+    """C_e . Bernstein = the B-splines of scipy (independent Cox-de Boor)."""
+    S = O.open_knots(p, n)
+    C = O.extraction(p, S)
+    rng = np.random.default_rng(p * 100 + n)
+    for e in range(n):
+        a, b = S[p + e], S[p + e + 1]
+        for u in rng.uniform(a, b, 3):
+            t = (u - a) / (b - a)
+            B, _ = O.bernstein(p, t)
+            N_ext = C[e] @ B
+            D = BSpline.design_matrix(np.array([u]), S, p).toarray()[0]
+            assert np.allclose(N_ext, D[e:e + p + 1], atol=1e-13)
+            Nc, span = O.bspline_basis(p, S, u)
+            assert span == p + e and np.allclose(Nc, N_ext, atol=1e-13)
+    # uniform knots: interior operators identical
+    if n > 2 * p + 1:
+        for e in range(p, n - p):
+            assert np.allclose(C[e], C[p], atol=1e-13)
+
+
+def test_greville_matches_reference():
+    for name in MF_GOLD:
+        g = gold(name)
+        p = int(g["p"])
+        assert np.array_equal(O.greville(p, g["knots"]), g["greville"])
+
+
+# ---------------------------------------------------------------- topology
+@pytest.mark.parametrize("name", MF_GOLD)
+def test_topology_matches_reference(name):
+    g = gold(name)
+    p, n = int(g["p"]), int(g["nelem"])
+    IEN, ID, Dir = O.topology(p, p, n, n)
+    assert np.array_equal(IEN, g["IEN"])
+    assert np.array_equal(ID, g["ID"])
+    assert np.array_equal(Dir, g["Dir"])
+
+
+def test_ien_closed_form():
+    p, q, nex, ney = 3, 2, 7, 5
+    IEN, ID, Dir = O.topology(p, q, nex, ney)
+    nfx = nex + p
+    IEN = IEN.reshape(ney, nex, q + 1, p + 1)
+    for ey in range(ney):
+        for ex in range(nex):
+            for b in range(q + 1):
+                for a in range(p + 1):
+                    assert IEN[ey, ex, b, a] == (ey + b) * nfx + ex + a
+    nfy = ney + q
+    ring = [j * nfx + i for j in range(nfy) for i in range(nfx)
+            if i in (0, nfx - 1) or j in (0, nfy - 1)]
+    assert list(Dir) == ring
+    assert all(ID[A] == -1 for A in ring) and (ID >= 0).sum() == (nfx - 2) * (nfy - 2)
+
+
+# ---------------------------------------------------------------- element numerics
+@pytest.mark.parametrize("name", MF_GOLD)
+def test_element_qp_matches_reference(name):
+    g = gold(name)
+    m = golden_mesh(g)
+    for tag in ("corner", "interior"):
+        ex, ey = g[f"{tag}_elem"]
+        R, dx, dy, JxW, _ = O.element_qp(m, int(ex), int(ey))
+        assert np.abs(R.reshape(-1) - g[f"{tag}_R"]).max() < 1e-14
+        assert rel(dx.reshape(-1), g[f"{tag}_dRdx"]) < 1e-13
+        assert rel(dy.reshape(-1), g[f"{tag}_dRdy"]) < 1e-13
+        assert rel(JxW, g[f"{tag}_JxW"]) < 1e-14
+
+
+# ---------------------------------------------------------------- apply
+@pytest.mark.parametrize("name", MF_GOLD)
+def test_apply_matches_reference_golden(name):
+    g = gold(name)
+    m = golden_mesh(g)
+    y = O.apply_mf(m, g["x"])
+    assert rel(y, g["y_mf"]) < 1e-13
+    ys = O.apply_sf_ref(m, g["x"])
+    assert rel(ys, g["y_sf"]) < 1e-13
+    assert np.all(y[g["Dir"]] == 0.0)
+
+
+@pytest.mark.parametrize("name", MF_GOLD)
+def test_load_matches_reference_golden(name):
+    g = gold(name)
+    assert rel(O.load(golden_mesh(g), O.LOAD_REF_POLY), g["F_ref"]) < 1e-13
+
+
+def _bspline_1d_mats(p, n, L=1.0):
+    """1D stiffness and mass matrices of the open uniform B-spline space by
+    high-order Gauss quadrature with scipy B-splines (independent of the
+    oracle's extraction and element loop)."""
+    S = O.open_knots(p, n, L)
+    nf = n + p
+    t, w = np.polynomial.legendre.leggauss(p + 3)
+    K = np.zeros((nf, nf)); M = np.zeros((nf, nf))
+    for e in range(n):
+        a, b = S[p + e], S[p + e + 1]
+        u = a + (b - a) * (t + 1) / 2
+        ww = w * (b - a) / 2
+        N = BSpline.design_matrix(u, S, p).toarray()
+        dN = np.zeros_like(N)
+        for i in range(nf):
+            c = np.zeros(nf); c[i] = 1.0
+            dN[:, i] = BSpline(S, c, p).derivative()(u)
+        K += dN.T @ (ww[:, None] * dN)
+        M += N.T @ (ww[:, None] * N)
+    return K, M
+
+
+@pytest.mark.parametrize("p,n", [(2, 8), (3, 7), (4, 9)])
+def test_apply_identity_geometry_closed_form(p, n):
+    """On the identity map the stiffness is -(K1 (x) M2 + M1 (x) K2)."""
+    m = O.mesh_unit_square(p, n)
+    K1, M1 = _bspline_1d_mats(p, n)
+    Kfull = -(np.kron(M1, K1) + np.kron(K1, M1))  # index A = j*nfx + i
+    x = O.rng_normal(11, m.nfunc)
+    y = O.apply_mf(m, x)
+    y_cf = Kfull @ x
+    y_cf[m.ID < 0] = 0.0
+    assert rel(y, y_cf) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["square", "annulus"])
+def test_apply_invariants(kind):
+    m = O.mesh_unit_square(3, 9) if kind == "square" else O.mesh_quarter_annulus(3, 8, 6)
+    cp = m.CP.reshape(-1, 2)
+    interior = m.ID >= 0
+    y1 = O.apply_mf(m, np.ones(m.nfunc))
+    lin = 0.4 - 1.3 * cp[:, 0] + 2.1 * cp[:, 1]
+    y2 = O.apply_mf(m, lin)
+    u = O.rng_normal(1, m.nfunc) * interior
+    v = O.rng_normal(2, m.nfunc) * interior
+    Au, Av = O.apply_mf(m, u), O.apply_mf(m, v)
+    s = np.abs(Au).max()
+    assert np.abs(y1).max() < 1e-12 * s
+    # linear reproduction is exact for polynomial maps; with rational basis
+    # functions Gauss quadrature of grad R is inexact, so only small
+    assert np.abs(y2).max() < (1e-11 if m.W is None else 1e-7) * s
+    assert abs(v @ Au - u @ Av) < 1e-12 * abs(v @ Au)
+    A = O.assembled_matrix(m)
+    x = O.rng_normal(3, m.nfunc)
+    assert rel(A @ x, O.apply_mf(m, x)) < 1e-13
+    yc, _ = O.apply_mf_colored(m, x, 4)
+    assert rel(yc, O.apply_mf(m, x)) < 1e-14
+
+
+def test_annulus_geometry_exact():
+    m = O.mesh_quarter_annulus(3, 6, 10, 1.0, 2.0)
+    area = 0.0
+    for ey in range(m.ney):
+        for ex in range(m.nex):
+            R, dx, dy, JxW, xq = O.element_qp(m, ex, ey)
+            area += JxW.sum()
+            r = np.hypot(xq[:, 0], xq[:, 1])
+            assert np.all(r >= 1 - 1e-14) and np.all(r <= 2 + 1e-14)
+    assert abs(area - 3 * np.pi / 4) < 1e-13
+    cp = m.CP.reshape(m.nfy, m.nfx, 2)
+    # the r = r0 and r = r1 rows of control points are circular arcs: check
+    # points of the boundary curves
+    for i, r0 in ((0, 1.0), (m.nfx - 1, 2.0)):
+        W = m.W.reshape(m.nfy, m.nfx)[:, i]
+        S = O.open_knots(3, m.ney)
+        for u in np.linspace(0, 1, 17)[:-1]:
+            N = BSpline.design_matrix(np.array([u]), S, 3).toarray()[0]
+            pt = (N * W) @ cp[:, i, :] / (N @ W)
+            assert abs(np.hypot(*pt) - r0) < 1e-14
+
+
+def test_manufactured_solution_converges():
+    """Load + apply: solve A u = F for -lap u = 2 pi^2 sin sin with the
+    reference sign, u_h -> -sin(pi x) sin(pi y) at rate ~h^(p+1)."""
+    errs = []
+    for n in (8, 16):
+        m = O.mesh_unit_square(2, n)
+        A = O.assembled_matrix(m)
+        F = O.load(m, O.LOAD_SINSIN, 2 * np.pi ** 2)
+        I = np.where(m.ID >= 0)[0]
+        u = np.zeros(m.nfunc)
+        u[I] = spla.spsolve(A[I][:, I].tocsc(), F[I])
+        err = 0.0
+        for ey in range(m.ney):
+            for ex in range(m.nex):
+                R, dx, dy, JxW, xq = O.element_qp(m, ex, ey)
+                e = ey * m.nex + ex
+                uh = R @ u[m.IEN[e * m.nloc:(e + 1) * m.nloc]]
+                ue = -np.sin(np.pi * xq[:, 0]) * np.sin(np.pi * xq[:, 1])
+                err += np.sum(JxW * (uh - ue) ** 2)
+        errs.append(np.sqrt(err))
+    rate = np.log2(errs[0] / errs[1])
+    assert errs[1] < 1e-3 and rate > 2.6, (errs, rate)
+
+
+def test_kappa_gradient_by_finite_differences():
+    m = O.mesh_quarter_annulus(2, 5, 4)
+    nk = m.nelem * m.nq1 * m.nq2
+    kappa = np.exp(0.3 * O.rng_normal(4, nk))
+    mk = m.with_kappa(kappa)
+    u = O.rng_normal(5, m.nfunc)
+    g = O.rng_normal(6, m.nfunc)
+    grad = O.kappa_grad(mk, u, g)
+    d = O.rng_normal(7, nk)
+    h = 1e-6
+    fp = g @ O.apply_mf(m.with_kappa(kappa + h * d), u)
+    fm = g @ O.apply_mf(m.with_kappa(kappa - h * d), u)
+    assert abs((fp - fm) / (2 * h) - grad @ d) < 1e-7 * abs(grad @ d)
+    # kappa = 1 reproduces the plain operator
+    assert rel(O.apply_mf(m.with_kappa(np.ones(nk)), u), O.apply_mf(m, u)) < 1e-15
+
+
+# ---------------------------------------------------------------- FEM preconditioner
+@pytest.mark.parametrize("name", ["fem_p3_n16.npz", "fem_p4_n12.npz"])
+def test_fem_matches_reference_golden(name):
+    g = gold(name)
+    p, n = int(g["p"]), int(g["nelem"])
+    nodes = O.fem_nodes(p, p, n, n)
+    assert np.abs(nodes - g["nodes"]).max() < 1e-15
+    rows, cols, e2c, d2c, val = O.fem_assemble(n + p, n + p, nodes)
+    assert np.array_equal(rows, g["rows"]) and np.array_equal(cols, g["cols"])
+    assert np.array_equal(e2c, g["elem2coo"]) and np.array_equal(d2c, g["dir2coo"])
+    assert rel(val, g["val"]) < 1e-14
+
+
+def test_fem_closed_forms():
+    p, n = 3, 10
+    nf = n + p
+    M = nf - 2
+    nodes = O.fem_nodes(p, p, n, n)
+    rows, cols, e2c, d2c, val = O.fem_assemble(nf, nf, nodes)
+    assert len(rows) == (3 * M - 2) ** 2 + (nf * nf - M * M)
+    key = rows.astype(np.int64) * nf * nf + cols
+    assert np.all(np.diff(key) > 0)  # sorted, unique
+    A = sp.coo_matrix((val, (rows, cols)), shape=(nf * nf, nf * nf)).tocsr()
+    # uniform grid at p = 3: the Q1 stencil 8/3, -1/3 with the reference sign
+    c = (nf // 2) * nf + nf // 2
+    row = A.getrow(c).toarray()[0]
+    assert abs(row[c] + 8.0 / 3.0) < 1e-13
+    for d in (-nf - 1, -nf, -nf + 1, -1, 1, nf - 1, nf, nf + 1):
+        assert abs(row[c + d] - 1.0 / 3.0) < 1e-13
+    assert np.all(val[d2c] == 1.0)
+    # p = 2: the reference divides by zero (degree-0 Greville); the oracle uses
+    # the uniform grid that p = 3 produces
+    n2 = O.fem_nodes(2, 2, 6, 6).reshape(-1, 2)
+    assert np.all(np.isfinite(n2)) and np.allclose(n2[:8, 0], np.arange(8) / 7)
+
+
+# ---------------------------------------------------------------- inverse LM
+def test_invlm_matches_reference_golden():
+    g = gold("invlm_p2_n6.npz")
+    p, n = int(g["p"]), int(g["nelem"])
+    nloc = (p + 1) ** 2
+    en, off, ei, bi = O.invlm(nloc, n * n, (n + p) ** 2, g["IEN"])
+    for a, b in ((en, "elemNum"), (off, "offset"), (ei, "elemIdx"), (bi, "baseIdx")):
+        assert np.array_equal(a, g[b])
+    assert en.sum() == n * n * nloc
+    IEN = g["IEN"].reshape(n * n, nloc)
+    assert np.all(IEN[ei, bi] == np.repeat(np.arange((n + p) ** 2), en))
+
+
+# ---------------------------------------------------------------- live reference
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("p,n", [(2, 9), (3, 8), (4, 7), (5, 7)])
+def test_oracle_matches_live_reference(p, n):
+    m = O.mesh_unit_square(p, n)
+    x = O.rng_normal(p, m.nfunc)
+    assert rel(O.apply_mf(m, x), O.ref_apply_mf(m, x)) < 1e-13
+    assert rel(O.apply_sf_ref(m, x), O.ref_apply_sf(m, x)) < 1e-13
+    assert rel(O.load(m, O.LOAD_REF_POLY), O.ref_load(m)) < 1e-13

@@ -1,0 +1,151 @@
+"""GPU tests of the FEM-preconditioner assembly and of the solver.
+
+FEM assembly: indices (COO rows/cols, CSR, elem2coo, dir2coo) bit-exact
+against the oracle and the reference golden vectors, values within 1e-13.
+Solver: converged solutions against a direct solve of the oracle's assembled
+FP64 matrix; FP32 + FP64 refinement against plain FP64; graph replay equal to
+eager launches; iteration-limit and invalid-option paths.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import igamf  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def fem_arrays(f):
+    t = f.to_torch()
+    return {k: v.cpu().numpy() for k, v in t.items()}
+
+
+@pytest.mark.parametrize("p,n", [(3, 16), (4, 12), (2, 20), (3, 37)])
+def test_fem_assembly_matches_oracle(p, n):
+    nf = n + p
+    nodes = igamf.fem_reference_nodes(p, p, n, n)
+    assert np.abs(nodes - O.fem_nodes(p, p, n, n)).max() < 1e-15
+    f = igamf.FemPC(nf, nf, nodes)
+    a = fem_arrays(f)
+    rows, cols, e2c, d2c, val = O.fem_assemble(nf, nf, nodes)
+    assert f.view.nnz == len(rows)
+    assert np.array_equal(a["coo_row"], rows)
+    assert np.array_equal(a["col"], cols)
+    assert np.array_equal(a["elem2coo"], e2c)
+    assert np.array_equal(a["dir2coo"], d2c)
+    assert np.abs(a["val"] - val).max() < 1e-13 * np.abs(val).max()
+    rp = a["rowptr"]
+    assert rp[0] == 0 and rp[-1] == len(rows)
+    assert np.array_equal(np.repeat(np.arange(nf * nf), np.diff(rp)), rows)
+
+
+@pytest.mark.parametrize("name", ["fem_p3_n16.npz", "fem_p4_n12.npz"])
+def test_fem_assembly_matches_reference_golden(name):
+    g = np.load(os.path.join(GOLD, name))
+    p, n = int(g["p"]), int(g["nelem"])
+    f = igamf.FemPC(n + p, n + p, g["nodes"])
+    a = fem_arrays(f)
+    assert np.array_equal(a["coo_row"], g["rows"])
+    assert np.array_equal(a["col"], g["cols"])
+    assert np.array_equal(a["elem2coo"], g["elem2coo"])
+    assert np.array_equal(a["dir2coo"], g["dir2coo"])
+    assert np.abs(a["val"] - g["val"]).max() < 1e-13
+
+
+def test_fem_assembly_is_deterministic():
+    p, n = 3, 64
+    nodes = igamf.fem_reference_nodes(p, p, n, n)
+    v0 = fem_arrays(igamf.FemPC(n + p, n + p, nodes))["val"]
+    v1 = fem_arrays(igamf.FemPC(n + p, n + p, nodes))["val"]
+    assert np.array_equal(v0, v1)
+
+
+def _direct(m, om, F):
+    A = O.assembled_matrix(om)
+    I = np.where(om.ID >= 0)[0]
+    u = np.zeros(om.nfunc)
+    u[I] = spla.spsolve(A[I][:, I].tocsc(), F[I])
+    return u
+
+
+def _oracle_mesh(m):
+    a = m.arrays()
+    return O.Mesh(a["p"], a["q"], a["nex"], a["ney"], a["CP"], a["W"], a["IEN"], a["ID"],
+                  a["Dir"], a["C1"], a["C2"], a["h1"], a["h2"])
+
+
+@pytest.mark.parametrize("geom", ["square", "annulus"])
+def test_pcg_solves(geom):
+    if geom == "square":
+        m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 2, 32)
+        kind, val = igamf.LOAD_SINSIN, 2 * np.pi ** 2
+    else:
+        m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 24)
+        kind, val = igamf.LOAD_CONST, 1.0
+    om = _oracle_mesh(m)
+    op = igamf.Operator(m)
+    F = op.load(kind, val)
+    u_ref = _direct(m, om, F.cpu().numpy())
+    for skind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        for asm in (igamf.INVLM_GATHER, igamf.IEN_ATOMIC):
+            s = igamf.Solver(op, None, kind=skind, pc=igamf.PC_JACOBI, rtol=1e-10, assembly=asm)
+            u, st = s.solve(F)
+            assert st.status == 0 and st.rel_residual <= 1e-10 * 1.0001, (skind, st.rel_residual)
+            err = np.abs(u.cpu().numpy() - u_ref).max() / np.abs(u_ref).max()
+            assert err < 1e-7, (skind, asm, err)
+    if geom == "square":
+        # reference sign: u_h ~ -sin(pi x) sin(pi y)
+        cp = m.arrays()["CP"].reshape(-1, 2)
+        assert u_ref[len(u_ref) // 2] < 0
+
+
+def test_graph_replay_equals_eager():
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 40)
+    op = igamf.Operator(m)
+    F = op.load(igamf.LOAD_CONST, 1.0)
+    for skind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        outs = []
+        for g in (0, 1, 1):
+            s = igamf.Solver(op, None, kind=skind, pc=igamf.PC_JACOBI, rtol=1e-9, use_graph=g)
+            u, st = s.solve(F)
+            outs.append((u.clone(), st.iterations))
+            u2, st2 = s.solve(F)  # replay the same solver
+            assert torch.equal(u, u2)
+        assert all(torch.equal(outs[0][0], o[0]) and outs[0][1] == o[1] for o in outs[1:])
+
+
+def test_fem_preconditioned_solve():
+    p, n = 3, 48
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, n)
+    op = igamf.Operator(m)
+    fem = igamf.FemPC(n + p, n + p, igamf.fem_reference_nodes(p, p, n, n))
+    F = op.load(igamf.LOAD_REF_POLY)
+    sj = igamf.Solver(op, None, pc=igamf.PC_JACOBI, rtol=1e-10)
+    uj, stj = sj.solve(F)
+    for kind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        sf = igamf.Solver(op, fem, kind=kind, pc=igamf.PC_FEM_CG, rtol=1e-10,
+                          pc_inner_maxit=200, pc_inner_rtol=1e-6)
+        uf, stf = sf.solve(F)
+        assert stf.status == 0 and stf.rel_residual <= 1.0001e-10
+        assert stf.iterations < stj.iterations / 2, (stf.iterations, stj.iterations)
+        assert (uf - uj).abs().max().item() < 1e-7 * uj.abs().max().item()
+
+
+def test_iteration_limit_and_bad_options():
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 2, 32)
+    op = igamf.Operator(m)
+    F = op.load(igamf.LOAD_REF_POLY)
+    s = igamf.Solver(op, None, pc=igamf.PC_JACOBI, rtol=1e-12, maxit=3)
+    with pytest.raises(igamf.IgaError) as e:
+        s.solve(F)
+    assert e.value.status == 11  # NOT_CONVERGED
+    with pytest.raises(igamf.IgaError) as e:
+        igamf.Solver(op, None, pc=igamf.PC_FEM_CG)  # FEM preconditioner without a matrix
+    assert e.value.status == 1
