@@ -107,7 +107,7 @@ __global__ void k_heads(const uint64_t *keys, int n, uint64_t sentinel, int *hea
   head[k] = (keys[k] != sentinel && (k == 0 || keys[k] != keys[k - 1])) ? 1 : 0;
 }
 
-// seg = inclusive_scan(head) - 1
+// seg = inclusive_scan(head): segment id + 1
 __global__ void k_scatter(const uint64_t *keys, const int *perm, const int *seg, const int *head,
                           int n, uint64_t sentinel, int64_t N, int64_t nelemc, int *rows,
                           int *cols, int *starts, int *elem2coo, int *dir2coo) {
@@ -118,7 +118,7 @@ __global__ void k_scatter(const uint64_t *keys, const int *perm, const int *seg,
     if (orig < nelemc) elem2coo[orig] = -1;
     return;
   }
-  const int s = seg[k];
+  const int s = seg[k] - 1;  // inclusive scan of the heads is 1-based
   if (head[k]) {
     rows[s] = (int)(keys[k] / (uint64_t)N);
     cols[s] = (int)(keys[k] % (uint64_t)N);

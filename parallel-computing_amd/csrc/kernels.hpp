@@ -20,6 +20,12 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+// 1: load all of an element's G_q into registers at kernel entry;
+// 0: load each quadrature row's factors where they are used
+#ifndef IGAK_PREFETCH_G
+#define IGAK_PREFETCH_G 1
+#endif
 #include <stdint.h>
 
 namespace igak {
@@ -77,115 +83,186 @@ __device__ __forceinline__ void atomic_add(T *p, T v) {
 }
 
 // --------------------------------------------------------------------------
-// element operators
+// G_q storage: per tile and component c (G11, G12, G22) a block of NQ2*NT
+// values; the first NQ2 - NQ2 % VW points are stored as VW-wide vectors per
+// thread ([chunk][t][VW], one 16-byte load each), the rest as [q][t].
+// --------------------------------------------------------------------------
+template <typename T, int NQ2>
+__device__ __forceinline__ size_t gq_index(int tile, int c, int q, int t, int NT) {
+  constexpr int VW = 16 / sizeof(T);
+  constexpr int NV = NQ2 / VW;
+  const size_t base = ((size_t)tile * 3 + c) * NQ2 * NT;
+  if (q < NV * VW) return base + ((size_t)(q / VW) * NT + t) * VW + q % VW;
+  return base + (size_t)NV * VW * NT + (size_t)(q - NV * VW) * NT + t;
+}
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename T, int NQ2>
+__device__ __forceinline__ void load_gq(const T *G, int tile, int t, int NT, T (&g)[3][NQ2]) {
+  constexpr int VW = 16 / sizeof(T);
+  constexpr int NV = NQ2 / VW;
+  typedef typename Vec16<T>::type V;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T *Gc = G + ((size_t)tile * 3 + c) * NQ2 * NT;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const V v = *reinterpret_cast<const V *>(Gc + ((size_t)k * NT + t) * VW);
+#pragma unroll
+      for (int l = 0; l < VW; ++l) g[c][k * VW + l] = v[l];
+    }
+#pragma unroll
+    for (int r = NV * VW; r < NQ2; ++r) g[c][r] = Gc[(size_t)NV * VW * NT + (size_t)(r - NV * VW) * NT + t];
+  }
+}
+
+// --------------------------------------------------------------------------
+// element operators.  Tables live in LDS as tb[component][lane]:
+// component i*NB+a is N(xi_i) of local function a, TAB + i*NB + a its
+// reference derivative; `lane` is the element column (x) or row (y) in the
+// tile.  X[b][a], ye[b][a]: b = row offset (eta), a = column offset (xi).
 // --------------------------------------------------------------------------
 
-// Sum-factorised element action: ye = -sum_q gradR^T G_q gradR xe, with the
-// 1D tables N (values) and D (reference derivatives) of the element column
-// (1) and row (2).  X[b][a], ye[b][a]: b = row offset (eta), a = column (xi).
-template <typename T, int P, int NQ>
+// Sum-factorised action ye = -sum_q gradR^T G_q gradR X.  The eta loop fuses
+// the forward eta contraction, the quadrature-point operator and the backward
+// eta contraction, so only one table row is live at a time.
+template <typename T, int P, int NQ, int LX, int LY, typename GL>
 __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
-                                           const T (&N1)[NQ][P + 1],
-                                           const T (&D1)[NQ][P + 1],
-                                           const T (&N2)[NQ][P + 1],
-                                           const T (&D2)[NQ][P + 1],
-                                           const T (&g11)[NQ * NQ],
-                                           const T (&g12)[NQ * NQ],
-                                           const T (&g22)[NQ * NQ],
+                                           const T (*tbx)[LX], const T (*tby)[LY],
+                                           int lx, int ly, const GL &gl,
                                            T (&ye)[P + 1][P + 1]) {
-  constexpr int NB = P + 1;
+  constexpr int NB = P + 1, TAB = NQ * NB;
   T t0[NB][NQ], t1[NB][NQ];
 #pragma unroll
-  for (int b = 0; b < NB; ++b)
+  for (int i = 0; i < NQ; ++i) {
+    T n1[NB], d1[NB];
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
+    for (int a = 0; a < NB; ++a) {
+      n1[a] = tbx[i * NB + a][lx];
+      d1[a] = tbx[TAB + i * NB + a][lx];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
       T s0 = T(0), s1 = T(0);
 #pragma unroll
       for (int a = 0; a < NB; ++a) {
-        s0 += N1[i][a] * X[b][a];
-        s1 += D1[i][a] * X[b][a];
+        s0 += n1[a] * X[b][a];
+        s1 += d1[a] * X[b][a];
       }
       t0[b][i] = s0;
       t1[b][i] = s1;
     }
-  T fx[NQ][NQ], fy[NQ][NQ];
-#pragma unroll
-  for (int j = 0; j < NQ; ++j)
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-      T ux = T(0), uy = T(0);
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        ux += N2[j][b] * t1[b][i];
-        uy += D2[j][b] * t0[b][i];
-      }
-      const int q = j * NQ + i;
-      fx[j][i] = -(g11[q] * ux + g12[q] * uy);
-      fy[j][i] = -(g12[q] * ux + g22[q] * uy);
-    }
+  }
   T s0[NB][NQ], s1[NB][NQ];
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-      T a0 = T(0), a1 = T(0);
+      s0[b][i] = T(0);
+      s1[b][i] = T(0);
+    }
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) {
-        a0 += N2[j][b] * fx[j][i];
-        a1 += D2[j][b] * fy[j][i];
+  for (int j = 0; j < NQ; ++j) {
+    T n2[NB], d2[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      n2[b] = tby[j * NB + b][ly];
+      d2[b] = tby[TAB + j * NB + b][ly];
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      T ux = T(0), uy = T(0);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        ux += n2[b] * t1[b][i];
+        uy += d2[b] * t0[b][i];
       }
-      s0[b][i] = a0;
-      s1[b][i] = a1;
+      const int q = j * NQ + i;
+      const T g11 = gl(0, q), g12 = gl(1, q), g22 = gl(2, q);
+      const T fx = -(g11 * ux + g12 * uy);
+      const T fy = -(g12 * ux + g22 * uy);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        s0[b][i] += n2[b] * fx;
+        s1[b][i] += d2[b] * fy;
+      }
     }
-#pragma unroll
-  for (int b = 0; b < NB; ++b)
-#pragma unroll
-    for (int a = 0; a < NB; ++a) {
-      T acc = T(0);
-#pragma unroll
-      for (int i = 0; i < NQ; ++i) acc += D1[i][a] * s0[b][i] + N1[i][a] * s1[b][i];
-      ye[b][a] = acc;
-    }
-}
-
-// Element-by-element action: per quadrature point the full reference
-// gradient of every local function (LocalAssemblyMF.cpp:31-67 shape).
-template <typename T, int P, int NQ>
-__device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
-                                            const T (&N1)[NQ][P + 1],
-                                            const T (&D1)[NQ][P + 1],
-                                            const T (&N2)[NQ][P + 1],
-                                            const T (&D2)[NQ][P + 1],
-                                            const T (&g11)[NQ * NQ],
-                                            const T (&g12)[NQ * NQ],
-                                            const T (&g22)[NQ * NQ],
-                                            T (&ye)[P + 1][P + 1]) {
-  constexpr int NB = P + 1;
+  }
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
     for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
 #pragma unroll
-  for (int j = 0; j < NQ; ++j)
+  for (int i = 0; i < NQ; ++i) {
+    T n1[NB], d1[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      n1[a] = tbx[i * NB + a][lx];
+      d1[a] = tbx[TAB + i * NB + a][lx];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int a = 0; a < NB; ++a) ye[b][a] += d1[a] * s0[b][i] + n1[a] * s1[b][i];
+  }
+}
+
+// Element-by-element action: per quadrature point the full reference
+// gradient of every local function (LocalAssemblyMF.cpp:31-67 shape).
+template <typename T, int P, int NQ, int LX, int LY, typename GL>
+__device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
+                                            const T (*tbx)[LX], const T (*tby)[LY],
+                                            int lx, int ly, const GL &gl,
+                                            T (&ye)[P + 1][P + 1]) {
+  constexpr int NB = P + 1, TAB = NQ * NB;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    T n2[NB], d2[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      n2[b] = tby[j * NB + b][ly];
+      d2[b] = tby[TAB + j * NB + b][ly];
+    }
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
+      T n1[NB], d1[NB];
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        n1[a] = tbx[i * NB + a][lx];
+        d1[a] = tbx[TAB + i * NB + a][lx];
+      }
       T ux = T(0), uy = T(0);
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int a = 0; a < NB; ++a) {
-          ux += (D1[i][a] * N2[j][b]) * X[b][a];
-          uy += (N1[i][a] * D2[j][b]) * X[b][a];
+          ux += (d1[a] * n2[b]) * X[b][a];
+          uy += (n1[a] * d2[b]) * X[b][a];
         }
       const int q = j * NQ + i;
-      const T fx = -(g11[q] * ux + g12[q] * uy);
-      const T fy = -(g12[q] * ux + g22[q] * uy);
+      const T g11 = gl(0, q), g12 = gl(1, q), g22 = gl(2, q);
+      const T fx = -(g11 * ux + g12 * uy);
+      const T fy = -(g12 * ux + g22 * uy);
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int a = 0; a < NB; ++a)
-          ye[b][a] += (D1[i][a] * N2[j][b]) * fx + (N1[i][a] * D2[j][b]) * fy;
+        for (int a = 0; a < NB; ++a) ye[b][a] += (d1[a] * n2[b]) * fx + (n1[a] * d2[b]) * fy;
     }
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -207,17 +284,15 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
   const int ex0 = tx * TX, ey0 = ty * TY;
   const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
 
-  // geometry factors of this element: issued first, consumed last
-  T g11[NQ2], g12[NQ2], g22[NQ2];
-  {
-    const T *G = static_cast<const T *>(A.G) + (size_t)tile * 3 * NQ2 * NT + t;
-#pragma unroll
-    for (int q = 0; q < NQ2; ++q) {
-      g11[q] = G[(size_t)(0 * NQ2 + q) * NT];
-      g12[q] = G[(size_t)(1 * NQ2 + q) * NT];
-      g22[q] = G[(size_t)(2 * NQ2 + q) * NT];
-    }
-  }
+  // geometry factors of this element
+#if IGAK_PREFETCH_G
+  T g[3][NQ2];
+  load_gq<T, NQ2>(static_cast<const T *>(A.G), tile, t, NT, g);
+  auto gl = [&](int c, int q) { return g[c][q]; };
+#else
+  const T *Gt = static_cast<const T *>(A.G);
+  auto gl = [&](int c, int q) { return Gt[gq_index<T, NQ2>(tile, c, q, t, NT)]; };
+#endif
   // x footprint
   {
     const T *x = static_cast<const T *>(A.x);
@@ -254,24 +329,15 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
 
   T ye[NB][NB];
   {
-    T X[NB][NB], N1[NQ][NB], D1[NQ][NB], N2[NQ][NB], D2[NQ][NB];
+    T X[NB][NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
-#pragma unroll
-    for (int i = 0; i < NQ; ++i)
-#pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        N1[i][a] = tbx[i * NB + a][lx];
-        D1[i][a] = tbx[TAB + i * NB + a][lx];
-        N2[i][a] = tby[i * NB + a][ly];
-        D2[i][a] = tby[TAB + i * NB + a][ly];
-      }
     if (EBE)
-      element_ebe<T, P, NQ>(X, N1, D1, N2, D2, g11, g12, g22, ye);
+      element_ebe<T, P, NQ, TX, TY>(X, tbx, tby, lx, ly, gl, ye);
     else
-      element_sf<T, P, NQ>(X, N1, D1, N2, D2, g11, g12, g22, ye);
+      element_sf<T, P, NQ, TX, TY>(X, tbx, tby, lx, ly, gl, ye);
     if (!valid) {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
@@ -400,13 +466,9 @@ __global__ void k_geometry(T *G, const double *tabx, const double *taby,
   for (int t = threadIdx.x; t < NT; t += blockDim.x) {
     const int tx = tile % ntx, ty = tile / ntx;
     const int ex = tx * TX + t % TX, ey = ty * TY + t / TX;
-    T *g = G + (size_t)tile * 3 * NQ2 * NT + t;
     if (ex >= nex || ey >= ney) {
-      for (int q = 0; q < NQ2; ++q) {
-        g[(size_t)(0 * NQ2 + q) * NT] = T(0);
-        g[(size_t)(1 * NQ2 + q) * NT] = T(0);
-        g[(size_t)(2 * NQ2 + q) * NT] = T(0);
-      }
+      for (int q = 0; q < NQ2; ++q)
+        for (int c = 0; c < 3; ++c) G[gq_index<T, NQ2>(tile, c, q, t, NT)] = T(0);
       continue;
     }
     double X[NB][NB], Y[NB][NB];
@@ -434,9 +496,9 @@ __global__ void k_geometry(T *G, const double *tabx, const double *taby,
         const int q = j * NQ + i;
         double s = wq[i] * wq[j] / det;
         if (kappa) s *= kappa[((size_t)ey * nex + ex) * NQ2 + q];
-        g[(size_t)(0 * NQ2 + q) * NT] = (T)(s * (xeta * xeta + yeta * yeta));
-        g[(size_t)(1 * NQ2 + q) * NT] = (T)(-s * (xxi * xeta + yxi * yeta));
-        g[(size_t)(2 * NQ2 + q) * NT] = (T)(s * (xxi * xxi + yxi * yxi));
+        G[gq_index<T, NQ2>(tile, 0, q, t, NT)] = (T)(s * (xeta * xeta + yeta * yeta));
+        G[gq_index<T, NQ2>(tile, 1, q, t, NT)] = (T)(-s * (xxi * xeta + yxi * yeta));
+        G[gq_index<T, NQ2>(tile, 2, q, t, NT)] = (T)(s * (xxi * xxi + yxi * yxi));
       }
   }
 }
@@ -527,7 +589,6 @@ __global__ void k_kappa_grad(double *out, const double *x, const double *g,
   const int ex = e % nex, ey = e / nex;
   const int NT = TX * TY;
   const int tile = (ey / TY) * ntx + ex / TX, t = (ey % TY) * TX + ex % TX;
-  const double *Gt = G0 + (size_t)tile * 3 * NQ2 * NT + t;
   const double *t1 = tabx + (size_t)ex * 2 * TAB;
   const double *t2 = taby + (size_t)ey * 2 * TAB;
   double xe[NB][NB], ge[NB][NB];
@@ -550,9 +611,9 @@ __global__ void k_kappa_grad(double *out, const double *x, const double *g,
           vy += deta * ge[b][a];
         }
       const int q = j * NQ + i;
-      const double a11 = Gt[(size_t)(0 * NQ2 + q) * NT];
-      const double a12 = Gt[(size_t)(1 * NQ2 + q) * NT];
-      const double a22 = Gt[(size_t)(2 * NQ2 + q) * NT];
+      const double a11 = G0[gq_index<double, NQ2>(tile, 0, q, t, NT)];
+      const double a12 = G0[gq_index<double, NQ2>(tile, 1, q, t, NT)];
+      const double a22 = G0[gq_index<double, NQ2>(tile, 2, q, t, NT)];
       out[(size_t)e * NQ2 + q] = -(ux * (a11 * vx + a12 * vy) + uy * (a12 * vx + a22 * vy));
     }
 }
@@ -584,15 +645,14 @@ __global__ void k_diag(double *diag, const double *G, const double *tabx,
       const double *t1 = tabx + (size_t)ex * 2 * TAB;
       const double *t2 = taby + (size_t)ey * 2 * TAB;
       const int tile = (ey / TY) * ntx + ex / TX, t = (ey % TY) * TX + ex % TX;
-      const double *Gt = G + (size_t)tile * 3 * NQ2 * NT + t;
       for (int j = 0; j < NQ; ++j)
         for (int i = 0; i < NQ; ++i) {
           const int q = j * NQ + i;
           const double dx = t1[TAB + i * NB + a] * t2[j * NB + b];
           const double dy = t1[i * NB + a] * t2[TAB + j * NB + b];
-          acc -= Gt[(size_t)(0 * NQ2 + q) * NT] * dx * dx +
-                 2.0 * Gt[(size_t)(1 * NQ2 + q) * NT] * dx * dy +
-                 Gt[(size_t)(2 * NQ2 + q) * NT] * dy * dy;
+          acc -= G[gq_index<double, NQ2>(tile, 0, q, t, NT)] * dx * dx +
+                 2.0 * G[gq_index<double, NQ2>(tile, 1, q, t, NT)] * dx * dy +
+                 G[gq_index<double, NQ2>(tile, 2, q, t, NT)] * dy * dy;
         }
     }
   }
