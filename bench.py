@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--assembly", choices=["gather", "atomic"], default="gather")
     ap.add_argument("--kernel", choices=["sf", "ebe"], default="sf")
     ap.add_argument("--detail", action="store_true", help="also time every apply variant")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 256 MiB device copy first (PMC byte-counter calibration)")
     return ap.parse_args()
 
 
@@ -83,6 +85,13 @@ def main():
 
     x64 = torch.from_numpy(igamf.rng_normal(0, nfunc)).cuda()
     x32 = x64.float()
+    if a.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # headline: FP32 SF apply
     med32, mean32, wall32 = time_applies(op, x32, a.steps, a.warmup, kern, asm)

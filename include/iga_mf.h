@@ -137,6 +137,8 @@ typedef struct {
   int want_f32;     /* keep FP32 G_q (default 1) */
   int tile_x_f64, tile_y_f64; /* element tile of the FP64 kernels, 0 = default */
   int tile_x_f32, tile_y_f32; /* element tile of the FP32 kernels, 0 = default */
+  int variant_f64, variant_f32; /* 0: one workgroup per tile; 1: persistent
+                                   workgroups prefetching the next tile (SF) */
 } iga_op_opts;
 
 typedef struct {
@@ -235,7 +237,8 @@ typedef enum {
 
 typedef enum {
   IGA_SOLVE_PCG_F64 = 0,        /* plain FP64 preconditioned CG */
-  IGA_SOLVE_PCG_F32_REFINE = 1  /* FP32 inner PCG under FP64 defect correction */
+  IGA_SOLVE_PCG_F32_REFINE = 1  /* FP32 PCG under FP64 defect correction
+                                   (reliable updates of the true residual) */
 } iga_solve_kind;
 
 typedef struct {
@@ -245,7 +248,8 @@ typedef struct {
   iga_assembly assembly;
   double rtol, abstol, divtol;  /* on ||f - A u||_2 (FP64) */
   int maxit;                    /* total inner iterations cap */
-  double inner_rtol;            /* FP32 inner solve reduction per outer step */
+  double inner_rtol;            /* F32_REFINE: residual reduction between
+                                   reliable (FP64) updates */
   int inner_maxit;
   int pc_inner_maxit;           /* FEM_CG: inner CG iteration cap */
   double pc_inner_rtol;         /* FEM_CG: inner CG tolerance */

@@ -188,7 +188,7 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
   }
   const bool prof = op->prof_on && op->prof_used < op->prof_cap;
   if (prof) IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used], s));
-  IGA_HIP_TRY(op->ops->apply(a, f64, ebe, tile, ntiles, s));
+  IGA_HIP_TRY(op->ops->apply(a, f64, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s));
   if (prof) {
     IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used + 1], s));
     ++op->prof_used;
@@ -354,6 +354,12 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   op->ops = ops;
   op->want64 = o.want_f64 != 0;
   op->want32 = o.want_f32 != 0;
+  op->variant64 = o.variant_f64;
+  op->variant32 = o.variant_f32;
+  if (op->variant64 < 0 || op->variant64 > 1 || op->variant32 < 0 || op->variant32 > 1) {
+    delete op;
+    return IGA_ERR_INVALID_ARGUMENT;
+  }
   auto fail = [&](int rc) {
     iga_op_destroy(op);
     return rc;

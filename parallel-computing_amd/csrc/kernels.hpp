@@ -402,6 +402,154 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
   }
 }
 
+// Persistent, software-pipelined form of k_apply (sum-factorised): a
+// workgroup walks tiles tile0, tile0 + G, ... and keeps the next tile's G_q
+// and x footprint loads in flight while it computes and reduces the current
+// tile, so HBM streaming does not stop during the compute and barrier phases.
+template <typename T, int P, int NQ, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
+  constexpr int NB = P + 1, NT = TX * TY, FX = TX + P, FY = TY + P;
+  constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY;
+  constexpr int NOWN = (NFP + NT - 1) / NT;
+  __shared__ T xs[NFP];
+  __shared__ T tbx[2 * TAB][TX];
+  __shared__ T tby[2 * TAB][TY];
+  __shared__ T buf[NB][NT];
+
+  if (A.skip && *A.skip) return;
+  const int t = threadIdx.x, lx = t % TX, ly = t / TX;
+  const T *Gp = static_cast<const T *>(A.G);
+  const T *x = static_cast<const T *>(A.x);
+  T *y = static_cast<T *>(A.y);
+  T *halo = static_cast<T *>(A.halo);
+
+  // this workgroup's tiles: start, start + G, ... (start XCD-contiguous)
+  const int G = gridDim.x;
+  int tile = xcd_tile(blockIdx.x, G);
+
+  T gn[3][NQ2];
+  T xn[NOWN];
+  auto prefetch = [&](int tl) {
+    load_gq<T, NQ2>(Gp, tl, t, NT, gn);
+    const int ex0 = (tl % A.ntx) * TX, ey0 = (tl / A.ntx) * TY;
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int k = t + o * NT;
+      T v = T(0);
+      if (k < NFP) {
+        const int gx = ex0 + k % FX, gy = ey0 + k / FX;
+        if (gx < A.nfx && gy < A.nfy) {
+          const int Ag = gy * A.nfx + gx;
+          v = x[Ag];
+          if (A.mask_in && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) v = T(0);
+        }
+      }
+      xn[o] = v;
+    }
+  };
+  if (tile < ntiles) prefetch(tile);
+  for (; tile < ntiles; tile += G) {
+    const int tx = tile % A.ntx, ty = tile / A.ntx;
+    const int ex0 = tx * TX, ey0 = ty * TY;
+    const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
+    T g[3][NQ2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < NQ2; ++q) g[c][q] = gn[c][q];
+    __syncthreads();  // previous tile's LDS reads are done
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int k = t + o * NT;
+      if (k < NFP) xs[k] = xn[o];
+    }
+    {
+      const T *tx_ = static_cast<const T *>(A.tabx);
+      const T *ty_ = static_cast<const T *>(A.taby);
+      for (int k = t; k < 2 * TAB * TX; k += NT) {
+        const int l = k % TX, c = k / TX;
+        int e = ex0 + l;
+        if (e > A.nex - 1) e = A.nex - 1;
+        tbx[c][l] = tx_[(size_t)e * 2 * TAB + c];
+      }
+      for (int k = t; k < 2 * TAB * TY; k += NT) {
+        const int l = k % TY, c = k / TY;
+        int e = ey0 + l;
+        if (e > A.ney - 1) e = A.ney - 1;
+        tby[c][l] = ty_[(size_t)e * 2 * TAB + c];
+      }
+    }
+    const int nxt = tile + G;
+    if (nxt < ntiles) prefetch(nxt);
+    __syncthreads();
+
+    T ye[NB][NB];
+    {
+      T X[NB][NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
+      auto gl = [&](int c, int q) { return g[c][q]; };
+      element_sf<T, P, NQ, TX, TY>(X, tbx, tby, lx, ly, gl, ye);
+      if (!valid) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
+      }
+    }
+    T acc[NOWN];
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) acc[o] = T(0);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b > 0) __syncthreads();
+#pragma unroll
+      for (int a = 0; a < NB; ++a) buf[a][t] = ye[b][a];
+      __syncthreads();
+#pragma unroll
+      for (int o = 0; o < NOWN; ++o) {
+        const int k = t + o * NT;
+        if (k < NFP) {
+          const int fx = k % FX, fy = k / FX;
+          const int el_y = fy - b;
+          if (el_y >= 0 && el_y < TY) {
+#pragma unroll
+            for (int a = 0; a < NB; ++a) {
+              const int el_x = fx - a;
+              if (el_x >= 0 && el_x < TX) acc[o] += buf[a][el_y * TX + el_x];
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int k = t + o * NT;
+      if (k >= NFP) continue;
+      const int fx = k % FX, fy = k / FX;
+      const int gx = ex0 + fx, gy = ey0 + fy;
+      if (gx >= A.nfx || gy >= A.nfy) continue;
+      int xlo, xhi, ylo, yhi;
+      touching_tiles<P>(gx, A.nex, TX, xlo, xhi);
+      touching_tiles<P>(gy, A.ney, TY, ylo, yhi);
+      const bool shared = (xlo != xhi) || (ylo != yhi);
+      const int Ag = gy * A.nfx + gx;
+      if (A.mask_out && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) {
+        if (!shared || A.mode == 0) y[Ag] = T(0);
+        continue;
+      }
+      if (!shared)
+        y[Ag] = acc[o];
+      else if (A.mode == 0)
+        atomic_add(&y[Ag], acc[o]);
+      else
+        halo[(size_t)tile * NFP + k] = acc[o];
+    }
+  }
+}
+
 // Seam functions (touched by more than one tile).  mode 0: zero them before
 // a scatter-add apply; mode 1: sum the halo slots of the touching tiles in
 // (tile row, tile column) order.

@@ -42,8 +42,40 @@ struct Launch {
     }
     return hipErrorInvalidValue;
   }
+  // persistent pipelined SF kernel: as many workgroups as fit at once
+  template <typename T, int TX, int TY>
+  static hipError_t apply_pf_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
+    static int resident = 0;
+    if (!resident) {
+      int dev = 0, cus = 0, per = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &per, reinterpret_cast<const void *>(&k_apply_pf<T, P, NQ, TX, TY>), TX * TY, 0) !=
+              hipSuccess)
+        return hipErrorUnknown;
+      resident = cus * (per > 0 ? per : 1);
+    }
+    const int grid = ntiles < resident ? ntiles : resident;
+    hipLaunchKernelGGL((k_apply_pf<T, P, NQ, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, a, ntiles);
+    return hipGetLastError();
+  }
+  template <typename T>
+  static hipError_t apply_pf_tile(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {
+    switch (tile) {
+      case TILE_32x32: return apply_pf_t<T, 32, 32>(a, ntiles, s);
+      case TILE_32x16: return apply_pf_t<T, 32, 16>(a, ntiles, s);
+      case TILE_16x16: return apply_pf_t<T, 16, 16>(a, ntiles, s);
+      case TILE_64x8: return apply_pf_t<T, 64, 8>(a, ntiles, s);
+    }
+    return hipErrorInvalidValue;
+  }
+  // variant: 0 one workgroup per tile, 1 persistent pipelined (SF only)
   static hipError_t apply(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles,
-                          hipStream_t s) {
+                          int variant, hipStream_t s) {
+    if (variant == 1 && !ebe)
+      return f64 ? apply_pf_tile<double>(a, tile, ntiles, s)
+                 : apply_pf_tile<float>(a, tile, ntiles, s);
     if (f64) return ebe ? apply_tile<double, true>(a, tile, ntiles, s)
                         : apply_tile<double, false>(a, tile, ntiles, s);
     return ebe ? apply_tile<float, true>(a, tile, ntiles, s)
@@ -111,7 +143,8 @@ struct Launch {
 
 // per-degree entry points (defined in kern_p<P>.hip)
 struct Ops {
-  hipError_t (*apply)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles, hipStream_t);
+  hipError_t (*apply)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles, int variant,
+                      hipStream_t);
   hipError_t (*seam64)(double *, const double *, const uint8_t *, int, int, int, int,
                        const Layout &, int, int, const int *, hipStream_t);
   hipError_t (*seam32)(float *, const float *, const uint8_t *, int, int, int, int,

@@ -33,6 +33,7 @@ struct iga_op {
   double *wq = nullptr;             // device Gauss weights on [0,1]
   igak::Layout L64{}, L32{};
   int tile64 = 0, tile32 = 0;       // igak::TileId
+  int variant64 = 0, variant32 = 0; // 0 per-tile workgroups, 1 persistent pipelined
   double *G64 = nullptr;            // kappa-weighted
   double *G0 = nullptr;             // geometry only (== G64 when kappa unset)
   float *G32 = nullptr;

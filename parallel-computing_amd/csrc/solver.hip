@@ -6,24 +6,36 @@
 // convergence on the unpreconditioned residual ||f - A u||_2 <=
 // max(rtol ||f||_2, abstol), divergence when it exceeds divtol ||f||_2.
 //
-// Everything stays on the device: the CG scalars live in device memory, each
-// reduction is a two-level (wavefront, then workgroup) sum whose last
-// workgroup folds the partials in index order (deterministic) and resets its
-// own counter (graph-replay safe), and every kernel returns at once when the
-// device "done" flag is set, so the host can enqueue -- or replay as a HIP
-// graph -- a fixed chunk of iterations and check convergence once per chunk.
+// Everything stays on the device.  A CG iteration is
+//   apply (Ap = A p) -> K1 partials of p.Ap
+//   -> K2 alpha = rz / sum(partials), u, r, z updates, partials of r.r, r.z
+//   -> [preconditioner kernels and their r.z partials when it is not local]
+//   -> K3 beta, convergence test, p = z + beta p.
+// Each kernel writes per-workgroup partial sums; the NEXT kernel folds them
+// in every workgroup, in the same fixed order, so all workgroups see the same
+// bitwise scalar without atomics, fences or counters (the per-XCD L2s make
+// device-scope fences expensive on MI355X).  Scalars that the next iteration
+// needs are written by workgroup 0 into slots no workgroup of the same kernel
+// reads.  Every kernel returns at once when the device "done" flag is set, so
+// the host enqueues -- or replays as a HIP graph -- a fixed chunk of
+// iterations and checks convergence once per chunk.
 //
 // Two solve kinds:
 //   PCG_F64         plain FP64 PCG;
-//   PCG_F32_REFINE  FP64 defect correction: r = f - A u in FP64, an FP32 PCG
-//                   solves A d = r / ||r|| to inner_rtol (FP32 operator,
-//                   FP32 vectors, FP64 dot-product accumulation), u += ||r|| d.
+//   PCG_F32_REFINE  FP32 PCG (FP32 operator and vectors, FP64 dot-product
+//                   accumulation) under FP64 defect correction by reliable
+//                   updates: each time its recursive residual has dropped by
+//                   inner_rtol, the FP32 iterate is folded into the FP64
+//                   solution u, r = f - A u is recomputed with the FP64
+//                   operator and the recursion continues from it with the
+//                   same search direction.
 // Preconditioners: Jacobi (diag A), or the reference's FEM preconditioner
 // applied by an inner CG + Jacobi on the Q1 matrix (a nonlinear
 // preconditioner: the outer iteration then uses the flexible, Polak-Ribiere
 // form of beta).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -36,15 +48,18 @@ struct iga_fem;  // defined in fem.hip; only its CSR view is used here
 
 namespace {
 
-constexpr int RB = 256;    // threads per reduction workgroup
-constexpr int RG = 1024;   // reduction workgroups (grid-stride)
+constexpr int RB = 256;    // threads per workgroup of the vector kernels
+constexpr int RG = 512;    // workgroups of the vector kernels (grid-stride)
 constexpr int CHUNK = 16;  // iterations per enqueued chunk / graph
 
 // scalar slots (doubles) of one CG instance
-enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_TOL2, S_DIV2, S_RR0, S_NS };
+enum { S_RZ = 0, S_RZ_OLD, S_RR, S_RR0, S_TOL2, S_DIV2, S_ALPHA, S_BETA, S_UPD2, S_NS };
 // integer slots
-enum { I_DONE = 0, I_ITER, I_MAXIT, I_CNT0, I_CNT1, I_NI };
-// I_DONE values: 0 running, 1 converged, 2 iteration limit, 3 diverged / breakdown
+enum { I_DONE = 0, I_ITER, I_NEXT, I_MAXIT, I_NI };
+// I_DONE values: 0 running, 1 converged, 2 iteration limit, 3 diverged /
+// breakdown, 4 residual reduced by the reliable-update factor (FP32 solve)
+// partial-sum slots (each RG doubles)
+enum { P_PAP = 0, P_RR, P_RZ, P_RZO, P_NP };
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -52,235 +67,289 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum over the workgroup; result valid in thread 0.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double sm[NV][RB / 64];
+// Sum over the workgroup, returned to every thread (fixed tree).
+__device__ __forceinline__ double block_sum_all(double v) {
+  __shared__ double sm[RB / 64];
+  __shared__ double res;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    v[k] = wave_sum(v[k]);
-    if (lane == 0) sm[k][w] = v[k];
-  }
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) sm[w] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int i = 0; i < RB / 64; ++i) s += sm[k][i];
-      v[k] = s;
-    }
+    double s = 0.0;
+    for (int i = 0; i < RB / 64; ++i) s += sm[i];
+    res = s;
   }
+  __syncthreads();
+  return res;
 }
 
-// Publish per-workgroup partials; returns true in thread 0 of the last
-// workgroup to arrive, which then owns the totals (summed in index order).
-template <int NV>
-__device__ __forceinline__ bool last_block(double (&v)[NV], double *part, int *cnt,
-                                           double (&tot)[NV]) {
-  __shared__ bool is_last;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&part[k * RG + blockIdx.x], v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const int prev = atomicAdd(cnt, 1);
-    is_last = (prev == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  if (threadIdx.x == 0) {
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int i = 0; i < (int)gridDim.x; ++i)
-        s += __hip_atomic_load(&part[k * RG + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      tot[k] = s;
-    }
-    *cnt = 0;
-  }
-  return threadIdx.x == 0;
+// Write this workgroup's partial sum of v.
+__device__ __forceinline__ void put_partial(double *slot, double v) {
+  v = block_sum_all(v);
+  if (threadIdx.x == 0) slot[blockIdx.x] = v;
 }
+
+// Fold nb partials written by an earlier kernel (same value in every
+// workgroup and thread).
+__device__ __forceinline__ double fold(const double *slot, int nb) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += RB) s += slot[i];
+  return block_sum_all(s);
+}
+
+__device__ __forceinline__ bool finite(double x) { return x == x && !isinf(x); }
+
+#define GRID_LOOP(i, n) \
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < (n); i += (int64_t)gridDim.x * RB)
 
 // ---------------------------------------------------------------- CG kernels
-// init: r = f (u = 0), z = M r (Jacobi or identity), p = z when z is local;
-// rr0 = |f|^2, rz = r.z, tolerances
+// init: u = 0, r = f*fscale, z = dinv r (or r), p = z when use_z;
+// partials r.r and r.z
 template <typename V, typename F>
 __global__ void k_cg_init(const F *f, double fscale, V *u, V *r, V *z, V *p, const double *dinv,
-                          int use_z, int64_t n, double *sc, int *ic, double *part, double rtol2,
-                          double abstol2, double divtol2, int maxit) {
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
-    const double ri = (double)f[i] * fscale;
+                          int use_z, int64_t n, double *part) {
+  double rr = 0.0, rz = 0.0;
+  GRID_LOOP(i, n) {
+    const V ri = (V)((double)f[i] * fscale);
     u[i] = V(0);
-    r[i] = (V)ri;
-    const double zi = dinv ? ri * dinv[i] : ri;
+    r[i] = ri;
+    const V zi = dinv ? (V)((double)ri * dinv[i]) : ri;
     if (use_z) {
-      z[i] = (V)zi;
-      p[i] = (V)zi;
+      z[i] = zi;
+      p[i] = zi;
     }
-    v[0] += (double)(V)ri * (double)(V)ri;
-    v[1] += (double)(V)ri * (double)(V)zi;
+    rr += (double)ri * (double)ri;
+    rz += (double)ri * (double)zi;
   }
-  block_sum(v);
-  double tot[2];
-  if (last_block(v, part, ic + I_CNT0, tot)) {
-    sc[S_RR0] = tot[0];
-    sc[S_RR] = tot[0];
-    sc[S_RZ] = tot[1];
-    sc[S_TOL2] = fmax(rtol2 * tot[0], abstol2);
-    sc[S_DIV2] = divtol2 * tot[0];
+  put_partial(part + P_RR * RG, rr);
+  put_partial(part + P_RZ * RG, rz);
+}
+
+// one workgroup: tolerances and the first r.z
+__global__ void k_cg_init_fin(const double *part, int nb, double *sc, int *ic, double rtol2,
+                              double abstol2, double divtol2, int maxit, double upd2) {
+  const double rr = fold(part + P_RR * RG, nb);
+  const double rz = fold(part + P_RZ * RG, nb);
+  if (threadIdx.x == 0) {
+    sc[S_RR0] = rr;
+    sc[S_RR] = rr;
+    sc[S_RZ] = rz;
+    sc[S_TOL2] = fmax(rtol2 * rr, abstol2);
+    sc[S_DIV2] = divtol2 * rr;
+    sc[S_UPD2] = upd2 * rr;
     ic[I_ITER] = 0;
+    ic[I_NEXT] = 0;
     ic[I_MAXIT] = maxit;
-    ic[I_DONE] = (tot[0] <= sc[S_TOL2]) ? 1 : 0;
-    if (!(tot[0] == tot[0]) || isinf(tot[0])) ic[I_DONE] = 3;
+    ic[I_DONE] = !finite(rr) ? 3 : (rr <= sc[S_TOL2] ? 1 : 0);
   }
 }
 
-// alpha = rz / (p . Ap)
+// K1: partials of p.Ap; bookkeeping for K3
 template <typename V>
-__global__ void k_cg_pap(const V *p, const V *Ap, int64_t n, double *sc, int *ic, double *part) {
-  if (ic[I_DONE]) return;
-  double v[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
-    v[0] += (double)p[i] * (double)Ap[i];
-  block_sum(v);
-  double tot[1];
-  if (last_block(v, part, ic + I_CNT0, tot)) {
-    sc[S_PAP] = tot[0];
-    const double a = sc[S_RZ] / tot[0];
-    sc[S_ALPHA] = a;
-    if (!(a == a) || isinf(a) || tot[0] == 0.0) ic[I_DONE] = 3;
+__global__ void k_cg_pap(const V *p, const V *Ap, int64_t n, double *part, double *sc, int *ic,
+                         const int *outer_done) {
+  if (ic[I_DONE] || (outer_done && *outer_done)) return;
+  double s = 0.0;
+  GRID_LOOP(i, n) s += (double)p[i] * (double)Ap[i];
+  put_partial(part + P_PAP * RG, s);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_RZ_OLD] = sc[S_RZ];
+    ic[I_NEXT] = ic[I_ITER] + 1;
   }
 }
 
-// u += alpha p; r -= alpha Ap; (rold = r before the update when flexible);
-// z = dinv r when the preconditioner is local; sums r.r and r.z
+// K2: alpha; u += alpha p; r -= alpha Ap (rold keeps r for the flexible
+// beta); z = dinv r when the preconditioner is local; partials of r.r (or
+// z.z with prec_norm) and r.z
 template <typename V>
 __global__ void k_cg_update(V *u, V *r, const V *p, const V *Ap, V *z, V *rold,
-                            const double *dinv, int local_pc, int64_t n, double *sc, int *ic,
-                            double *part) {
-  if (ic[I_DONE]) return;
-  const double a = sc[S_ALPHA];
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+                            const double *dinv, int local_pc, int prec_norm, int64_t n,
+                            double *part, double *sc, int *ic, int nb, const int *outer_done) {
+  if (ic[I_DONE] || (outer_done && *outer_done)) return;
+  const double pap = fold(part + P_PAP * RG, nb);
+  const double a = sc[S_RZ] / pap;
+  if (!finite(a) || pap == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ic[I_DONE] = 3;
+    return;
+  }
+  double rr = 0.0, rz = 0.0;
+  GRID_LOOP(i, n) {
     u[i] = (V)((double)u[i] + a * (double)p[i]);
     if (rold) rold[i] = r[i];
     const V ri = (V)((double)r[i] - a * (double)Ap[i]);
     r[i] = ri;
-    v[0] += (double)ri * (double)ri;
     if (local_pc) {
       const V zi = dinv ? (V)((double)ri * dinv[i]) : ri;
       z[i] = zi;
-      v[1] += (double)ri * (double)zi;
+      rz += (double)ri * (double)zi;
+      rr += prec_norm ? (double)zi * (double)zi : (double)ri * (double)ri;
+    } else {
+      rr += (double)ri * (double)ri;
     }
   }
-  block_sum(v);
-  double tot[2];
-  if (last_block(v, part, ic + I_CNT1, tot)) {
-    sc[S_RR] = tot[0];
-    const int it = ic[I_ITER] + 1;
+  put_partial(part + P_RR * RG, rr);
+  if (local_pc) put_partial(part + P_RZ * RG, rz);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHA] = a;
+}
+
+// after a preconditioner applied by its own kernels: partials of r.z and
+// (flexible) rold.z
+template <typename V>
+__global__ void k_cg_rz(const V *r, const V *z, const V *rold, int64_t n, double *part,
+                        const int *ic) {
+  if (ic[I_DONE]) return;
+  double rz = 0.0, ro = 0.0;
+  GRID_LOOP(i, n) {
+    rz += (double)r[i] * (double)z[i];
+    if (rold) ro += (double)rold[i] * (double)z[i];
+  }
+  put_partial(part + P_RZ * RG, rz);
+  if (rold) put_partial(part + P_RZO * RG, ro);
+}
+
+// K3: convergence test, beta, p = z + beta p
+template <typename V>
+__global__ void k_cg_pupdate(V *p, const V *z, int64_t n, const double *part, double *sc,
+                             int *ic, int nb, int flexible, const int *outer_done) {
+  if (ic[I_DONE] || (outer_done && *outer_done)) return;
+  const double rr = fold(part + P_RR * RG, nb);
+  const double rz = fold(part + P_RZ * RG, nb);
+  const double ro = flexible ? fold(part + P_RZO * RG, nb) : 0.0;
+  const int it = ic[I_NEXT];
+  int st = 0;
+  if (rr <= sc[S_TOL2]) st = 1;
+  else if (!finite(rr) || rr > sc[S_DIV2]) st = 3;
+  else if (it >= ic[I_MAXIT]) st = 2;
+  else if (rr <= sc[S_UPD2]) st = 4;
+  const double b = (rz - ro) / sc[S_RZ_OLD];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_RR] = rr;
+    sc[S_RZ] = rz;
+    sc[S_BETA] = b;
     ic[I_ITER] = it;
-    if (tot[0] <= sc[S_TOL2]) ic[I_DONE] = 1;
-    else if (!(tot[0] == tot[0]) || tot[0] > sc[S_DIV2]) ic[I_DONE] = 3;
-    else if (it >= ic[I_MAXIT]) ic[I_DONE] = 2;
-    if (local_pc) {
-      sc[S_BETA] = tot[1] / sc[S_RZ];
-      sc[S_RZ] = tot[1];
-    }
+    if (st) ic[I_DONE] = st;
   }
+  if (st && st != 4) return;  // a reliable update keeps the search direction
+  GRID_LOOP(i, n) p[i] = (V)((double)z[i] + b * (double)p[i]);
 }
 
-// for a preconditioner applied by its own kernels: beta from z (flexible form
-// z.(r - rold) when rold is given)
+// first preconditioner application of a non-local preconditioner:
+// partials of r.z and p = z
 template <typename V>
-__global__ void k_cg_rz(const V *r, const V *z, const V *rold, int64_t n, double *sc, int *ic,
-                        double *part) {
+__global__ void k_cg_rz0(const V *r, const V *z, V *p, int64_t n, double *part, const int *ic) {
   if (ic[I_DONE]) return;
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
-    v[0] += (double)r[i] * (double)z[i];
-    if (rold) v[1] += (double)rold[i] * (double)z[i];
-  }
-  block_sum(v);
-  double tot[2];
-  if (last_block(v, part, ic + I_CNT0, tot)) {
-    sc[S_BETA] = (tot[0] - tot[1]) / sc[S_RZ];
-    sc[S_RZ] = tot[0];
-  }
-}
-
-// rz after the first preconditioner application (no beta)
-template <typename V>
-__global__ void k_cg_rz0(const V *r, const V *z, V *p, int64_t n, double *sc, int *ic,
-                         double *part) {
-  if (ic[I_DONE]) return;
-  double v[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
-    v[0] += (double)r[i] * (double)z[i];
+  double s = 0.0;
+  GRID_LOOP(i, n) {
+    s += (double)r[i] * (double)z[i];
     p[i] = z[i];
   }
-  block_sum(v);
-  double tot[1];
-  if (last_block(v, part, ic + I_CNT0, tot)) sc[S_RZ] = tot[0];
+  put_partial(part + P_RZ * RG, s);
 }
 
-template <typename V>
-__global__ void k_cg_pupdate(V *p, const V *z, int64_t n, const double *sc, const int *ic) {
+__global__ void k_cg_rz0_fin(const double *part, int nb, double *sc, const int *ic) {
   if (ic[I_DONE]) return;
-  const double b = sc[S_BETA];
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
-    p[i] = (V)((double)z[i] + b * (double)p[i]);
+  const double rz = fold(part + P_RZ * RG, nb);
+  if (threadIdx.x == 0) sc[S_RZ] = rz;
+}
+
+// ---------------------------------------------------------------- reliable update
+// x64 += x32, x32 = 0
+__global__ void k_ru_accum(double *x64, float *x32, int64_t n) {
+  GRID_LOOP(i, n) {
+    x64[i] += (double)x32[i];
+    x32[i] = 0.f;
+  }
+}
+
+// r32 = r64 (true residual); z32 = dinv r32 when local; partials r.z
+__global__ void k_ru_reset(const double *r64, float *r32, float *z32, const double *dinv,
+                           int local_pc, int64_t n, double *part) {
+  double rz = 0.0;
+  GRID_LOOP(i, n) {
+    const float ri = (float)r64[i];
+    r32[i] = ri;
+    if (local_pc) {
+      const float zi = dinv ? (float)((double)ri * dinv[i]) : ri;
+      z32[i] = zi;
+      rz += (double)ri * (double)zi;
+    }
+  }
+  if (local_pc) put_partial(part + P_RZ * RG, rz);
+}
+
+__global__ void k_ru_clear(int *ic) {
+  if (threadIdx.x == 0) ic[I_DONE] = 0;
+}
+
+// partials of r.z after a non-local preconditioner
+__global__ void k_ru_rz(const float *r, const float *z, int64_t n, double *part) {
+  double rz = 0.0;
+  GRID_LOOP(i, n) rz += (double)r[i] * (double)z[i];
+  put_partial(part + P_RZ * RG, rz);
+}
+
+// restart the recursion from the true residual: rz, update threshold, done = 0
+__global__ void k_ru_fin(const double *part, int nb, double *sc, int *ic, double upd2) {
+  const double rz = fold(part + P_RZ * RG, nb);
+  if (threadIdx.x == 0) {
+    sc[S_RZ] = rz;
+    sc[S_UPD2] = upd2;
+    ic[I_DONE] = 0;
+  }
 }
 
 // ---------------------------------------------------------------- outer (FP64)
-// r = f - Au, returns |r|^2 and |f|^2 in sc_out[0], sc_out[1]
+// r = f - Au; partials |r|^2 and |f|^2
 __global__ void k_residual(const double *f, const double *Au, double *r, int64_t n,
-                           double *out, int *cnt, double *part) {
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+                           double *part) {
+  double rr = 0.0, ff = 0.0;
+  GRID_LOOP(i, n) {
     const double ri = f[i] - Au[i];
     r[i] = ri;
-    v[0] += ri * ri;
-    v[1] += f[i] * f[i];
+    rr += ri * ri;
+    ff += f[i] * f[i];
   }
-  block_sum(v);
-  double tot[2];
-  if (last_block(v, part, cnt, tot)) {
-    out[0] = tot[0];
-    out[1] = tot[1];
+  put_partial(part + P_RR * RG, rr);
+  put_partial(part + P_RZ * RG, ff);
+}
+
+__global__ void k_fold2(const double *part, int nb, double *out) {
+  const double a = fold(part + P_RR * RG, nb);
+  const double b = fold(part + P_RZ * RG, nb);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = b;
   }
 }
 
 // u += scale * d (FP32 correction into the FP64 iterate)
 __global__ void k_axpy_f32(double *u, const float *d, double scale, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
-    u[i] += scale * (double)d[i];
+  GRID_LOOP(i, n) u[i] += scale * (double)d[i];
 }
 
 __global__ void k_inv(double *d, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
-    d[i] = 1.0 / d[i];
+  GRID_LOOP(i, n) d[i] = 1.0 / d[i];
 }
 
-__global__ void k_dot(const double *a, const double *b, int64_t n, double *out, int *cnt,
-                      double *part) {
-  double v[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
-    v[0] += a[i] * b[i];
-  block_sum(v);
-  double tot[1];
-  if (last_block(v, part, cnt, tot)) out[0] = tot[0];
+__global__ void k_dot(const double *a, const double *b, int64_t n, double *part) {
+  double s = 0.0;
+  GRID_LOOP(i, n) s += a[i] * b[i];
+  put_partial(part + P_RR * RG, s);
+}
+
+__global__ void k_fold1(const double *part, int nb, double *out) {
+  const double a = fold(part + P_RR * RG, nb);
+  if (threadIdx.x == 0) out[0] = a;
 }
 
 // ---------------------------------------------------------------- FEM preconditioner
 // CSR SpMV y = K x (one thread per row; the Q1 rows hold at most 9 entries)
 template <typename V>
 __global__ void k_spmv(const int *rowptr, const int *col, const V *val, const V *x, V *y,
-                       int nrows, const int *skip) {
-  if (skip && *skip) return;
+                       int nrows, const int *skip, const int *skip2) {
+  if ((skip && *skip) || (skip2 && *skip2)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
   double s = 0.0;
@@ -300,72 +369,52 @@ __global__ void k_csr_diag_inv(const int *rowptr, const int *col, const double *
 
 template <typename V>
 __global__ void k_cast(const double *a, V *b, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
-    b[i] = (V)a[i];
+  GRID_LOOP(i, n) b[i] = (V)a[i];
 }
 
 // inner CG on the FEM matrix solving K z = r (z0 = 0, Jacobi, PETSc's default
-// preconditioned-norm test |B r_k| <= max(rtol |B r_0|, abstol)); its
-// scalars/flags are a second CG instance whose I_DONE is reset by init.
+// preconditioned-norm test |B r_k| <= max(rtol |B r_0|, abstol))
 template <typename V>
-__global__ void k_pc_init(const V *r, V *zz, V *rin, V *w, V *pp, const double *dinv,
-                          int64_t n, double *sc, int *ic, double *part, double rtol2,
-                          double abstol2, int maxit, const int *outer_done) {
+__global__ void k_pc_init(const V *r, V *zz, V *rin, V *w, V *pp, const double *dinv, int64_t n,
+                          double *part, const int *outer_done) {
   if (*outer_done) return;
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+  double rz = 0.0, ww = 0.0;
+  GRID_LOOP(i, n) {
     const double ri = (double)r[i];
     zz[i] = V(0);
     rin[i] = r[i];
     const V wi = (V)(ri * dinv[i]);
     w[i] = wi;
     pp[i] = wi;
-    v[0] += ri * (double)wi;
-    v[1] += (double)wi * (double)wi;
+    rz += ri * (double)wi;
+    ww += (double)wi * (double)wi;
   }
-  block_sum(v);
-  double tot[2];
-  if (last_block(v, part, ic + I_CNT0, tot)) {
-    sc[S_RZ] = tot[0];
-    sc[S_RR0] = tot[1];
-    sc[S_TOL2] = fmax(rtol2 * tot[1], abstol2);
+  put_partial(part + P_RR * RG, ww);
+  put_partial(part + P_RZ * RG, rz);
+}
+
+__global__ void k_pc_init_fin(const double *part, int nb, double *sc, int *ic, double rtol2,
+                              int maxit, const int *outer_done) {
+  if (*outer_done) return;
+  const double ww = fold(part + P_RR * RG, nb);
+  const double rz = fold(part + P_RZ * RG, nb);
+  if (threadIdx.x == 0) {
+    sc[S_RZ] = rz;
+    sc[S_RR0] = ww;
+    sc[S_TOL2] = rtol2 * ww;
+    sc[S_DIV2] = 1e300;
     ic[I_ITER] = 0;
+    ic[I_NEXT] = 0;
     ic[I_MAXIT] = maxit;
-    ic[I_DONE] = (tot[1] <= sc[S_TOL2]) ? 1 : 0;
+    ic[I_DONE] = (ww <= sc[S_TOL2]) ? 1 : 0;
   }
 }
 
-template <typename V>
-__global__ void k_pc_update(V *zz, V *rin, V *w, const V *pp, const V *q, const double *dinv,
-                            int64_t n, double *sc, int *ic, double *part) {
-  if (ic[I_DONE]) return;
-  const double a = sc[S_ALPHA];
-  double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
-    zz[i] = (V)((double)zz[i] + a * (double)pp[i]);
-    const double ri = (double)rin[i] - a * (double)q[i];
-    rin[i] = (V)ri;
-    const V wi = (V)(ri * dinv[i]);
-    w[i] = wi;
-    v[0] += ri * (double)wi;
-    v[1] += (double)wi * (double)wi;
-  }
-  block_sum(v);
-  double tot[2];
-  if (last_block(v, part, ic + I_CNT1, tot)) {
-    const int it = ic[I_ITER] + 1;
-    ic[I_ITER] = it;
-    sc[S_BETA] = tot[0] / sc[S_RZ];
-    sc[S_RZ] = tot[0];
-    if (tot[1] <= sc[S_TOL2] || it >= ic[I_MAXIT]) ic[I_DONE] = 1;
-  }
-}
-
-inline dim3 rgrid(int64_t n) {
+inline int nblocks(int64_t n) {
   int64_t b = (n + RB - 1) / RB;
   if (b > RG) b = RG;
   if (b < 1) b = 1;
-  return dim3((unsigned)b);
+  return (int)b;
 }
 
 }  // namespace
@@ -374,6 +423,7 @@ struct iga_solver {
   iga_op *op = nullptr;
   iga_solver_opts o{};
   int64_t n = 0;
+  int nb = 1;
   int device = 0;
   // FP64 state
   double *r = nullptr, *p = nullptr, *Ap = nullptr, *z = nullptr, *rold = nullptr;
@@ -382,23 +432,22 @@ struct iga_solver {
   // FP32 inner state
   float *d32 = nullptr, *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *z32 = nullptr,
         *rold32 = nullptr;
-  // scalars: [0..S_NS) main CG, [S_NS..2S_NS) inner PC CG, then outer slots
+  // scalars: [0, S_NS) main CG, [S_NS, 2 S_NS) inner PC CG, then outer slots
   double *sc = nullptr;
   int *ic = nullptr;
-  double *part = nullptr;
+  double *part = nullptr, *part_in = nullptr;
   // FEM preconditioner (CSR of the reference's Q1 matrix)
   int fem_rows = 0;
   const int *rowptr = nullptr, *col = nullptr;
   const double *val64 = nullptr;
   float *val32 = nullptr;
   double *fdinv = nullptr;
-  double *zz = nullptr, *rin = nullptr, *w = nullptr, *pp = nullptr, *q = nullptr;
-  float *zz32 = nullptr, *rin32 = nullptr, *w32 = nullptr, *pp32 = nullptr, *q32 = nullptr;
+  double *rin = nullptr, *w = nullptr, *pp = nullptr, *q = nullptr;
+  float *rin32 = nullptr, *w32 = nullptr, *pp32 = nullptr, *q32 = nullptr;
   std::vector<void *> allocs;
   int gen = -1;  // operator generation the Jacobi diagonal was built for
   // graphs of one chunk of iterations (FP64 PCG, FP32 inner PCG)
   hipGraphExec_t g64 = nullptr, g32 = nullptr;
-  hipStream_t gstream64 = nullptr, gstream32 = nullptr;
 };
 
 namespace {
@@ -415,27 +464,43 @@ int salloc(iga_solver *s, void **p, size_t bytes) {
 
 bool local_pc(const iga_solver *s) { return s->o.pc == IGA_PC_NONE || s->o.pc == IGA_PC_JACOBI; }
 
+#define LAUNCH(kern, grid, ...)                                              \
+  do {                                                                       \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(RB), 0, st, __VA_ARGS__);      \
+  } while (0)
+
 // one application of the FEM preconditioner z = K^-1 r (inner CG + Jacobi)
 template <typename V>
 int fem_pc(iga_solver *s, const V *r, V *z, V *rin, V *w, V *pp, V *q, const V *val,
            hipStream_t st) {
   double *sc = s->sc + S_NS;
   int *ic = s->ic + I_NI;
+  const int *od = s->ic + I_DONE;
   const int64_t n = s->n;
+  const int nb = s->nb;
   const double rt = s->o.pc_inner_rtol;
-  hipLaunchKernelGGL((k_pc_init<V>), rgrid(n), dim3(RB), 0, st, r, z, rin, w, pp, s->fdinv, n, sc,
-                     ic, s->part, rt * rt, 0.0, s->o.pc_inner_maxit,
-                     s->ic + I_DONE);
+  LAUNCH((k_pc_init<V>), nb, r, z, rin, w, pp, s->fdinv, n, s->part_in, od);
+  hipLaunchKernelGGL(k_pc_init_fin, dim3(1), dim3(RB), 0, st, s->part_in, nb, sc, ic, rt * rt,
+                     s->o.pc_inner_maxit, od);
   const int bs = 256;
   for (int it = 0; it < s->o.pc_inner_maxit; ++it) {
     hipLaunchKernelGGL((k_spmv<V>), dim3((s->fem_rows + bs - 1) / bs), dim3(bs), 0, st, s->rowptr,
-                       s->col, val, pp, q, s->fem_rows, ic + I_DONE);
-    hipLaunchKernelGGL((k_cg_pap<V>), rgrid(n), dim3(RB), 0, st, pp, q, n, sc, ic, s->part);
-    hipLaunchKernelGGL((k_pc_update<V>), rgrid(n), dim3(RB), 0, st, z, rin, w, pp, q, s->fdinv, n,
-                       sc, ic, s->part);
-    hipLaunchKernelGGL((k_cg_pupdate<V>), rgrid(n), dim3(RB), 0, st, pp, w, n, sc, ic);
+                       s->col, val, pp, q, s->fem_rows, ic + I_DONE, od);
+    LAUNCH((k_cg_pap<V>), nb, pp, q, n, s->part_in, sc, ic, od);
+    LAUNCH((k_cg_update<V>), nb, z, rin, pp, q, w, (V *)nullptr, s->fdinv, 1, 1, n, s->part_in,
+           sc, ic, nb, od);
+    LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od);
   }
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+}
+
+template <typename V>
+int apply_pc(iga_solver *s, bool f64, const V *r, V *z, hipStream_t st) {
+  if (f64)
+    return fem_pc<double>(s, (const double *)r, (double *)z, s->rin, s->w, s->pp, s->q, s->val64,
+                          st);
+  return fem_pc<float>(s, (const float *)r, (float *)z, s->rin32, s->w32, s->pp32, s->q32,
+                       s->val32, st);
 }
 
 // One CG iteration on vectors of type V (FP64 main solve or FP32 inner solve).
@@ -443,27 +508,23 @@ template <typename V>
 int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold,
                  hipStream_t st) {
   const int64_t n = s->n;
+  const int nb = s->nb;
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
   int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_cg_pap<V>), rgrid(n), dim3(RB), 0, st, p, Ap, n, s->sc, s->ic, s->part);
+  LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr);
   const bool loc = local_pc(s);
   const double *dinv = s->o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
-  hipLaunchKernelGGL((k_cg_update<V>), rgrid(n), dim3(RB), 0, st, u, r, p, Ap, z,
-                     loc ? (V *)nullptr : rold, dinv, loc ? 1 : 0, n, s->sc, s->ic, s->part);
+  LAUNCH((k_cg_update<V>), nb, u, r, p, Ap, z, loc ? (V *)nullptr : rold, dinv, loc ? 1 : 0, 0,
+         n, s->part, s->sc, s->ic, nb, (const int *)nullptr);
   if (!loc) {
-    if (f64)
-      rc = fem_pc<double>(s, (const double *)r, (double *)z, s->rin, s->w, s->pp, s->q, s->val64,
-                          st);
-    else
-      rc = fem_pc<float>(s, (const float *)r, (float *)z, s->rin32, s->w32, s->pp32, s->q32,
-                         s->val32, st);
+    rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_cg_rz<V>), rgrid(n), dim3(RB), 0, st, r, z, rold, n, s->sc, s->ic,
-                       s->part);
+    LAUNCH((k_cg_rz<V>), nb, r, z, rold, n, s->part, s->ic);
   }
-  hipLaunchKernelGGL((k_cg_pupdate<V>), rgrid(n), dim3(RB), 0, st, p, z, n, s->sc, s->ic);
+  LAUNCH((k_cg_pupdate<V>), nb, p, z, n, s->part, s->sc, s->ic, nb, loc ? 0 : 1,
+         (const int *)nullptr);
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
@@ -476,18 +537,17 @@ int cg_chunk(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hi
   return IGA_OK;
 }
 
-// Run a CG to completion: init must already have been enqueued on st.
+// Run a CG to completion: init (k_cg_init + k_cg_init_fin) must already have
+// been enqueued on st.
 template <typename V>
 int cg_run(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
            hipGraphExec_t *gexec, int *iters, int *done) {
   const int64_t n = s->n;
   if (!local_pc(s)) {
-    int rc = f64 ? fem_pc<double>(s, (const double *)r, (double *)z, s->rin, s->w, s->pp, s->q,
-                                  s->val64, st)
-                 : fem_pc<float>(s, (const float *)r, (float *)z, s->rin32, s->w32, s->pp32,
-                                 s->q32, s->val32, st);
+    int rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_cg_rz0<V>), rgrid(n), dim3(RB), 0, st, r, z, p, n, s->sc, s->ic, s->part);
+    LAUNCH((k_cg_rz0<V>), s->nb, r, z, p, n, s->part, s->ic);
+    hipLaunchKernelGGL(k_cg_rz0_fin, dim3(1), dim3(RB), 0, st, s->part, s->nb, s->sc, s->ic);
   }
   if (s->o.use_graph && !*gexec) {
     // capture one chunk; the graph reads every scalar from device memory
@@ -527,6 +587,44 @@ int cg_run(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipS
   return IGA_OK;
 }
 
+// Continue a CG whose device state was restarted by k_ru_fin.
+template <typename V>
+int cg_resume(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
+              hipGraphExec_t *gexec, int *iters, int *done) {
+  int hic[2];
+  for (;;) {
+    if (s->o.use_graph && *gexec) {
+      if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
+    } else {
+      int rc = cg_chunk<V>(s, f64, u, r, p, Ap, z, rold, st);
+      if (rc) return rc;
+    }
+    if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return IGA_ERR_HIP;
+    if (hic[I_DONE]) break;
+  }
+  *iters = hic[I_ITER];
+  *done = hic[I_DONE];
+  return IGA_OK;
+}
+
+// |f - A u|^2 and |f|^2 into h (host), r = f - A u
+int outer_residual(iga_solver *s, const double *f, const double *u, double *r, double *h,
+                   hipStream_t st) {
+  const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
+  const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
+  int rc = igai::apply_raw(s->op, true, ebe, gather, u, s->Au, 0, 1, st, nullptr);
+  if (rc) return rc;
+  double *osc = s->sc + 2 * S_NS;
+  LAUNCH(k_residual, s->nb, f, s->Au, r, s->n, s->part);
+  hipLaunchKernelGGL(k_fold2, dim3(1), dim3(RB), 0, st, s->part, s->nb, osc);
+  if (hipMemcpyAsync(h, osc, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return IGA_ERR_HIP;
+  return IGA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -542,7 +640,7 @@ void iga_solver_opts_default(iga_solver_opts *o) {
   o->abstol = 0.0;
   o->divtol = 1e4;
   o->maxit = 100000;
-  o->inner_rtol = 1e-4;
+  o->inner_rtol = 1e-1;
   o->inner_maxit = 100000;
   o->pc_inner_maxit = 20;
   o->pc_inner_rtol = 1e-2;
@@ -581,6 +679,7 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   s->op = op;
   s->o = o;
   s->n = op->nfunc;
+  s->nb = nblocks(s->n);
   s->device = op->device;
   auto fail = [&](int r) {
     iga_solver_destroy(s);
@@ -602,7 +701,8 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   SA(s->x64, n * 8);
   SA(s->sc, 64 * 8);
   SA(s->ic, 32 * 4);
-  SA(s->part, (size_t)4 * RG * 8);
+  SA(s->part, (size_t)P_NP * RG * 8);
+  SA(s->part_in, (size_t)P_NP * RG * 8);
   if (o.kind == IGA_SOLVE_PCG_F32_REFINE) {
     SA(s->d32, n * 4);
     SA(s->r32, n * 4);
@@ -622,7 +722,6 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     s->col = v.col;
     s->val64 = v.val;
     SA(s->fdinv, n * 8);
-    SA(s->zz, n * 8);
     SA(s->rin, n * 8);
     SA(s->w, n * 8);
     SA(s->pp, n * 8);
@@ -632,13 +731,12 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
                        v.rowptr, v.col, v.val, s->fdinv, v.nrows);
     if (o.kind == IGA_SOLVE_PCG_F32_REFINE) {
       SA(s->val32, (size_t)v.nnz * 4);
-      SA(s->zz32, n * 4);
       SA(s->rin32, n * 4);
       SA(s->w32, n * 4);
       SA(s->pp32, n * 4);
       SA(s->q32, n * 4);
-      hipLaunchKernelGGL((k_cast<float>), rgrid(v.nnz), dim3(RB), 0, nullptr, v.val, s->val32,
-                         (int64_t)v.nnz);
+      hipLaunchKernelGGL((k_cast<float>), dim3(nblocks(v.nnz)), dim3(RB), 0, nullptr, v.val,
+                         s->val32, (int64_t)v.nnz);
     }
   }
 #undef SA
@@ -654,31 +752,29 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   const int64_t n = s->n;
+  const int nb = s->nb;
   const iga_solver_opts &o = s->o;
   iga_solve_stats stt;
   std::memset(&stt, 0, sizeof(stt));
-  const bool gather = o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
-  const bool ebe = o.kernel == IGA_KERNEL_EBE;
   const bool loc = local_pc(s);
   const double *dinv = o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
-  double *osc = s->sc + 2 * S_NS;  // outer slots: |r|^2, |f|^2
-  int *ocnt = s->ic + 2 * I_NI;
   if (s->gen != s->op->gen) {
     // Jacobi of A for the operator's current G_q
     iga_op *op = s->op;
     if (op->ops->diag(s->dinv, op->G64, op->tabx64, op->taby64, op->dirmask, op->nex, op->ney,
                       op->nfx, op->nfy, op->L64, st) != hipSuccess)
       return IGA_ERR_HIP;
-    hipLaunchKernelGGL(k_inv, rgrid(n), dim3(RB), 0, st, s->dinv, n);
+    LAUNCH(k_inv, nb, s->dinv, n);
     s->gen = op->gen;
   }
   if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
   const auto t0 = std::chrono::steady_clock::now();
   int status = IGA_OK;
   if (o.kind == IGA_SOLVE_PCG_F64) {
-    hipLaunchKernelGGL((k_cg_init<double, double>), rgrid(n), dim3(RB), 0, st, f, 1.0, s->x64,
-                       s->r, s->z, s->p, dinv, loc ? 1 : 0, n, s->sc, s->ic, s->part,
-                       o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit);
+    LAUNCH((k_cg_init<double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, dinv, loc ? 1 : 0,
+           n, s->part);
+    hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
+                       o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit, 0.0);
     int iters = 0, done = 0;
     rc = cg_run<double>(s, true, s->x64, s->r, s->p, s->Ap, s->z, s->rold, st, &s->g64, &iters,
                         &done);
@@ -689,58 +785,72 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
     stt.outer_iterations = 1;
     status = done == 1 ? IGA_OK : (done == 2 ? IGA_ERR_NOT_CONVERGED : IGA_ERR_DIVERGED);
   } else {
-    // FP64 defect correction around an FP32 PCG
-    if (hipMemsetAsync(u, 0, n * 8, st) != hipSuccess) return IGA_ERR_HIP;
-    hipLaunchKernelGGL(k_residual, rgrid(n), dim3(RB), 0, st, f, u /* = 0 */, s->r, n, osc, ocnt,
-                       s->part);
+    // FP32 PCG with reliable updates: whenever its recursive residual has
+    // dropped by RU_FACTOR (or looks converged), the FP32 iterate is added
+    // to the FP64 solution, the true residual f - A u is formed with the FP64
+    // operator, and the recursion restarts from it keeping the search
+    // direction -- FP64 defect correction without losing the Krylov rate.
+    const double RU2 = o.inner_rtol * o.inner_rtol;
+    if (hipMemsetAsync(s->x64, 0, n * 8, st) != hipSuccess) return IGA_ERR_HIP;
     double h[2];
-    if (hipMemcpyAsync(h, osc, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return IGA_ERR_HIP;
+    rc = outer_residual(s, f, s->x64, s->r, h, st);
+    if (rc) return rc;
     const double ff = h[1];
     double rr = h[0];
     const double tol = std::fmax(o.rtol * std::sqrt(ff), o.abstol);
-    int total = 0, outer = 0;
+    const double tol2 = tol * tol;
+    int outer = 0, iters = 0, done = 0;
     status = IGA_ERR_NOT_CONVERGED;
-    for (;;) {
-      const double rn = std::sqrt(rr);
-      if (!(rn == rn) || rn > o.divtol * std::sqrt(ff)) { status = IGA_ERR_DIVERGED; break; }
-      if (rn <= tol) { status = IGA_OK; break; }
-      if (total >= o.maxit) break;
-      // inner: A32 d = r / |r|
-      const int imax = std::min(o.inner_maxit, o.maxit - total);
-      hipLaunchKernelGGL((k_cg_init<float, double>), rgrid(n), dim3(RB), 0, st, s->r, 1.0 / rn,
-                         s->d32, s->r32, s->z32, s->p32, dinv, loc ? 1 : 0, n, s->sc, s->ic,
-                         s->part, o.inner_rtol * o.inner_rtol, 0.0, 1e300, imax);
-      int iters = 0, done = 0;
-      rc = cg_run<float>(s, false, s->d32, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st, &s->g32,
-                         &iters, &done);
-      if (rc) return rc;
-      total += iters;
-      ++outer;
-      hipLaunchKernelGGL(k_axpy_f32, rgrid(n), dim3(RB), 0, st, u, s->d32, rn, n);
-      rc = igai::apply_raw(s->op, true, ebe, gather, u, s->Au, 0, 1, st, nullptr);
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_residual, rgrid(n), dim3(RB), 0, st, f, s->Au, s->r, n, osc, ocnt,
-                         s->part);
-      if (hipMemcpyAsync(h, osc, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
-        return IGA_ERR_HIP;
-      rr = h[0];
-      if (done == 3) { status = IGA_ERR_DIVERGED; break; }
+    if (!(rr == rr)) {
+      status = IGA_ERR_DIVERGED;
+    } else if (rr <= tol2) {
+      status = IGA_OK;
+    } else {
+      LAUNCH((k_cg_init<float, double>), nb, s->r, 1.0, s->d32, s->r32, s->z32, s->p32, dinv,
+             loc ? 1 : 0, n, s->part);
+      hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
+                         tol2 / rr, 0.0, o.divtol * o.divtol * ff / rr, o.maxit, RU2);
+      bool first = true;
+      for (;;) {
+        if (first) {
+          rc = cg_run<float>(s, false, s->d32, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
+                             &s->g32, &iters, &done);
+          first = false;
+        } else {
+          rc = cg_resume<float>(s, false, s->d32, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
+                                &s->g32, &iters, &done);
+        }
+        if (rc) return rc;
+        if (done == 3) { status = IGA_ERR_DIVERGED; break; }
+        // reliable update
+        ++outer;
+        LAUNCH(k_ru_accum, nb, s->x64, s->d32, n);
+        rc = outer_residual(s, f, s->x64, s->r, h, st);
+        if (rc) return rc;
+        rr = h[0];
+        if (!(rr == rr) || rr > o.divtol * o.divtol * ff) { status = IGA_ERR_DIVERGED; break; }
+        if (rr <= tol2) { status = IGA_OK; break; }
+        if (done == 2) break;
+        LAUNCH(k_ru_reset, nb, s->r, s->r32, s->z32, dinv, loc ? 1 : 0, n, s->part);
+        if (!loc) {
+          hipLaunchKernelGGL(k_ru_clear, dim3(1), dim3(64), 0, st, s->ic);
+          rc = apply_pc<float>(s, false, s->r32, s->z32, st);
+          if (rc) return rc;
+          LAUNCH(k_ru_rz, nb, s->r32, s->z32, n, s->part);
+        }
+        hipLaunchKernelGGL(k_ru_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
+                           RU2 * rr);
+      }
     }
-    stt.iterations = total;
+    if (hipMemcpyAsync(u, s->x64, n * 8, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return IGA_ERR_HIP;
+    stt.iterations = iters;
     stt.outer_iterations = outer;
   }
-  // final FP64 residual
-  rc = igai::apply_raw(s->op, true, ebe, gather, u, s->Au, 0, 1, st, nullptr);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_residual, rgrid(n), dim3(RB), 0, st, f, s->Au, s->Ap, n, osc, ocnt,
-                     s->part);
+  // final FP64 residual (into Ap as scratch)
   double h[2];
-  if (hipMemcpyAsync(h, osc, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    return IGA_ERR_HIP;
+  rc = outer_residual(s, f, u, s->Ap, h, st);
+  if (rc) return rc;
   const auto t1 = std::chrono::steady_clock::now();
   stt.rel_residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
   stt.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -753,12 +863,13 @@ int iga_dot_f64(const double *a, const double *b, int64_t n, double *out_host, v
   if (!a || !b || !out_host || n < 0) return IGA_ERR_INVALID_ARGUMENT;
   hipStream_t st = (hipStream_t)stream;
   double *buf = nullptr;
-  if (hipMallocAsync((void **)&buf, (RG + 8) * 8, st) != hipSuccess) return IGA_ERR_OUT_OF_MEMORY;
-  int *cnt = (int *)(buf + RG + 4);
-  if (hipMemsetAsync(cnt, 0, 4, st) != hipSuccess) return IGA_ERR_HIP;
-  hipLaunchKernelGGL(k_dot, rgrid(n), dim3(RB), 0, st, a, b, n, buf + RG, cnt, buf);
+  if (hipMallocAsync((void **)&buf, (size_t)(P_NP * RG + 8) * 8, st) != hipSuccess)
+    return IGA_ERR_OUT_OF_MEMORY;
+  const int nb = nblocks(n);
+  LAUNCH(k_dot, nb, a, b, n, buf);
+  hipLaunchKernelGGL(k_fold1, dim3(1), dim3(RB), 0, st, buf, nb, buf + P_NP * RG);
   double r = 0.0;
-  if (hipMemcpyAsync(&r, buf + RG, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipMemcpyAsync(&r, buf + P_NP * RG, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return IGA_ERR_HIP;
   (void)hipFreeAsync(buf, st);

@@ -50,7 +50,8 @@ class MeshParams(C.Structure):
 
 class OpOpts(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("device", "want_f64", "want_f32", "tile_x_f64",
-                                       "tile_y_f64", "tile_x_f32", "tile_y_f32")]
+                                       "tile_y_f64", "tile_x_f32", "tile_y_f32",
+                                       "variant_f64", "variant_f32")]
 
 
 class OpInfo(C.Structure):
@@ -232,7 +233,7 @@ class Operator:
     """The matrix-free operator y = A x on one GPU (iga_op)."""
 
     def __init__(self, mesh, device=0, want_f64=True, want_f32=True, tile_f64=None,
-                 tile_f32=None, kappa=None):
+                 tile_f32=None, kappa=None, variant_f64=0, variant_f32=0):
         import torch
         self.mesh = mesh
         o = OpOpts()
@@ -242,6 +243,7 @@ class Operator:
             o.tile_x_f64, o.tile_y_f64 = tile_f64
         if tile_f32:
             o.tile_x_f32, o.tile_y_f32 = tile_f32
+        o.variant_f64, o.variant_f32 = variant_f64, variant_f32
         desc = mesh.desc
         keep = None
         if kappa is not None:

@@ -72,6 +72,25 @@ def test_apply_matches_oracle(built, name, kw):
     assert np.all(y64[om.Dir] == 0.0)
 
 
+@pytest.mark.parametrize("tile", [(32, 16), (16, 16), (64, 8), (32, 32)])
+def test_pipelined_variant_and_tiles(tile):
+    """Persistent pipelined SF kernel and every compiled tile shape, on a
+    ragged mesh whose element counts are not multiples of the tile."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 77, 45)
+    om = oracle_mesh(m)
+    x = igamf.rng_normal(2, om.nfunc)
+    y_ref = O.apply_mf(om, x)
+    xd = torch.from_numpy(x).cuda()
+    for variant in (0, 1):
+        op = igamf.Operator(m, tile_f64=tile, tile_f32=tile, variant_f64=variant,
+                            variant_f32=variant)
+        for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
+            y64 = op.apply(xd, assembly=asm).cpu().numpy()
+            assert rel(y64, y_ref) < TOL64, (tile, variant, asm, rel(y64, y_ref))
+            y32 = op.apply(xd.float(), assembly=asm).double().cpu().numpy()
+            assert rel(y32, y_ref) < TOL32, (tile, variant, asm, rel(y32, y_ref))
+
+
 def test_gather_bitwise_reproducible(built):
     m, om, op = built("annulus_p3_n24", CASES[3][1])
     x = torch.from_numpy(igamf.rng_normal(5, op.nfunc)).cuda()
