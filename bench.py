@@ -1,12 +1,25 @@
-"""Headline benchmark: matrix-free IGA Poisson apply (GDoF/s) and the
-CG solve-to-1e-8 time, FP32 apply + FP64 refinement vs plain FP64, on the
+"""Headline benchmark of the matrix-free IGA Poisson operator on one GPU.
+
+Headline (the JSON ``value``): the FP32 sum-factorised apply y = A x on the
 quarter annulus r in [1,2], exact NURBS, p = 3, 1024 x 1024 elements, 4 x 4
-Gauss points (1,050,625 interior unknowns), synthetic x ~ N(0,1) (seed 0).
+Gauss points (1,050,625 interior unknowns), x ~ N(0,1) (splitmix64 seed 0),
+operator and vectors resident in HBM; a step is one apply (inverse-LM gather
+assembly).  Alongside, in the same JSON line:
 
-A step is one FP32 sum-factorised apply y = A x (inverse-LM gather
-assembly) with x and the operator resident in HBM.  Prints one JSON line.
+  solve     CG to ||f - A u|| / ||f|| <= 1e-8, f = 1: plain FP64 PCG against
+            FP32 PCG with FP64 reliable updates (defect correction), Jacobi,
+            both assemblies;
+  block     multi-RHS apply, quarter annulus p = 4, 512 x 512, k = 32
+            (autotuned over k in {8, 16, 32});
+  c5        unit square p = 5, 2048 x 2048, perturbed interior points and
+            per-function weights U[0.8, 1.2]: FP32 SF and FP64 EBE applies and
+            the GPU FEM-preconditioner assembly;
+  c1 / c4   the plain method (p = 2, 32^2, FP64 EBE + FEM preconditioner) and
+            the kappa adjoint (p = 3, 256^2) with their checks;
+  cpu_baseline  the oracle's OpenMP restatement of the reference apply on the
+            host cores.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-solve] ...
 """
 import argparse
 import json
@@ -17,7 +30,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "parallel-computing_amd"))
 
-HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, vendor figure
+FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 (vector with packed math / MFMA)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_apply_f32_annulus_p3_1024.json")
 
 
 def parse():
@@ -28,6 +43,9 @@ def parse():
     ap.add_argument("--nelem", type=int, default=1024)
     ap.add_argument("--p", type=int, default=3)
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-block", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C1 and C4 checks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-applies", type=int, default=3)
     ap.add_argument("--assembly", choices=["gather", "atomic"], default="gather")
@@ -35,6 +53,8 @@ def parse():
     ap.add_argument("--detail", action="store_true", help="also time every apply variant")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB device copy first (PMC byte-counter calibration)")
+    ap.add_argument("--only-apply", action="store_true",
+                    help="headline apply only (for profiler runs)")
     return ap.parse_args()
 
 
@@ -43,46 +63,54 @@ def median(v):
     return v[len(v) // 2]
 
 
-def time_applies(op, x, steps, warmup, kernel, assembly):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ev_time(fn, steps, warmup):
+    """Median and mean of per-call HIP-event times (ms) on the current stream."""
     import torch
-    y = torch.empty_like(x)
     for _ in range(warmup):
-        op.apply(x, y, kernel=kernel, assembly=assembly)
+        fn()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(steps)]
-    t0 = time.perf_counter()
     for a, b in ev:
         a.record()
-        op.apply(x, y, kernel=kernel, assembly=assembly)
+        fn()
         b.record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     per = [a.elapsed_time(b) for a, b in ev]
-    return median(per), sum(per) / len(per), wall
+    return median(per), sum(per) / len(per)
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path "
-              f"(--gpus {a.gpus} requested). Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def kernel_avg_ms(op, x, y, steps, kern, asm):
+    """Average duration of the apply kernel itself: HIP events recorded by
+    the library around each k_apply launch on its stream."""
+    import torch
+    op.profile(True, steps)
+    for _ in range(steps):
+        op.apply(x, y, kernel=kern, assembly=asm)
+    torch.cuda.synchronize()
+    cnt, tot, kmin, kmax = op.profile_read()
+    op.profile(False)
+    return tot / max(cnt, 1), kmin
+
+
+def headline(a, out):
     import torch
     import igamf
-
     p, n = a.p, a.nelem
-    t_setup0 = time.perf_counter()
+    t0 = time.perf_counter()
     mesh = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, p, n, r0=1.0, r1=2.0)
     op = igamf.Operator(mesh)
-    setup_s = time.perf_counter() - t_setup0
+    setup_s = time.perf_counter() - t0
     info = op.info
     nfunc = op.nfunc
     n_int = (info.nfx - 2) * (info.nfy - 2)
     E, nqp = op.nelem, op.nqp
     kern = igamf.SF if a.kernel == "sf" else igamf.EBE
     asm = igamf.INVLM_GATHER if a.assembly == "gather" else igamf.IEN_ATOMIC
-
     x64 = torch.from_numpy(igamf.rng_normal(0, nfunc)).cuda()
     x32 = x64.float()
     if a.calibrate_copy:
@@ -92,23 +120,25 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-
-    # headline: FP32 SF apply
-    med32, mean32, wall32 = time_applies(op, x32, a.steps, a.warmup, kern, asm)
-    # the dominant kernel alone (HIP events around k_apply on its stream)
-    op.profile(True, a.steps)
-    y = torch.empty_like(x32)
-    for _ in range(a.steps):
-        op.apply(x32, y, kernel=kern, assembly=asm)
-    torch.cuda.synchronize()
-    cnt, tot, kmin, kmax = op.profile_read()
-    op.profile(False)
-    k_avg_ms = tot / max(cnt, 1)
+    y32 = torch.empty_like(x32)
+    med32, mean32 = ev_time(lambda: op.apply(x32, y32, kernel=kern, assembly=asm),
+                            a.steps, a.warmup)
+    k_avg, k_min = kernel_avg_ms(op, x32, y32, a.steps, kern, asm)
     alg_bytes32 = 4 * (2 * nfunc) + 4 * 3 * E * nqp
-    achieved = alg_bytes32 / (k_avg_ms * 1e-3) / 1e9
+    achieved = alg_bytes32 / (k_avg * 1e-3) / 1e9
     gdofs = n_int / (med32 * 1e-3) / 1e9
-
-    out = {
+    variant = info.variant_f32
+    kname = (f"k_apply_pf<float, {p}, {p + 1}, {info.tile_x_f32}, {info.tile_y_f32}>"
+             if variant == 1 and a.kernel == "sf" else
+             f"k_apply<float, {p}, {p + 1}, {info.tile_x_f32}, {info.tile_y_f32}, "
+             f"{'true' if a.kernel == 'ebe' else 'false'}, false>")
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("kernel") == kname and tr.get("nelem") == n and tr.get("p") == p:
+            traffic = tr.get("bytes_per_launch")
+    out.update({
         "metric": "MF apply GDoF/s & CG solve-to-1e-8 time: FP32+refine vs FP64, p=3 1024² elems",
         "value": round(gdofs, 4),
         "unit": "GDoF/s",
@@ -120,87 +150,286 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: x ~ N(0,1) splitmix64 seed 0; geometry generated (exact NURBS quarter annulus)",
+        "data": "synthetic: x ~ N(0,1) splitmix64 seed 0; exact NURBS quarter annulus generated "
+                "on the host (no checkpoint or dataset)",
         "config": {"workload": f"quarter annulus r in [1,2], p={p}, {n}x{n} elements, "
                                f"{p + 1}x{p + 1} Gauss, FP32 sum-factorised apply, "
                                f"{a.assembly} assembly",
                    "p": p, "nelem": n, "interior_dofs": n_int, "control_points": nfunc,
                    "kernel": a.kernel, "assembly": a.assembly,
-                   "tile_f32": [info.tile_x_f32, info.tile_y_f32],
-                   "tile_f64": [info.tile_x_f64, info.tile_y_f64]},
+                   "tile_f32": [info.tile_x_f32, info.tile_y_f32], "variant_f32": variant,
+                   "tile_f64": [info.tile_x_f64, info.tile_y_f64],
+                   "variant_f64": info.variant_f64},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_apply<float,3,4,...>",
-                     "kernel_avg_ms": round(k_avg_ms, 5), "kernel_min_ms": round(kmin, 5),
+                     "traffic": traffic, "kernel": kname,
+                     "kernel_avg_ms": round(k_avg, 5), "kernel_min_ms": round(k_min, 5),
                      "algorithmic_bytes": alg_bytes32},
         "apply": {"f32_median_ms": round(med32, 5), "f32_mean_ms": round(mean32, 5),
                   "setup_s": round(setup_s, 3), "op_setup_ms": round(info.setup_ms, 2),
                   "device_bytes": info.device_bytes},
-    }
+    })
+    log(f"headline f32 apply {med32 * 1e3:.1f} us, kernel {k_avg * 1e3:.1f} us, "
+        f"{achieved / HBM_PEAK_GBS:.3f} of HBM")
+    return mesh, op, x64, x32, kern, asm, n_int
 
-    # FP64 apply
-    med64, mean64, _ = time_applies(op, x64, a.steps, a.warmup, kern, asm)
+
+def fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, nfunc, E, nqp):
+    import torch
+    import igamf
+    y64 = torch.empty_like(x64)
+    med64, _ = ev_time(lambda: op.apply(x64, y64, kernel=kern, assembly=asm), a.steps, a.warmup)
     alg_bytes64 = 8 * (2 * nfunc) + 8 * 3 * E * nqp
     out["apply"]["f64_median_ms"] = round(med64, 5)
     out["apply"]["f64_gdofs"] = round(n_int / (med64 * 1e-3) / 1e9, 4)
-    out["apply"]["f64_hbm_frac_wall"] = round(alg_bytes64 / (med64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    out["apply"]["f64_hbm_frac"] = round(alg_bytes64 / (med64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     if a.detail:
         var = {}
         for kn, kv in (("sf", igamf.SF), ("ebe", igamf.EBE)):
             for an, av in (("gather", igamf.INVLM_GATHER), ("atomic", igamf.IEN_ATOMIC)):
                 for dn, xx in (("f32", x32), ("f64", x64)):
-                    m, _, _ = time_applies(op, xx, max(10, a.steps // 4), 3, kv, av)
+                    yy = torch.empty_like(xx)
+                    m, _ = ev_time(lambda: op.apply(xx, yy, kernel=kv, assembly=av),
+                                   max(10, a.steps // 4), 3)
                     var[f"{kn}_{an}_{dn}_ms"] = round(m, 5)
         out["apply"]["variants"] = var
 
-    # solves: plain FP64 PCG vs FP32 PCG under FP64 defect correction
-    if not a.no_solve:
-        F = op.load(igamf.LOAD_CONST, 1.0)
-        res = {}
-        sols = {}
-        for name, kind in (("fp64", igamf.SOLVE_PCG_F64), ("fp32_refine", igamf.SOLVE_PCG_F32_REFINE)):
-            s = igamf.Solver(op, None, kind=kind, pc=igamf.PC_JACOBI, rtol=1e-8,
-                             inner_rtol=1e-4, maxit=200000)
-            u, st = s.solve(F)  # warm (graph capture)
-            u, st = s.solve(F)
-            torch.cuda.synchronize()
-            res[name] = {"solve_ms": round(st.solve_ms, 2), "iterations": st.iterations,
-                         "outer": st.outer_iterations, "rel_residual": st.rel_residual,
-                         "status": st.status}
-            sols[name] = u
-        d = (sols["fp32_refine"] - sols["fp64"]).norm().item() / sols["fp64"].norm().item()
-        res["solution_rel_diff"] = d
-        res["speedup_fp32_refine_vs_fp64"] = round(res["fp64"]["solve_ms"] /
-                                                   res["fp32_refine"]["solve_ms"], 3)
-        res["preconditioner"] = "jacobi"
-        out["solve"] = res
 
-    # CPU baseline: the oracle's restatement of the reference apply on host cores
-    if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        arr = mesh.arrays()
-        om = O.Mesh(arr["p"], arr["q"], arr["nex"], arr["ney"], arr["CP"], arr["W"], arr["IEN"],
-                    arr["ID"], arr["Dir"], arr["C1"], arr["C2"], arr["h1"], arr["h2"])
-        xh = x64.cpu().numpy()
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ts = []
-        for _ in range(a.cpu_applies):
-            t0 = time.perf_counter()
-            yc, used = O.apply_mf_colored(om, xh, threads)
-            ts.append(time.perf_counter() - t0)
-        tc = median(ts)
-        out["cpu_baseline"] = {"value": round(n_int / tc / 1e9, 6), "unit": "GDoF/s",
-                               "cores": used, "kind": "port",
-                               "sample": f"{a.cpu_applies} full FP64 applies of the same "
-                                         f"{n}x{n} p={p} mesh (median {tc:.3f} s), OpenMP "
-                                         f"over a (p+1)^2 element colouring"}
-        # the GPU FP64 apply agrees with it
-        y64 = op.apply(x64, kernel=kern, assembly=asm).cpu().numpy()
-        import numpy as np
-        out["cpu_baseline"]["gpu_f64_rel_diff"] = float(np.abs(y64 - yc).max() /
-                                                        np.abs(yc).max())
-    print(json.dumps(out))
+def solves(out, op):
+    """Plain FP64 PCG against FP32 PCG + FP64 reliable updates, Jacobi, both
+    assemblies, f = 1, to 1e-8 relative residual (FP64 residual check)."""
+    import torch
+    import igamf
+    F = op.load(igamf.LOAD_CONST, 1.0)
+    res = {}
+    sols = {}
+    for an, av in (("gather", igamf.INVLM_GATHER), ("atomic", igamf.IEN_ATOMIC)):
+        for name, kind in (("fp64", igamf.SOLVE_PCG_F64),
+                           ("fp32_refine", igamf.SOLVE_PCG_F32_REFINE)):
+            s = igamf.Solver(op, None, kind=kind, pc=igamf.PC_JACOBI, rtol=1e-8,
+                             assembly=av, maxit=200000)
+            s.solve(F)  # first call captures the CG chunk graph
+            ts = []
+            for _ in range(3):
+                u, st = s.solve(F)
+                ts.append(st.solve_ms)
+            torch.cuda.synchronize()
+            res[f"{name}_{an}"] = {"solve_ms": round(median(ts), 2), "iterations": st.iterations,
+                                   "reliable_updates": st.outer_iterations,
+                                   "rel_residual": st.rel_residual, "status": st.status}
+            sols[f"{name}_{an}"] = u
+            log(f"solve {name} {an}: {median(ts):.1f} ms, {st.iterations} its")
+    ref = sols["fp64_gather"]
+    for k in ("fp32_refine_gather", "fp64_atomic", "fp32_refine_atomic"):
+        res[f"{k}_solution_rel_diff"] = (sols[k] - ref).norm().item() / ref.norm().item()
+    for an in ("gather", "atomic"):
+        res[f"speedup_fp32_refine_vs_fp64_{an}"] = round(
+            res[f"fp64_{an}"]["solve_ms"] / res[f"fp32_refine_{an}"]["solve_ms"], 3)
+    res["preconditioner"] = "jacobi"
+    res["inner_rtol"] = 0.1
+    out["solve"] = res
+
+
+def block(out):
+    """Multi-RHS apply, quarter annulus p = 4, 512^2, 5 x 5 Gauss; RHS j is
+    sum_{m,n<=8} a_mn sin(m pi xi) sin(n pi eta) at the control points'
+    parameters (Greville abscissae), a_mn ~ N(0,1) from seed j."""
+    import numpy as np
+    import torch
+    import igamf
+    p, ne, k = 4, 512, 32
+    mesh = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, p, ne, r0=1.0, r1=2.0)
+    op = igamf.Operator(mesh)
+    n = op.nfunc
+    nf = ne + p
+    # Greville abscissae of the open uniform knot vector on [0, 1]
+    knots = np.concatenate([np.zeros(p), np.linspace(0, 1, ne + 1), np.ones(p)])
+    gr = np.array([knots[i + 1:i + p + 1].mean() for i in range(nf)])
+    m_ = np.arange(1, 9)
+    S = np.sin(np.pi * np.outer(gr, m_))          # [nf][8]
+    X = np.empty((k, n))
+    for j in range(k):
+        amn = igamf.rng_normal(j, 64).reshape(8, 8)
+        X[j] = (S @ amn @ S.T).T.reshape(-1)      # row eta, column xi
+    X = torch.from_numpy(X).cuda().float()
+    Y = torch.empty_like(X)
+    x0 = X[0].contiguous()
+    t_single, _ = ev_time(lambda: op.apply(x0), 20, 3)
+    cfgs = igamf.block_configs()
+    tuned = {}
+    for kk in (8, 16, 32):
+        cfg, ms = op.tune_block(X[:kk], Y[:kk], reps=5)
+        tuned[kk] = {"cfg": list(cfgs[cfg]), "tuned_ms": round(ms, 4)}
+    t_block, _ = ev_time(lambda: op.apply_block(X, Y), 20, 3)
+    single = torch.stack([op.apply(X[j].contiguous(), assembly=igamf.IEN_ATOMIC)
+                          for j in range(k)])
+    rel = ((Y - single).abs().max() / single.abs().max()).item()
+    flops = (16 * (p + 1) ** 3 + 6 * (p + 1) ** 2) * op.nelem * k
+    tf = flops / (t_block * 1e-3) / 1e12
+    out["block"] = {
+        "workload": f"quarter annulus p={p}, {ne}x{ne}, {p + 1}x{p + 1} Gauss, k={k}, FP32",
+        "block_ms": round(t_block, 4), "per_rhs_us": round(t_block * 1e3 / k, 2),
+        "single_apply_us": round(t_single * 1e3, 2),
+        "speedup_vs_single": round(t_single * k / t_block, 3),
+        "tflops": round(tf, 2), "frac_fp32_peak": round(tf / FP32_PEAK_TFLOPS, 4),
+        "flops_per_call": flops, "tuned": tuned,
+        "config_used": list(cfgs[op.block_config(k)]),
+        "max_rel_diff_vs_single": rel,
+        "note": "VALU (non-packed FP32) kernel; the flop count is the sum-factorised "
+                "16(p+1)^3 + 6(p+1)^2 per element and rhs"}
+    log(f"block k={k}: {t_block * 1e3 / k:.1f} us/rhs, {tf:.1f} TFLOP/s")
+
+
+def c5(out):
+    """p = 5, 2048^2 unit square, interior points moved U[-0.1h, 0.1h] (seed
+    3), weights U[0.8, 1.2]: FP32 SF and FP64 EBE applies, FEM assembly."""
+    import torch
+    import igamf
+    p, ne = 5, 2048
+    t0 = time.perf_counter()
+    mesh = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, p, ne, perturb=0.1, w_lo=0.8,
+                            w_hi=1.2, seed=3)
+    op = igamf.Operator(mesh)
+    setup = time.perf_counter() - t0
+    info = op.info
+    n_int = (info.nfx - 2) * (info.nfy - 2)
+    x = torch.from_numpy(igamf.rng_normal(0, op.nfunc)).cuda()
+    x32 = x.float()
+    y32 = torch.empty_like(x32)
+    y64 = torch.empty_like(x)
+    t32, _ = ev_time(lambda: op.apply(x32, y32), 10, 2)
+    k32, _ = kernel_avg_ms(op, x32, y32, 10, igamf.SF, igamf.INVLM_GATHER)
+    t64e, _ = ev_time(lambda: op.apply(x, y64, kernel=igamf.EBE), 5, 1)
+    nc = 5 if info.rational == 2 else 3
+    b32 = 4 * (2 * op.nfunc) + 4 * nc * op.nelem * op.nqp + 4 * op.nfunc
+    nodes = igamf.fem_reference_nodes(p, p, ne, ne)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fem = igamf.FemPC(info.nfx, info.nfy, nodes)
+    torch.cuda.synchronize()
+    t_fem = time.perf_counter() - t0
+    out["c5"] = {
+        "workload": f"unit square p={p}, {ne}x{ne}, perturbed, weights U[0.8,1.2] "
+                    "(general rational kernels)",
+        "interior_dofs": n_int, "rational_mode": info.rational,
+        "f32_sf_ms": round(t32, 4), "f32_sf_gdofs": round(n_int / (t32 * 1e-3) / 1e9, 4),
+        "f32_kernel_ms": round(k32, 4),
+        "f32_hbm_frac": round(b32 / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "f32_algorithmic_bytes": b32,
+        "f64_ebe_ms": round(t64e, 4), "speedup_f32_sf_vs_f64_ebe": round(t64e / t32, 3),
+        "fem_assembly_ms": round(t_fem * 1e3, 2), "fem_nnz": fem.view.nnz,
+        "setup_s": round(setup, 2)}
+    log(f"c5 f32 sf {t32 * 1e3:.0f} us, f64 ebe {t64e * 1e3:.0f} us")
+
+
+def c1_c4(out):
+    import numpy as np
+    import torch
+    import igamf
+    from igamf import autograd as IA
+    # C1: FP64 EBE apply inside CG with the FEM (Q1) preconditioner, rtol 1e-10
+    p, ne = 2, 32
+    mesh = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, ne)
+    op = igamf.Operator(mesh)
+    x = torch.from_numpy(igamf.rng_normal(0, op.nfunc)).cuda()
+    y = torch.empty_like(x)
+    t100, _ = ev_time(lambda: op.apply(x, y, kernel=igamf.EBE), 100, 10)
+    fem = igamf.FemPC(ne + p, ne + p, igamf.fem_reference_nodes(p, p, ne, ne))
+    F = op.load(igamf.LOAD_SINSIN, 2 * np.pi ** 2)
+    s = igamf.Solver(op, fem, pc=igamf.PC_FEM_CG, kernel=igamf.EBE, rtol=1e-10,
+                     pc_inner_maxit=400, pc_inner_rtol=1e-10)
+    u, st = s.solve(F)
+    a = mesh.arrays()
+    cp = a["CP"].reshape(-1, 2)
+    # reference sign convention: u_h ~ -sin(pi x) sin(pi y) (Greville interpolant)
+    err = float(np.abs(u.cpu().numpy() + np.sin(np.pi * cp[:, 0]) * np.sin(np.pi * cp[:, 1]))
+                .max())
+    out["c1"] = {"workload": "unit square p=2 32x32, FP64 EBE apply, FEM-PC CG, rtol 1e-10",
+                 "apply_ms": round(t100, 5), "solve_ms": round(st.solve_ms, 2),
+                 "iterations": st.iterations, "rel_residual": st.rel_residual,
+                 "max_err_vs_minus_sinsin_at_cp": err}
+    # C4: kappa adjoint gradient against central differences
+    p, ne = 3, 256
+    mesh = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, ne)
+    op = igamf.Operator(mesh)
+    E, nqp = op.nelem, op.nqp
+    kap = torch.ones(E * nqp, dtype=torch.float64, device="cuda")
+    op.set_kappa(kap)
+    F = op.load(igamf.LOAD_SINSIN, 2 * np.pi ** 2)
+    s = igamf.Solver(op, None, pc=igamf.PC_JACOBI, rtol=1e-12, maxit=200000)
+    uobs = torch.from_numpy(igamf.rng_normal(2, op.nfunc)).cuda() * 1e-3
+    k = kap.clone().requires_grad_()
+    t0 = time.perf_counter()
+    u = IA.solve_kappa(s, F, k)
+    J = 0.5 * ((u - uobs) ** 2).sum()
+    J.backward()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    errs = []
+    with torch.no_grad():
+        for d in range(4):
+            dk = torch.from_numpy(igamf.rng_normal(100 + d, E * nqp)).cuda()
+            h = 1e-6
+            jp = 0.5 * ((IA.solve_kappa(s, F, kap + h * dk) - uobs) ** 2).sum().item()
+            jm = 0.5 * ((IA.solve_kappa(s, F, kap - h * dk) - uobs) ** 2).sum().item()
+            fd = (jp - jm) / (2 * h)
+            an = torch.dot(k.grad, dk).item()
+            errs.append(abs(fd - an) / max(abs(fd), 1e-300))
+    out["c4"] = {"workload": "unit square p=3 256x256, kappa = 1, J = 0.5|u - u_obs|^2",
+                 "forward_adjoint_grad_s": round(t_grad, 3),
+                 "fd_rel_err_max_4dirs": max(errs)}
+    log(f"c1 solve {st.iterations} its; c4 fd err {max(errs):.2e}")
+
+
+def cpu_baseline(a, out, mesh, x64, op, kern, asm, n_int):
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    arr = mesh.arrays()
+    om = O.Mesh(arr["p"], arr["q"], arr["nex"], arr["ney"], arr["CP"], arr["W"], arr["IEN"],
+                arr["ID"], arr["Dir"], arr["C1"], arr["C2"], arr["h1"], arr["h2"])
+    xh = x64.cpu().numpy()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ts = []
+    used = threads
+    yc = None
+    for _ in range(a.cpu_applies):
+        t0 = time.perf_counter()
+        yc, used = O.apply_mf_colored(om, xh, threads)
+        ts.append(time.perf_counter() - t0)
+    tc = median(ts)
+    out["cpu_baseline"] = {"value": round(n_int / tc / 1e9, 6), "unit": "GDoF/s",
+                           "cores": used, "kind": "port",
+                           "sample": f"{a.cpu_applies} full FP64 applies of the same "
+                                     f"{a.nelem}x{a.nelem} p={a.p} mesh (median {tc:.3f} s), "
+                                     f"OpenMP over a (p+1)^2 element colouring"}
+    y64 = op.apply(x64, kernel=kern, assembly=asm).cpu().numpy()
+    out["cpu_baseline"]["gpu_f64_rel_diff"] = float(np.abs(y64 - yc).max() / np.abs(yc).max())
+    log(f"cpu baseline {tc:.2f} s per apply on {used} threads")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {a.gpus} requested). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    out = {}
+    mesh, op, x64, x32, kern, asm, n_int = headline(a, out)
+    if not a.only_apply:
+        fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, op.nfunc, op.nelem, op.nqp)
+        if not a.no_solve:
+            solves(out, op)
+        if not a.no_cpu:
+            cpu_baseline(a, out, mesh, x64, op, kern, asm, n_int)
+        del op
+        if not a.no_block:
+            block(out)
+        if not a.no_c5:
+            c5(out)
+        if not a.no_extra:
+            c1_c4(out)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -135,18 +135,23 @@ typedef struct {
   int device;       /* HIP device ordinal */
   int want_f64;     /* keep FP64 G_q (default 1) */
   int want_f32;     /* keep FP32 G_q (default 1) */
-  int tile_x_f64, tile_y_f64; /* element tile of the FP64 kernels, 0 = default */
-  int tile_x_f32, tile_y_f32; /* element tile of the FP32 kernels, 0 = default */
+  int tile_x_f64, tile_y_f64; /* element tile of the FP64 kernels, 0 = default
+                                 (16x16); 32x16 for every degree, 32x32 and
+                                 64x8 for p <= 4; general rational meshes
+                                 16x16, and 32x16 for p <= 4 */
+  int tile_x_f32, tile_y_f32; /* element tile of the FP32 kernels, as above */
   int variant_f64, variant_f32; /* 0: one workgroup per tile; 1: persistent
-                                   workgroups prefetching the next tile (SF) */
+                                   workgroups prefetching the next tile (SF);
+                                   -1 (default): 1 for p <= 4, else 0 */
 } iga_op_opts;
 
 typedef struct {
   int nfunc, nfx, nfy, nelem, nqp, nloc, ndir;
-  int rational;               /* 0 unit weights, 1 tensor-product weights */
+  int rational;               /* 0 unit weights, 1 tensor-product weights, 2 general */
   int tile_x_f64, tile_y_f64, tile_x_f32, tile_y_f32;
   int64_t device_bytes;       /* device memory held by the operator */
   double setup_ms;            /* wall time of iga_op_create */
+  int variant_f64, variant_f32; /* apply variant in use (see iga_op_opts) */
 } iga_op_info;
 
 void iga_op_opts_default(iga_op_opts *o);
@@ -190,9 +195,26 @@ int iga_op_apply_backward(iga_op *op, const double *x, const double *g,
 
 /* ---- multi right-hand side ----------------------------------------------
  * Y[:,j] = A X[:,j] for k right-hand sides stored column-major with leading
- * dimension ld >= nfunc (FP32, device).  The 1D contractions run on MFMA. */
+ * dimension ld >= nfunc (FP32, device), Y overwritten.  No counterpart in the
+ * reference (one MatMult per vector there); the operator's G_q and tables
+ * are read once per element for each chunk of columns.  Seam sums use
+ * atomics, so the result is not bitwise reproducible (FP32 rounding order).
+ * Not available on general rational meshes (IGA_ERR_UNSUPPORTED).  Uses the
+ * configuration chosen by iga_op_tune_block for this k when there is one. */
 int iga_op_apply_block(iga_op *op, int k, int ld, const float *X, float *Y,
                        void *stream);
+/* Autotuner: times every compiled (element tile, rhs chunk) configuration
+ * valid for the degree and k on the given buffers (reps launches each, HIP
+ * events), keeps the fastest for this k and reports it. */
+int iga_op_tune_block(iga_op *op, int k, int ld, const float *X, float *Y,
+                      void *stream, int reps, int *best_cfg, double *best_ms);
+/* Block configurations: count, shape (element tile, rhs chunk per
+ * workgroup), and the per-k choice (set cfg = -1 to return k to the
+ * built-in default), so a tuning result can be stored and restored. */
+int iga_block_config_count(void);
+int iga_block_config_desc(int cfg, int *tile_x, int *tile_y, int *rhs_chunk);
+int iga_op_set_block_config(iga_op *op, int k, int cfg);
+int iga_op_get_block_config(const iga_op *op, int k, int *cfg);
 
 /* ---- FEM preconditioner (elem2coo -> coo2aij) ---------------------------
  * Q1 stiffness on the nfx x nfy node grid `nodes` ([nfx*nfy][2] host),
@@ -272,6 +294,16 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o,
 /* Solve A u = f; f, u FP64 device (u overwritten, zero initial guess). */
 int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
                      iga_solve_stats *stats);
+/* Block solve: A U[:,j] = F[:,j] for k FP64 columns (leading dimension ld >=
+ * nfunc, U overwritten, zero initial guesses).  No counterpart in the
+ * reference (one KSPSolve per right-hand side there).  Each column is an
+ * FP32 PCG with FP64 reliable updates (the solver's kind must be
+ * IGA_SOLVE_PCG_F32_REFINE, preconditioner Jacobi or none); all columns
+ * advance together on the multi-RHS apply, a converged column is frozen.
+ * stats: k entries (solve_ms is the whole block's) or NULL.  Returns
+ * IGA_OK or the status of the first column that failed. */
+int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double *U,
+                           void *stream, iga_solve_stats *stats);
 int iga_solver_destroy(iga_solver *s);
 
 /* ---- device vector helpers (FP64 / FP32, used by drivers and tests) ----- */

@@ -24,11 +24,18 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
 
 
 def sources():
+    """Translation units, longest compiles first (high-degree kernels)."""
     out = []
     for fn in sorted(os.listdir(SRC)):
         if fn.endswith(".hip") or fn.endswith(".cpp"):
             out.append(os.path.join(SRC, fn))
-    return out
+
+    def cost(path):
+        fn = os.path.basename(path)
+        if fn.startswith("kern_p"):
+            return -(int(fn[6]) + (2 if "_block" in fn else 0))
+        return 0
+    return sorted(out, key=cost)
 
 
 def compile_one(src):

@@ -14,6 +14,7 @@
 #include <new>
 #include <vector>
 
+#include "block.hpp"
 #include "op_internal.hpp"
 
 namespace igak {
@@ -165,6 +166,7 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
   a.taby = f64 ? (const void *)op->taby64 : (const void *)op->taby32;
   a.dirmask = op->dirmask;
   a.halo = f64 ? (void *)op->halo64 : (void *)op->halo32;
+  a.wf = f64 ? (const void *)op->wts64 : (const void *)op->wts32;
   a.nex = op->nex;
   a.ney = op->ney;
   a.nfx = op->nfx;
@@ -188,7 +190,10 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
   }
   const bool prof = op->prof_on && op->prof_used < op->prof_cap;
   if (prof) IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used], s));
-  IGA_HIP_TRY(op->ops->apply(a, f64, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s));
+  if (a.wf)
+    IGA_HIP_TRY(op->ops->apply_rat(a, f64, ebe, tile, ntiles, s));
+  else
+    IGA_HIP_TRY(op->ops->apply(a, f64, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s));
   if (prof) {
     IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used + 1], s));
     ++op->prof_used;
@@ -232,6 +237,8 @@ void iga_op_opts_default(iga_op_opts *o) {
   std::memset(o, 0, sizeof(*o));
   o->want_f64 = 1;
   o->want_f32 = 1;
+  o->variant_f64 = -1;
+  o->variant_f32 = -1;
 }
 
 int iga_op_destroy(iga_op *op) {
@@ -326,12 +333,15 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
       const double w00 = m->weights[0];
       for (int i = 0; i < nfx; ++i) wx[i] = m->weights[i];
       for (int j = 0; j < nfy; ++j) wy[j] = m->weights[(size_t)j * nfx] / w00;
-      for (int j = 0; j < nfy; ++j)
+      rational = 1;
+      for (int j = 0; j < nfy && rational == 1; ++j)
         for (int i = 0; i < nfx; ++i) {
           const double w = m->weights[(size_t)j * nfx + i];
-          if (std::fabs(w - wx[i] * wy[j]) > 1e-13 * w) return IGA_ERR_UNSUPPORTED;
+          if (std::fabs(w - wx[i] * wy[j]) > 1e-13 * w) {
+            rational = 2;  // not a tensor product: per-function weights
+            break;
+          }
         }
-      rational = 1;
     }
   }
 
@@ -350,13 +360,14 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   op->nloc = nloc;
   op->ndir = m->ndir;
   op->rational = rational;
+  op->nc = rational == 2 ? 5 : 3;
   op->ring = ring;
   op->ops = ops;
   op->want64 = o.want_f64 != 0;
   op->want32 = o.want_f32 != 0;
-  op->variant64 = o.variant_f64;
-  op->variant32 = o.variant_f32;
-  if (op->variant64 < 0 || op->variant64 > 1 || op->variant32 < 0 || op->variant32 > 1) {
+  op->variant64 = o.variant_f64 < 0 ? (p <= 4 ? 1 : 0) : o.variant_f64;
+  op->variant32 = o.variant_f32 < 0 ? (p <= 4 ? 1 : 0) : o.variant_f32;
+  if (o.variant_f64 < -1 || op->variant64 > 1 || o.variant_f32 < -1 || op->variant32 > 1) {
     delete op;
     return IGA_ERR_INVALID_ARGUMENT;
   }
@@ -370,24 +381,34 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   // tiles
   auto pick = [&](int tx, int ty, int dflt, int &id, igak::Layout &L) {
     id = (tx || ty) ? igak::tile_id(tx, ty) : dflt;
-    if (id < 0) return false;
+    if (id < 0 || !igak::tile_compiled(p, id)) return false;
     igak::tile_shape(id, L.tx, L.ty);
     if (L.tx < p + 1 || L.ty < p + 1) return false;
     L.ntx = ceil_div(nex, L.tx);
     L.nty = ceil_div(ney, L.ty);
     return true;
   };
-  if (!pick(o.tile_x_f64, o.tile_y_f64, igak::TILE_32x16, op->tile64, op->L64) ||
-      !pick(o.tile_x_f32, o.tile_y_f32, igak::TILE_32x16, op->tile32, op->L32))
+  if (!pick(o.tile_x_f64, o.tile_y_f64, igak::TILE_16x16, op->tile64, op->L64) ||
+      !pick(o.tile_x_f32, o.tile_y_f32, igak::TILE_16x16, op->tile32, op->L32))
+    return fail(IGA_ERR_UNSUPPORTED);
+  if (rational == 2) {
+    // general rational kernels: one workgroup per tile, few tile shapes
+    for (int id : {op->tile64, op->tile32})
+      if (!igak::rat_compiled(p, id)) return fail(IGA_ERR_UNSUPPORTED);
+    if (o.variant_f64 == 1 || o.variant_f32 == 1) return fail(IGA_ERR_UNSUPPORTED);
+    op->variant64 = op->variant32 = 0;
+  }
+  if ((op->variant64 == 1 && !igak::pf_compiled(p, op->tile64)) ||
+      (op->variant32 == 1 && !igak::pf_compiled(p, op->tile32)))
     return fail(IGA_ERR_UNSUPPORTED);
 
   // tables and Gauss rule (host FP64)
   std::vector<double> gx(nq), gw(nq);
   gauss01(nq, gx.data(), gw.data());
   op->h_wq = gw;
-  op->h_tabx = basis_tables(p, nq, nex, m->extraction_x, rational ? wx.data() : nullptr,
+  op->h_tabx = basis_tables(p, nq, nex, m->extraction_x, rational == 1 ? wx.data() : nullptr,
                             gx.data());
-  op->h_taby = basis_tables(p, nq, ney, m->extraction_y, rational ? wy.data() : nullptr,
+  op->h_taby = basis_tables(p, nq, ney, m->extraction_y, rational == 1 ? wy.data() : nullptr,
                             gx.data());
   const size_t tabn_x = op->h_tabx.size(), tabn_y = op->h_taby.size();
   std::vector<float> tx32(op->h_tabx.begin(), op->h_tabx.end());
@@ -416,6 +437,13 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   H2D(op->taby32, ty32.data(), tabn_y * 4);
   H2D(op->cp, m->cp, (size_t)2 * nfunc * 8);
   H2D(op->wq, gw.data(), (size_t)nq * 8);
+  if (rational == 2) {
+    std::vector<float> w32(m->weights, m->weights + nfunc);
+    ALLOC(op->wts64, (size_t)nfunc * 8);
+    ALLOC(op->wts32, (size_t)nfunc * 4);
+    H2D(op->wts64, m->weights, (size_t)nfunc * 8);
+    H2D(op->wts32, w32.data(), (size_t)nfunc * 4);
+  }
   if (!ring) {
     ALLOC(op->dirmask, (size_t)nfunc);
     H2D(op->dirmask, mask.data(), (size_t)nfunc);
@@ -428,15 +456,15 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   // G_q: FP64 always (the backward pass and the refinement residual need it)
   {
     const igak::Layout &L = op->L64;
-    const size_t n = (size_t)L.ntx * L.nty * 3 * nqp * L.tx * L.ty;
+    const size_t n = (size_t)L.ntx * L.nty * op->nc * nqp * L.tx * L.ty;
     ALLOC(op->G64, n * 8);
-    if (hipSuccess != ops->geom64(op->G64, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
-                                  nex, ney, nfx, L, s))
+    if (hipSuccess != ops->geom64(op->G64, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
+                                  op->kappa, nex, ney, nfx, L, s))
       return fail(IGA_ERR_HIP);
     if (op->kappa) {
       ALLOC(op->G0, n * 8);
-      if (hipSuccess != ops->geom64(op->G0, op->tabx64, op->taby64, op->cp, op->wq, nullptr,
-                                    nex, ney, nfx, L, s))
+      if (hipSuccess != ops->geom64(op->G0, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
+                                    nullptr, nex, ney, nfx, L, s))
         return fail(IGA_ERR_HIP);
     } else {
       op->G0 = op->G64;
@@ -445,10 +473,10 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   }
   if (op->want32) {
     const igak::Layout &L = op->L32;
-    const size_t n = (size_t)L.ntx * L.nty * 3 * nqp * L.tx * L.ty;
+    const size_t n = (size_t)L.ntx * L.nty * op->nc * nqp * L.tx * L.ty;
     ALLOC(op->G32, n * 4);
-    if (hipSuccess != ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
-                                  nex, ney, nfx, L, s))
+    if (hipSuccess != ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
+                                  op->kappa, nex, ney, nfx, L, s))
       return fail(IGA_ERR_HIP);
     ALLOC(op->halo32, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 4);
   }
@@ -477,6 +505,8 @@ int iga_op_get_info(const iga_op *op, iga_op_info *info) {
   info->tile_y_f32 = op->L32.ty;
   info->device_bytes = op->bytes;
   info->setup_ms = op->setup_ms;
+  info->variant_f64 = op->variant64;
+  info->variant_f32 = op->variant32;
   return IGA_OK;
 }
 
@@ -492,6 +522,129 @@ int iga_op_apply(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb, c
   return igai::apply_raw(op, dt == IGA_F64, kern == IGA_KERNEL_EBE,
                          asmb == IGA_ASSEMBLY_INVLM_GATHER, x, y, 0, 1, (hipStream_t)stream,
                          nullptr);
+}
+
+namespace {
+
+bool block_cfg_ok(const iga_op *op, int cfg, int k) {
+  // compiled for the degree; a chunk wider than k leaves most threads idle
+  if (!igak::block_cfg_compiled(op->p, cfg)) return false;
+  const igak::BlockCfg c = igak::block_cfg(cfg);
+  return c.kr <= 8 || c.kr <= k;
+}
+
+int block_default(const iga_op *op, int k) {
+  // without tuning: 16x16 tiles, the widest rhs chunk that k fills
+  (void)op;
+  return k >= 16 ? 2 : (k >= 8 ? 1 : 0);
+}
+
+int block_launch(iga_op *op, int cfg, int k, int ld, const float *X, float *Y, hipStream_t s) {
+  igak::BlockArgs a;
+  a.X = X;
+  a.Y = Y;
+  a.ld = ld;
+  a.k = k;
+  a.G = op->G32;
+  a.tabx = op->tabx32;
+  a.taby = op->taby32;
+  a.dirmask = op->dirmask;
+  a.nex = op->nex;
+  a.ney = op->ney;
+  a.nfx = op->nfx;
+  a.nfy = op->nfy;
+  a.L = op->L32;
+  a.nbx = a.nby = a.nch = 0;
+  IGA_HIP_TRY(op->ops->block(a, cfg, s));
+  return IGA_OK;
+}
+
+}  // namespace
+
+int iga_op_apply_block(iga_op *op, int k, int ld, const float *X, float *Y, void *stream) {
+  if (!op || !X || !Y || k < 1 || ld < op->nfunc || X == Y) return IGA_ERR_INVALID_ARGUMENT;
+  if (!op->G32 || op->rational == 2) return IGA_ERR_UNSUPPORTED;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  return igai::apply_block_raw(op, k, ld, X, Y, (hipStream_t)stream);
+}
+
+int iga_op_tune_block(iga_op *op, int k, int ld, const float *X, float *Y, void *stream,
+                      int reps, int *best_cfg, double *best_ms) {
+  if (!op || !X || !Y || k < 1 || ld < op->nfunc || reps < 1) return IGA_ERR_INVALID_ARGUMENT;
+  if (!op->G32 || op->rational == 2) return IGA_ERR_UNSUPPORTED;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  IGA_HIP_TRY(hipEventCreate(&e0));
+  IGA_HIP_TRY(hipEventCreate(&e1));
+  int best = -1;
+  float bms = 1e30f;
+  for (int cfg = 0; cfg < igak::NBLOCKCFG; ++cfg) {
+    if (!block_cfg_ok(op, cfg, k)) continue;
+    rc = block_launch(op, cfg, k, ld, X, Y, s);  // warm-up
+    if (rc) break;
+    IGA_HIP_TRY(hipEventRecord(e0, s));
+    for (int r = 0; r < reps && !rc; ++r) rc = block_launch(op, cfg, k, ld, X, Y, s);
+    IGA_HIP_TRY(hipEventRecord(e1, s));
+    IGA_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    IGA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    if (ms < bms) {
+      bms = ms;
+      best = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (best < 0) return IGA_ERR_UNSUPPORTED;
+  bool found = false;
+  for (auto &kv : op->block_tuned)
+    if (kv.first == k) {
+      kv.second = best;
+      found = true;
+    }
+  if (!found) op->block_tuned.push_back({k, best});
+  if (best_cfg) *best_cfg = best;
+  if (best_ms) *best_ms = bms;
+  return IGA_OK;
+}
+
+int iga_block_config_count(void) { return igak::NBLOCKCFG; }
+
+int iga_block_config_desc(int cfg, int *tile_x, int *tile_y, int *rhs_chunk) {
+  if (cfg < 0 || cfg >= igak::NBLOCKCFG) return IGA_ERR_INVALID_ARGUMENT;
+  const igak::BlockCfg c = igak::block_cfg(cfg);
+  if (tile_x) *tile_x = c.txb;
+  if (tile_y) *tile_y = c.tyb;
+  if (rhs_chunk) *rhs_chunk = c.kr;
+  return IGA_OK;
+}
+
+int iga_op_set_block_config(iga_op *op, int k, int cfg) {
+  if (!op || k < 1 || cfg < -1 || cfg >= igak::NBLOCKCFG) return IGA_ERR_INVALID_ARGUMENT;
+  if (cfg >= 0 && !igak::block_cfg_compiled(op->p, cfg)) return IGA_ERR_UNSUPPORTED;
+  for (size_t i = 0; i < op->block_tuned.size(); ++i)
+    if (op->block_tuned[i].first == k) {
+      if (cfg < 0)
+        op->block_tuned.erase(op->block_tuned.begin() + i);
+      else
+        op->block_tuned[i].second = cfg;
+      return IGA_OK;
+    }
+  if (cfg >= 0) op->block_tuned.push_back({k, cfg});
+  return IGA_OK;
+}
+
+int iga_op_get_block_config(const iga_op *op, int k, int *cfg) {
+  if (!op || k < 1 || !cfg) return IGA_ERR_INVALID_ARGUMENT;
+  *cfg = block_default(op, k);
+  for (auto &kv : op->block_tuned)
+    if (kv.first == k) *cfg = kv.second;
+  return IGA_OK;
 }
 
 int iga_op_profile(iga_op *op, int enable, int max_calls) {
@@ -540,7 +693,8 @@ int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream) {
     rc = igai::dev_alloc(op, (void **)&op->fJ, (size_t)op->nelem * op->nqp * 8);
     if (rc) return rc;
   }
-  IGA_HIP_TRY(op->ops->load(F, op->fJ, op->tabx64, op->taby64, op->cp, op->wq, op->dirmask,
+  IGA_HIP_TRY(op->ops->load(F, op->fJ, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
+                            op->dirmask,
                             op->nex, op->ney, op->nfx, op->nfy, kind, value,
                             (hipStream_t)stream));
   return IGA_OK;
@@ -552,7 +706,7 @@ int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream) {
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const igak::Layout &L = op->L64;
-  const size_t n = (size_t)L.ntx * L.nty * 3 * op->nqp * L.tx * L.ty;
+  const size_t n = (size_t)L.ntx * L.nty * op->nc * op->nqp * L.tx * L.ty;
   if (!op->kappa) {
     rc = igai::dev_alloc(op, (void **)&op->kappa, (size_t)op->nelem * op->nqp * 8);
     if (rc) return rc;
@@ -565,11 +719,11 @@ int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream) {
   }
   IGA_HIP_TRY(hipMemcpyAsync(op->kappa, kappa, (size_t)op->nelem * op->nqp * 8,
                              hipMemcpyDeviceToDevice, s));
-  IGA_HIP_TRY(op->ops->geom64(op->G64, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
-                              op->nex, op->ney, op->nfx, op->L64, s));
+  IGA_HIP_TRY(op->ops->geom64(op->G64, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
+                              op->kappa, op->nex, op->ney, op->nfx, op->L64, s));
   if (op->G32)
-    IGA_HIP_TRY(op->ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wq, op->kappa,
-                                op->nex, op->ney, op->nfx, op->L32, s));
+    IGA_HIP_TRY(op->ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
+                                op->kappa, op->nex, op->ney, op->nfx, op->L32, s));
   ++op->gen;
   return IGA_OK;
 }
@@ -588,9 +742,21 @@ int iga_op_apply_backward(iga_op *op, const double *x, const double *g, double *
   }
   if (grad_kappa)
     IGA_HIP_TRY(op->ops->kappa_grad(grad_kappa, x, g, op->G0, op->tabx64, op->taby64,
-                                    op->dirmask, op->nex, op->ney, op->nfx, op->nfy, op->L64,
+                                    op->wts64, op->dirmask, op->nex, op->ney, op->nfx, op->nfy, op->L64,
                                     s));
   return IGA_OK;
 }
 
 }  // extern "C"
+
+namespace igai {
+int apply_block_raw(iga_op *op, int k, int ld, const float *X, float *Y, hipStream_t s) {
+  if (!op->G32 || op->rational == 2) return IGA_ERR_UNSUPPORTED;
+  int cfg = -1;
+  for (auto &kv : op->block_tuned)
+    if (kv.first == k) cfg = kv.second;
+  if (cfg < 0) cfg = block_default(op, k);
+  return block_launch(op, cfg, k, ld, X, Y, s);
+}
+}  // namespace igai
+

@@ -1,0 +1,5 @@
+// Multi right-hand-side apply kernels of degree 2.
+#include "block.hpp"
+#include "launch.hpp"
+
+IGAK_DEFINE_BLOCK(2)

@@ -1,0 +1,5 @@
+// Multi right-hand-side apply kernels of degree 4.
+#include "block.hpp"
+#include "launch.hpp"
+
+IGAK_DEFINE_BLOCK(4)

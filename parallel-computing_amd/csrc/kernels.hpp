@@ -43,6 +43,7 @@ struct ApplyArgs {
   const void *taby;       // [nelem_y][2][NQ][P+1]
   const uint8_t *dirmask; // nullptr = Dirichlet on the boundary ring
   void *halo;             // [tile][(TX+P)*(TY+P)]
+  const void *wf;         // per-function NURBS weights (general rational meshes) or nullptr
   int nex, ney, nfx, nfy, ntx, nty;
   int mode;               // 0 scatter-add, 1 gather
   int mask_in;            // treat x as P_int x
@@ -83,15 +84,16 @@ __device__ __forceinline__ void atomic_add(T *p, T v) {
 }
 
 // --------------------------------------------------------------------------
-// G_q storage: per tile and component c (G11, G12, G22) a block of NQ2*NT
-// values; the first NQ2 - NQ2 % VW points are stored as VW-wide vectors per
-// thread ([chunk][t][VW], one 16-byte load each), the rest as [q][t].
+// G_q storage: per tile and component c (G11, G12, G22; on general rational
+// meshes also a_xi, a_eta, so NC = 5) a block of NQ2*NT values; the first
+// NQ2 - NQ2 % VW points are stored as VW-wide vectors per thread
+// ([chunk][t][VW], one 16-byte load each), the rest as [q][t].
 // --------------------------------------------------------------------------
-template <typename T, int NQ2>
+template <typename T, int NQ2, int NC = 3>
 __device__ __forceinline__ size_t gq_index(int tile, int c, int q, int t, int NT) {
   constexpr int VW = 16 / sizeof(T);
   constexpr int NV = NQ2 / VW;
-  const size_t base = ((size_t)tile * 3 + c) * NQ2 * NT;
+  const size_t base = ((size_t)tile * NC + c) * NQ2 * NT;
   if (q < NV * VW) return base + ((size_t)(q / VW) * NT + t) * VW + q % VW;
   return base + (size_t)NV * VW * NT + (size_t)(q - NV * VW) * NT + t;
 }
@@ -107,14 +109,14 @@ struct Vec16<double> {
   typedef double type __attribute__((ext_vector_type(2)));
 };
 
-template <typename T, int NQ2>
-__device__ __forceinline__ void load_gq(const T *G, int tile, int t, int NT, T (&g)[3][NQ2]) {
+template <typename T, int NQ2, int NC = 3>
+__device__ __forceinline__ void load_gq(const T *G, int tile, int t, int NT, T (&g)[NC][NQ2]) {
   constexpr int VW = 16 / sizeof(T);
   constexpr int NV = NQ2 / VW;
   typedef typename Vec16<T>::type V;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T *Gc = G + ((size_t)tile * 3 + c) * NQ2 * NT;
+  for (int c = 0; c < NC; ++c) {
+    const T *Gc = G + ((size_t)tile * NC + c) * NQ2 * NT;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const V v = *reinterpret_cast<const V *>(Gc + ((size_t)k * NT + t) * VW);
@@ -136,7 +138,7 @@ __device__ __forceinline__ void load_gq(const T *G, int tile, int t, int NT, T (
 // Sum-factorised action ye = -sum_q gradR^T G_q gradR X.  The eta loop fuses
 // the forward eta contraction, the quadrature-point operator and the backward
 // eta contraction, so only one table row is live at a time.
-template <typename T, int P, int NQ, int LX, int LY, typename GL>
+template <typename T, int P, int NQ, int LX, int LY, bool RAT = false, typename GL>
 __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
                                            const T (*tbx)[LX], const T (*tby)[LY],
                                            int lx, int ly, const GL &gl,
@@ -181,20 +183,30 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
     }
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-      T ux = T(0), uy = T(0);
+      T ux = T(0), uy = T(0), uv = T(0);
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         ux += n2[b] * t1[b][i];
         uy += d2[b] * t0[b][i];
+        if (RAT) uv += n2[b] * t0[b][i];
       }
       const int q = j * NQ + i;
+      T ax = T(0), ay = T(0);
+      if (RAT) {
+        // grad(U/W) W = grad U - U a with a = grad W / W (1/W^2 is in G)
+        ax = gl(3, q);
+        ay = gl(4, q);
+        ux -= uv * ax;
+        uy -= uv * ay;
+      }
       const T g11 = gl(0, q), g12 = gl(1, q), g22 = gl(2, q);
       const T fx = -(g11 * ux + g12 * uy);
       const T fy = -(g12 * ux + g22 * uy);
+      const T fv = RAT ? -(ax * fx + ay * fy) : T(0);
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         s0[b][i] += n2[b] * fx;
-        s1[b][i] += d2[b] * fy;
+        s1[b][i] += RAT ? d2[b] * fy + n2[b] * fv : d2[b] * fy;
       }
     }
   }
@@ -219,7 +231,7 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
 
 // Element-by-element action: per quadrature point the full reference
 // gradient of every local function (LocalAssemblyMF.cpp:31-67 shape).
-template <typename T, int P, int NQ, int LX, int LY, typename GL>
+template <typename T, int P, int NQ, int LX, int LY, bool RAT = false, typename GL>
 __device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
                                             const T (*tbx)[LX], const T (*tby)[LY],
                                             int lx, int ly, const GL &gl,
@@ -245,37 +257,245 @@ __device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
         n1[a] = tbx[i * NB + a][lx];
         d1[a] = tbx[TAB + i * NB + a][lx];
       }
-      T ux = T(0), uy = T(0);
+      T ux = T(0), uy = T(0), uv = T(0);
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int a = 0; a < NB; ++a) {
           ux += (d1[a] * n2[b]) * X[b][a];
           uy += (n1[a] * d2[b]) * X[b][a];
+          if (RAT) uv += (n1[a] * n2[b]) * X[b][a];
         }
       const int q = j * NQ + i;
+      T ax = T(0), ay = T(0);
+      if (RAT) {
+        ax = gl(3, q);
+        ay = gl(4, q);
+        ux -= uv * ax;
+        uy -= uv * ay;
+      }
       const T g11 = gl(0, q), g12 = gl(1, q), g22 = gl(2, q);
       const T fx = -(g11 * ux + g12 * uy);
       const T fy = -(g12 * ux + g22 * uy);
+      const T fv = RAT ? -(ax * fx + ay * fy) : T(0);
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int a = 0; a < NB; ++a) ye[b][a] += (d1[a] * n2[b]) * fx + (n1[a] * d2[b]) * fy;
+        for (int a = 0; a < NB; ++a) {
+          ye[b][a] += (d1[a] * n2[b]) * fx + (n1[a] * d2[b]) * fy;
+          if (RAT) ye[b][a] += (n1[a] * n2[b]) * fv;
+        }
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Tile staging: the x footprint and the tile's 1D table entries are loaded
+// into registers first (load), the G_q loads are issued after them, and
+// only then are the staged values written to LDS (store).  The memory
+// counter completes in issue order, so the LDS stores wait for the x and
+// table loads alone while the G_q loads stay in flight.
+// --------------------------------------------------------------------------
+template <typename T, int P, int NQ, int TX, int TY, bool RAT>
+struct TileStage {
+  static constexpr int NB = P + 1, NT = TX * TY, FX = TX + P, FY = TY + P, NFP = FX * FY;
+  static constexpr int TAB = NQ * NB, NOWN = (NFP + NT - 1) / NT;
+  static constexpr int NTBX = (2 * TAB * TX + NT - 1) / NT, NTBY = (2 * TAB * TY + NT - 1) / NT;
+  T xv[NOWN];
+  T wv[RAT ? NOWN : 1];
+  T tbx[NTBX], tby[NTBY];
+
+  __device__ __forceinline__ void load(const ApplyArgs &A, int ex0, int ey0, int t) {
+    const T *x = static_cast<const T *>(A.x);
+    const T *wf = static_cast<const T *>(A.wf);
+    const T *tx_ = static_cast<const T *>(A.tabx);
+    const T *ty_ = static_cast<const T *>(A.taby);
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int k = t + o * NT;
+      xv[o] = T(0);
+      if (RAT) wv[o] = T(0);
+      if (k < NFP) {
+        const int gx = ex0 + k % FX, gy = ey0 + k / FX;
+        if (gx < A.nfx && gy < A.nfy) {
+          const int Ag = gy * A.nfx + gx;
+          xv[o] = x[Ag];
+          if (RAT) wv[o] = wf[Ag];
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < NTBX; ++o) {
+      const int k = t + o * NT;
+      tbx[o] = T(0);
+      if (k < 2 * TAB * TX) {
+        int e = ex0 + k % TX;
+        if (e > A.nex - 1) e = A.nex - 1;
+        tbx[o] = tx_[(size_t)e * 2 * TAB + k / TX];
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < NTBY; ++o) {
+      const int k = t + o * NT;
+      tby[o] = T(0);
+      if (k < 2 * TAB * TY) {
+        int e = ey0 + k % TY;
+        if (e > A.ney - 1) e = A.ney - 1;
+        tby[o] = ty_[(size_t)e * 2 * TAB + k / TY];
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(const ApplyArgs &A, int ex0, int ey0, int t, T *xs,
+                                        T (*ltx)[TX], T (*lty)[TY]) const {
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int k = t + o * NT;
+      if (k < NFP) {
+        T v = xv[o];
+        if (RAT) v *= wv[o];
+        if (A.mask_in) {
+          const int gx = ex0 + k % FX, gy = ey0 + k / FX;
+          if (gx < A.nfx && gy < A.nfy &&
+              is_dirichlet(A.dirmask, gy * A.nfx + gx, gx, gy, A.nfx, A.nfy))
+            v = T(0);
+        }
+        xs[k] = v;
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < NTBX; ++o) {
+      const int k = t + o * NT;
+      if (k < 2 * TAB * TX) ltx[k / TX][k % TX] = tbx[o];
+    }
+#pragma unroll
+    for (int o = 0; o < NTBY; ++o) {
+      const int k = t + o * NT;
+      if (k < 2 * TAB * TY) lty[k / TY][k % TY] = tby[o];
+    }
+  }
+};
+
+// --------------------------------------------------------------------------
+// footprint reduction and store
+// --------------------------------------------------------------------------
+// One footprint value v at (fx, fy) of the tile at (ex0, ey0): Dirichlet rows
+// get 0; functions only this tile touches are stored; seam functions are
+// atomically added (mode 0, after k_seam zeroed them) or written to the
+// tile's halo slot fy*FX+fx (mode 1, summed by k_seam).  General rational
+// meshes scale by the function's weight.
+template <typename T, int P, int TX, int TY, bool RAT>
+__device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo, const T *wf,
+                                              int tile, int ex0, int ey0, int fx, int fy, T v) {
+  constexpr int FX = TX + P;
+  const int gx = ex0 + fx, gy = ey0 + fy;
+  if (gx >= A.nfx || gy >= A.nfy) return;
+  int xlo, xhi, ylo, yhi;
+  touching_tiles<P>(gx, A.nex, TX, xlo, xhi);
+  touching_tiles<P>(gy, A.ney, TY, ylo, yhi);
+  const bool shared = (xlo != xhi) || (ylo != yhi);
+  const int Ag = gy * A.nfx + gx;
+  if (A.mask_out && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) {
+    if (!shared || A.mode == 0) y[Ag] = T(0);
+    return;
+  }
+  if (RAT) v *= wf[Ag];
+  if (!shared)
+    y[Ag] = v;
+  else if (A.mode == 0)
+    atomic_add(&y[Ag], v);
+  else
+    halo[(size_t)tile * (FX * (TY + P)) + fy * FX + fx] = v;
+}
+
+// Cross-lane move inside 16-lane DPP rows (zero where the source lane falls
+// outside the row).  CTRL: 0x110 + n = row_shr:n, 0x100 + n = row_shl:n.
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Overlap-add of the element vectors of a 16-wide tile (element (lx, ly) on
+// lane lx of DPP row ly % 4).  Along x inside each row: column lx of the
+// tile collects ye[b][a] of lane lx - a (row_shr:a), the P columns past the
+// tile's right edge collect lanes 16 + c - a (row_shl:16-a, valid on lanes
+// c < P).  Along y through LDS: rows ly - b.  One barrier; fixed order, so
+// bitwise reproducible.  lds needs 2 * (P + 1) * 16 * TY values.
+template <int P, int A_, typename T>
+__device__ __forceinline__ void row16_x(const T (&ye)[P + 1][P + 1], T (&rin)[P + 1],
+                                        T (&rwr)[P + 1]) {
+  if constexpr (A_ <= P) {
+#pragma unroll
+    for (int b = 0; b <= P; ++b) {
+      rin[b] += dpp_row<0x110 + A_>(ye[b][A_]);
+      rwr[b] += dpp_row<0x100 + 16 - A_>(ye[b][A_]);
+    }
+    row16_x<P, A_ + 1>(ye, rin, rwr);
+  }
+}
+
+template <typename T, int P, int TY, bool RAT>
+__device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (&ye)[P + 1][P + 1],
+                                                   T *lds, int t, int tile, int ex0, int ey0,
+                                                   T *y, T *halo, const T *wf) {
+  constexpr int NB = P + 1, NT = 16 * TY;
+  const int lx = t & 15, ly = t >> 4;
+  T rin[NB], rwr[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    rin[b] = ye[b][0];
+    rwr[b] = T(0);
+  }
+  row16_x<P, 1>(ye, rin, rwr);
+  T *bin = lds, *bwr = lds + NB * NT;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    bin[b * NT + t] = rin[b];
+    if (lx < P) bwr[b * NT + t] = rwr[b];
+  }
+  __syncthreads();
+  // rows 0..TY-1 of this thread's column, and rows TY..TY+P-1 on ly < P
+  T v0 = T(0), v1 = T(0), w0 = T(0), w1 = T(0);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b <= ly) {
+      v0 += bin[b * NT + (ly - b) * 16 + lx];
+      if (lx < P) w0 += bwr[b * NT + (ly - b) * 16 + lx];
+    }
+    if (ly < P && b > ly) {
+      v1 += bin[b * NT + (TY + ly - b) * 16 + lx];
+      if (lx < P) w1 += bwr[b * NT + (TY + ly - b) * 16 + lx];
+    }
+  }
+  put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, ly, v0);
+  if (lx < P) put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, ly, w0);
+  if (ly < P) {
+    put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, TY + ly, v1);
+    if (lx < P)
+      put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, TY + ly, w1);
   }
 }
 
 // --------------------------------------------------------------------------
 // the apply kernel
 // --------------------------------------------------------------------------
-template <typename T, int P, int NQ, int TX, int TY, bool EBE>
+template <typename T, int P, int NQ, int TX, int TY, bool EBE, bool RAT = false>
 __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
   constexpr int NB = P + 1, NT = TX * TY, FX = TX + P, FY = TY + P;
   constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY;
   __shared__ T xs[NFP];
   __shared__ T tbx[2 * TAB][TX];
   __shared__ T tby[2 * TAB][TY];
-  __shared__ T buf[NB][NT];
+  __shared__ T bufm[(TX == 16 ? 2 : 1) * NB * NT];
+  T(*buf)[NT] = reinterpret_cast<T(*)[NT]>(bufm);
 
   if (A.skip && *A.skip) return;
   const int t = threadIdx.x, lx = t % TX, ly = t / TX;
@@ -284,47 +504,23 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
   const int ex0 = tx * TX, ey0 = ty * TY;
   const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
 
-  // geometry factors of this element
-#if IGAK_PREFETCH_G
-  T g[3][NQ2];
-  load_gq<T, NQ2>(static_cast<const T *>(A.G), tile, t, NT, g);
-  auto gl = [&](int c, int q) { return g[c][q]; };
-#else
+  // x footprint and tables first, then the geometry factors of this element
+  // (held in registers unless they are many: high degree, general rational)
+  TileStage<T, P, NQ, TX, TY, RAT> stg;
+  stg.load(A, ex0, ey0, t);
+  constexpr int NC = RAT ? 5 : 3;
+  constexpr bool PREF = IGAK_PREFETCH_G && NC * NQ2 * sizeof(T) <= 320;
+  T g[PREF ? NC : 1][PREF ? NQ2 : 1];
   const T *Gt = static_cast<const T *>(A.G);
-  auto gl = [&](int c, int q) { return Gt[gq_index<T, NQ2>(tile, c, q, t, NT)]; };
-#endif
-  // x footprint
-  {
-    const T *x = static_cast<const T *>(A.x);
-    for (int k = t; k < NFP; k += NT) {
-      const int fx = k % FX, fy = k / FX;
-      const int gx = ex0 + fx, gy = ey0 + fy;
-      T v = T(0);
-      if (gx < A.nfx && gy < A.nfy) {
-        const int Ag = gy * A.nfx + gx;
-        v = x[Ag];
-        if (A.mask_in && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) v = T(0);
-      }
-      xs[k] = v;
-    }
-  }
-  // 1D tables of the tile's columns and rows
-  {
-    const T *tx_ = static_cast<const T *>(A.tabx);
-    const T *ty_ = static_cast<const T *>(A.taby);
-    for (int k = t; k < 2 * TAB * TX; k += NT) {
-      const int l = k % TX, c = k / TX;
-      int e = ex0 + l;
-      if (e > A.nex - 1) e = A.nex - 1;
-      tbx[c][l] = tx_[(size_t)e * 2 * TAB + c];
-    }
-    for (int k = t; k < 2 * TAB * TY; k += NT) {
-      const int l = k % TY, c = k / TY;
-      int e = ey0 + l;
-      if (e > A.ney - 1) e = A.ney - 1;
-      tby[c][l] = ty_[(size_t)e * 2 * TAB + c];
-    }
-  }
+  if constexpr (PREF) load_gq<T, NQ2, NC>(Gt, tile, t, NT, g);
+  auto gl = [&](int c, int q) {
+    if constexpr (PREF)
+      return g[c][q];
+    else
+      return Gt[gq_index<T, NQ2, NC>(tile, c, q, t, NT)];
+  };
+  const T *wf = static_cast<const T *>(A.wf);
+  stg.store(A, ex0, ey0, t, xs, tbx, tby);
   __syncthreads();
 
   T ye[NB][NB];
@@ -335,9 +531,9 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
 #pragma unroll
       for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
     if (EBE)
-      element_ebe<T, P, NQ, TX, TY>(X, tbx, tby, lx, ly, gl, ye);
+      element_ebe<T, P, NQ, TX, TY, RAT>(X, tbx, tby, lx, ly, gl, ye);
     else
-      element_sf<T, P, NQ, TX, TY>(X, tbx, tby, lx, ly, gl, ye);
+      element_sf<T, P, NQ, TX, TY, RAT>(X, tbx, tby, lx, ly, gl, ye);
     if (!valid) {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
@@ -346,6 +542,12 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
     }
   }
 
+  T *y = static_cast<T *>(A.y);
+  T *halo = static_cast<T *>(A.halo);
+  if constexpr (TX == 16) {
+    reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, tile, ex0, ey0, y, halo, wf);
+    return;
+  }
   // fixed-order reduction over the footprint: pass b adds element row offset b
   constexpr int NOWN = (NFP + NT - 1) / NT;
   T acc[NOWN];
@@ -373,32 +575,11 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
       }
     }
   }
-
-  // output
-  T *y = static_cast<T *>(A.y);
-  T *halo = static_cast<T *>(A.halo);
 #pragma unroll
   for (int o = 0; o < NOWN; ++o) {
     const int k = t + o * NT;
-    if (k >= NFP) continue;
-    const int fx = k % FX, fy = k / FX;
-    const int gx = ex0 + fx, gy = ey0 + fy;
-    if (gx >= A.nfx || gy >= A.nfy) continue;
-    int xlo, xhi, ylo, yhi;
-    touching_tiles<P>(gx, A.nex, TX, xlo, xhi);
-    touching_tiles<P>(gy, A.ney, TY, ylo, yhi);
-    const bool shared = (xlo != xhi) || (ylo != yhi);
-    const int Ag = gy * A.nfx + gx;
-    if (A.mask_out && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) {
-      if (!shared || A.mode == 0) y[Ag] = T(0);
-      continue;
-    }
-    if (!shared)
-      y[Ag] = acc[o];
-    else if (A.mode == 0)
-      atomic_add(&y[Ag], acc[o]);
-    else
-      halo[(size_t)tile * NFP + k] = acc[o];
+    if (k < NFP)
+      put_footprint<T, P, TX, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, k % FX, k / FX, acc[o]);
   }
 }
 
@@ -414,12 +595,12 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
   __shared__ T xs[NFP];
   __shared__ T tbx[2 * TAB][TX];
   __shared__ T tby[2 * TAB][TY];
-  __shared__ T buf[NB][NT];
+  __shared__ T bufm[(TX == 16 ? 2 : 1) * NB * NT];
+  T(*buf)[NT] = reinterpret_cast<T(*)[NT]>(bufm);
 
   if (A.skip && *A.skip) return;
   const int t = threadIdx.x, lx = t % TX, ly = t / TX;
   const T *Gp = static_cast<const T *>(A.G);
-  const T *x = static_cast<const T *>(A.x);
   T *y = static_cast<T *>(A.y);
   T *halo = static_cast<T *>(A.halo);
 
@@ -427,58 +608,25 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
   const int G = gridDim.x;
   int tile = xcd_tile(blockIdx.x, G);
 
+  // next tile's x footprint and tables (first), then its G_q, in registers
+  TileStage<T, P, NQ, TX, TY, false> stn;
   T gn[3][NQ2];
-  T xn[NOWN];
   auto prefetch = [&](int tl) {
+    stn.load(A, (tl % A.ntx) * TX, (tl / A.ntx) * TY, t);
     load_gq<T, NQ2>(Gp, tl, t, NT, gn);
-    const int ex0 = (tl % A.ntx) * TX, ey0 = (tl / A.ntx) * TY;
-#pragma unroll
-    for (int o = 0; o < NOWN; ++o) {
-      const int k = t + o * NT;
-      T v = T(0);
-      if (k < NFP) {
-        const int gx = ex0 + k % FX, gy = ey0 + k / FX;
-        if (gx < A.nfx && gy < A.nfy) {
-          const int Ag = gy * A.nfx + gx;
-          v = x[Ag];
-          if (A.mask_in && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) v = T(0);
-        }
-      }
-      xn[o] = v;
-    }
   };
   if (tile < ntiles) prefetch(tile);
   for (; tile < ntiles; tile += G) {
     const int tx = tile % A.ntx, ty = tile / A.ntx;
     const int ex0 = tx * TX, ey0 = ty * TY;
     const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
+    __syncthreads();  // previous tile's LDS reads are done
+    stn.store(A, ex0, ey0, t, xs, tbx, tby);
     T g[3][NQ2];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int q = 0; q < NQ2; ++q) g[c][q] = gn[c][q];
-    __syncthreads();  // previous tile's LDS reads are done
-#pragma unroll
-    for (int o = 0; o < NOWN; ++o) {
-      const int k = t + o * NT;
-      if (k < NFP) xs[k] = xn[o];
-    }
-    {
-      const T *tx_ = static_cast<const T *>(A.tabx);
-      const T *ty_ = static_cast<const T *>(A.taby);
-      for (int k = t; k < 2 * TAB * TX; k += NT) {
-        const int l = k % TX, c = k / TX;
-        int e = ex0 + l;
-        if (e > A.nex - 1) e = A.nex - 1;
-        tbx[c][l] = tx_[(size_t)e * 2 * TAB + c];
-      }
-      for (int k = t; k < 2 * TAB * TY; k += NT) {
-        const int l = k % TY, c = k / TY;
-        int e = ey0 + l;
-        if (e > A.ney - 1) e = A.ney - 1;
-        tby[c][l] = ty_[(size_t)e * 2 * TAB + c];
-      }
-    }
     const int nxt = tile + G;
     if (nxt < ntiles) prefetch(nxt);
     __syncthreads();
@@ -498,6 +646,10 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
 #pragma unroll
           for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
       }
+    }
+    if constexpr (TX == 16) {
+      reduce_store_row16<T, P, TY, false>(A, ye, bufm, t, tile, ex0, ey0, y, halo, nullptr);
+      continue;
     }
     T acc[NOWN];
 #pragma unroll
@@ -527,25 +679,9 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
 #pragma unroll
     for (int o = 0; o < NOWN; ++o) {
       const int k = t + o * NT;
-      if (k >= NFP) continue;
-      const int fx = k % FX, fy = k / FX;
-      const int gx = ex0 + fx, gy = ey0 + fy;
-      if (gx >= A.nfx || gy >= A.nfy) continue;
-      int xlo, xhi, ylo, yhi;
-      touching_tiles<P>(gx, A.nex, TX, xlo, xhi);
-      touching_tiles<P>(gy, A.ney, TY, ylo, yhi);
-      const bool shared = (xlo != xhi) || (ylo != yhi);
-      const int Ag = gy * A.nfx + gx;
-      if (A.mask_out && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) {
-        if (!shared || A.mode == 0) y[Ag] = T(0);
-        continue;
-      }
-      if (!shared)
-        y[Ag] = acc[o];
-      else if (A.mode == 0)
-        atomic_add(&y[Ag], acc[o]);
-      else
-        halo[(size_t)tile * NFP + k] = acc[o];
+      if (k < NFP)
+        put_footprint<T, P, TX, TY, false>(A, y, halo, nullptr, tile, ex0, ey0, k % FX, k / FX,
+                                           acc[o]);
     }
   }
 }
@@ -602,51 +738,98 @@ __global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex,
 // G_q = kappa_q w_q det(J_q) J_q^-1 J_q^-T with J the Jacobian of the map
 // from the element's reference square [0,1]^2 (the reference's parametric
 // Jacobian times diag(hx,hy), matrix_free/src/ElementMF.cpp:159-206).
+// On general rational meshes (per-function weights wts, plain B-spline
+// tables) R_A = w_A B_A / W, and the kernels apply the operator to w o x:
+// G_q is stored divided by W_q^2 and components 3, 4 hold a = grad W / W.
 // One thread per tile slot; slots past the mesh edge get zeros.
+//
+// The map at one quadrature point: physical point, reference-square
+// Jacobian and W, grad W (W = 1 without wts).
+template <int P, int NQ>
+__device__ __forceinline__ void map_q(const double *cp, const double *wts, int nfx, int ex, int ey,
+                                      const double *t1, const double *t2, int i, int j,
+                                      double (&m)[9]) {
+  constexpr int NB = P + 1, TAB = NQ * NB;
+  double x = 0, y = 0, xxi = 0, xeta = 0, yxi = 0, yeta = 0, W = 0, Wxi = 0, Weta = 0;
+  for (int b = 0; b < NB; ++b)
+    for (int a = 0; a < NB; ++a) {
+      const size_t A = (size_t)(ey + b) * nfx + ex + a;
+      const double R = t1[i * NB + a] * t2[j * NB + b];
+      const double dxi = t1[TAB + i * NB + a] * t2[j * NB + b];
+      const double deta = t1[i * NB + a] * t2[TAB + j * NB + b];
+      double px = cp[2 * A], py = cp[2 * A + 1];
+      if (wts) {
+        const double w = wts[A];
+        px *= w;
+        py *= w;
+        W += w * R;
+        Wxi += w * dxi;
+        Weta += w * deta;
+      }
+      x += px * R;
+      y += py * R;
+      xxi += px * dxi;
+      xeta += px * deta;
+      yxi += py * dxi;
+      yeta += py * deta;
+    }
+  if (wts) {
+    // quotient rule for x = (sum w P B) / W
+    x /= W;
+    y /= W;
+    xxi = (xxi - x * Wxi) / W;
+    xeta = (xeta - x * Weta) / W;
+    yxi = (yxi - y * Wxi) / W;
+    yeta = (yeta - y * Weta) / W;
+  } else {
+    W = 1.0;
+  }
+  m[0] = x; m[1] = y; m[2] = xxi; m[3] = xeta; m[4] = yxi; m[5] = yeta;
+  m[6] = W; m[7] = Wxi; m[8] = Weta;
+}
+
 template <typename T, int P, int NQ>
 __global__ void k_geometry(T *G, const double *tabx, const double *taby,
-                           const double *cp, const double *wq,
+                           const double *cp, const double *wts, const double *wq,
                            const double *kappa, int nex, int ney, int nfx,
                            int TX, int TY, int ntx) {
-  constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
+  constexpr int NQ2 = NQ * NQ, TAB = NQ * (P + 1);
   const int NT = TX * TY;
   const int tile = blockIdx.x;
+  auto put = [&](int c, int q, int t, double v) {
+    if (wts)
+      G[gq_index<T, NQ2, 5>(tile, c, q, t, NT)] = (T)v;
+    else
+      G[gq_index<T, NQ2, 3>(tile, c, q, t, NT)] = (T)v;
+  };
+  const int nc = wts ? 5 : 3;
   for (int t = threadIdx.x; t < NT; t += blockDim.x) {
     const int tx = tile % ntx, ty = tile / ntx;
     const int ex = tx * TX + t % TX, ey = ty * TY + t / TX;
     if (ex >= nex || ey >= ney) {
       for (int q = 0; q < NQ2; ++q)
-        for (int c = 0; c < 3; ++c) G[gq_index<T, NQ2>(tile, c, q, t, NT)] = T(0);
+        for (int c = 0; c < nc; ++c) put(c, q, t, 0.0);
       continue;
     }
-    double X[NB][NB], Y[NB][NB];
-    for (int b = 0; b < NB; ++b)
-      for (int a = 0; a < NB; ++a) {
-        const size_t A = (size_t)(ey + b) * nfx + ex + a;
-        X[b][a] = cp[2 * A];
-        Y[b][a] = cp[2 * A + 1];
-      }
     const double *t1 = tabx + (size_t)ex * 2 * TAB;
     const double *t2 = taby + (size_t)ey * 2 * TAB;
     for (int j = 0; j < NQ; ++j)
       for (int i = 0; i < NQ; ++i) {
-        double xxi = 0, xeta = 0, yxi = 0, yeta = 0;
-        for (int b = 0; b < NB; ++b)
-          for (int a = 0; a < NB; ++a) {
-            const double dxi = t1[TAB + i * NB + a] * t2[j * NB + b];
-            const double deta = t1[i * NB + a] * t2[TAB + j * NB + b];
-            xxi += X[b][a] * dxi;
-            xeta += X[b][a] * deta;
-            yxi += Y[b][a] * dxi;
-            yeta += Y[b][a] * deta;
-          }
+        double m[9];
+        map_q<P, NQ>(cp, wts, nfx, ex, ey, t1, t2, i, j, m);
+        const double xxi = m[2], xeta = m[3], yxi = m[4], yeta = m[5];
         const double det = xxi * yeta - xeta * yxi;
         const int q = j * NQ + i;
         double s = wq[i] * wq[j] / det;
         if (kappa) s *= kappa[((size_t)ey * nex + ex) * NQ2 + q];
-        G[gq_index<T, NQ2>(tile, 0, q, t, NT)] = (T)(s * (xeta * xeta + yeta * yeta));
-        G[gq_index<T, NQ2>(tile, 1, q, t, NT)] = (T)(-s * (xxi * xeta + yxi * yeta));
-        G[gq_index<T, NQ2>(tile, 2, q, t, NT)] = (T)(s * (xxi * xxi + yxi * yxi));
+        if (wts) s /= m[6] * m[6];
+        put(0, q, t, s * (xeta * xeta + yeta * yeta));
+        put(1, q, t, -s * (xxi * xeta + yxi * yeta));
+        put(2, q, t, s * (xxi * xxi + yxi * yxi));
+        if (wts) {
+          put(3, q, t, m[7] / m[6]);
+          put(4, q, t, m[8] / m[6]);
+        }
       }
   }
 }
@@ -658,8 +841,8 @@ __global__ void k_geometry(T *G, const double *tabx, const double *taby,
 // --------------------------------------------------------------------------
 template <int P, int NQ>
 __global__ void k_load_qp(double *fJ, const double *tabx, const double *taby,
-                          const double *cp, const double *wq, int nex, int ney,
-                          int nfx, int kind, double value) {
+                          const double *cp, const double *wts, const double *wq, int nex,
+                          int ney, int nfx, int kind, double value) {
   constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nex * ney) return;
@@ -669,31 +852,23 @@ __global__ void k_load_qp(double *fJ, const double *tabx, const double *taby,
   const double pi = 3.14159265358979323846264338327950288;
   for (int j = 0; j < NQ; ++j)
     for (int i = 0; i < NQ; ++i) {
-      double x = 0, y = 0, xxi = 0, xeta = 0, yxi = 0, yeta = 0;
-      for (int b = 0; b < NB; ++b)
-        for (int a = 0; a < NB; ++a) {
-          const size_t A = (size_t)(ey + b) * nfx + ex + a;
-          const double R = t1[i * NB + a] * t2[j * NB + b];
-          const double dxi = t1[TAB + i * NB + a] * t2[j * NB + b];
-          const double deta = t1[i * NB + a] * t2[TAB + j * NB + b];
-          x += cp[2 * A] * R;
-          y += cp[2 * A + 1] * R;
-          xxi += cp[2 * A] * dxi;
-          xeta += cp[2 * A] * deta;
-          yxi += cp[2 * A + 1] * dxi;
-          yeta += cp[2 * A + 1] * deta;
-        }
+      double m[9];
+      map_q<P, NQ>(cp, wts, nfx, ex, ey, t1, t2, i, j, m);
+      const double x = m[0], y = m[1];
       double f;
       if (kind == 0) f = x * (1.0 - x) * y * (1.0 - y);
       else if (kind == 1) f = value;
       else f = value * sin(pi * x) * sin(pi * y);
-      fJ[(size_t)e * NQ2 + j * NQ + i] = f * wq[i] * wq[j] * (xxi * yeta - xeta * yxi);
+      // general rational: R_A = w_A B_A / W, the 1/W here, w_A in the gather
+      double v = f * wq[i] * wq[j] * (m[2] * m[5] - m[3] * m[4]);
+      if (wts) v /= m[6];
+      fJ[(size_t)e * NQ2 + j * NQ + i] = v;
     }
 }
 
 template <int P, int NQ>
 __global__ void k_load_gather(double *F, const double *fJ, const double *tabx,
-                              const double *taby, const uint8_t *dirmask,
+                              const double *taby, const double *wts, const uint8_t *dirmask,
                               int nex, int ney, int nfx, int nfy) {
   constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
   const int A = blockIdx.x * blockDim.x + threadIdx.x;
@@ -719,7 +894,7 @@ __global__ void k_load_gather(double *F, const double *fJ, const double *tabx,
       acc += s;
     }
   }
-  F[A] = acc;
+  F[A] = wts ? acc * wts[A] : acc;
 }
 
 // --------------------------------------------------------------------------
@@ -728,7 +903,7 @@ __global__ void k_load_gather(double *F, const double *fJ, const double *tabx,
 template <int P, int NQ>
 __global__ void k_kappa_grad(double *out, const double *x, const double *g,
                              const double *G0, const double *tabx,
-                             const double *taby, const uint8_t *dirmask,
+                             const double *taby, const double *wts, const uint8_t *dirmask,
                              int nex, int ney, int nfx, int nfy, int TX, int TY,
                              int ntx) {
   constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
@@ -739,16 +914,24 @@ __global__ void k_kappa_grad(double *out, const double *x, const double *g,
   const int tile = (ey / TY) * ntx + ex / TX, t = (ey % TY) * TX + ex % TX;
   const double *t1 = tabx + (size_t)ex * 2 * TAB;
   const double *t2 = taby + (size_t)ey * 2 * TAB;
+  auto gq = [&](int c, int q) {
+    return wts ? G0[gq_index<double, NQ2, 5>(tile, c, q, t, NT)]
+               : G0[gq_index<double, NQ2, 3>(tile, c, q, t, NT)];
+  };
   double xe[NB][NB], ge[NB][NB];
   for (int b = 0; b < NB; ++b)
     for (int a = 0; a < NB; ++a) {
       const int gx = ex + a, gy = ey + b, A = gy * nfx + gx;
       xe[b][a] = x[A];
       ge[b][a] = is_dirichlet(dirmask, A, gx, gy, nfx, nfy) ? 0.0 : g[A];
+      if (wts) {
+        xe[b][a] *= wts[A];
+        ge[b][a] *= wts[A];
+      }
     }
   for (int j = 0; j < NQ; ++j)
     for (int i = 0; i < NQ; ++i) {
-      double ux = 0, uy = 0, vx = 0, vy = 0;
+      double ux = 0, uy = 0, vx = 0, vy = 0, uv = 0, vv = 0;
       for (int b = 0; b < NB; ++b)
         for (int a = 0; a < NB; ++a) {
           const double dxi = t1[TAB + i * NB + a] * t2[j * NB + b];
@@ -757,11 +940,21 @@ __global__ void k_kappa_grad(double *out, const double *x, const double *g,
           uy += deta * xe[b][a];
           vx += dxi * ge[b][a];
           vy += deta * ge[b][a];
+          if (wts) {
+            const double R = t1[i * NB + a] * t2[j * NB + b];
+            uv += R * xe[b][a];
+            vv += R * ge[b][a];
+          }
         }
       const int q = j * NQ + i;
-      const double a11 = G0[gq_index<double, NQ2>(tile, 0, q, t, NT)];
-      const double a12 = G0[gq_index<double, NQ2>(tile, 1, q, t, NT)];
-      const double a22 = G0[gq_index<double, NQ2>(tile, 2, q, t, NT)];
+      if (wts) {
+        const double ax = gq(3, q), ay = gq(4, q);
+        ux -= uv * ax;
+        uy -= uv * ay;
+        vx -= vv * ax;
+        vy -= vv * ay;
+      }
+      const double a11 = gq(0, q), a12 = gq(1, q), a22 = gq(2, q);
       out[(size_t)e * NQ2 + q] = -(ux * (a11 * vx + a12 * vy) + uy * (a12 * vx + a22 * vy));
     }
 }
@@ -772,7 +965,7 @@ __global__ void k_kappa_grad(double *out, const double *x, const double *g,
 // --------------------------------------------------------------------------
 template <int P, int NQ>
 __global__ void k_diag(double *diag, const double *G, const double *tabx,
-                       const double *taby, const uint8_t *dirmask, int nex,
+                       const double *taby, const double *wts, const uint8_t *dirmask, int nex,
                        int ney, int nfx, int nfy, int TX, int TY, int ntx) {
   constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
   const int A = blockIdx.x * blockDim.x + threadIdx.x;
@@ -793,18 +986,25 @@ __global__ void k_diag(double *diag, const double *G, const double *tabx,
       const double *t1 = tabx + (size_t)ex * 2 * TAB;
       const double *t2 = taby + (size_t)ey * 2 * TAB;
       const int tile = (ey / TY) * ntx + ex / TX, t = (ey % TY) * TX + ex % TX;
+      auto gq = [&](int c, int q) {
+        return wts ? G[gq_index<double, NQ2, 5>(tile, c, q, t, NT)]
+                   : G[gq_index<double, NQ2, 3>(tile, c, q, t, NT)];
+      };
       for (int j = 0; j < NQ; ++j)
         for (int i = 0; i < NQ; ++i) {
           const int q = j * NQ + i;
-          const double dx = t1[TAB + i * NB + a] * t2[j * NB + b];
-          const double dy = t1[i * NB + a] * t2[TAB + j * NB + b];
-          acc -= G[gq_index<double, NQ2>(tile, 0, q, t, NT)] * dx * dx +
-                 2.0 * G[gq_index<double, NQ2>(tile, 1, q, t, NT)] * dx * dy +
-                 G[gq_index<double, NQ2>(tile, 2, q, t, NT)] * dy * dy;
+          double dx = t1[TAB + i * NB + a] * t2[j * NB + b];
+          double dy = t1[i * NB + a] * t2[TAB + j * NB + b];
+          if (wts) {
+            const double R = t1[i * NB + a] * t2[j * NB + b];
+            dx -= R * gq(3, q);
+            dy -= R * gq(4, q);
+          }
+          acc -= gq(0, q) * dx * dx + 2.0 * gq(1, q) * dx * dy + gq(2, q) * dy * dy;
         }
     }
   }
-  diag[A] = acc;
+  diag[A] = wts ? acc * wts[A] * wts[A] : acc;
 }
 
 }  // namespace igak

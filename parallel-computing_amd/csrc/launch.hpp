@@ -23,22 +23,34 @@ inline int tile_id(int tx, int ty) {
   return -1;
 }
 
+// compiled (tile, variant) pairs: high degrees spill at 512+ threads
+inline bool tile_compiled(int p, int id) {
+  return p <= 4 || id == TILE_32x16 || id == TILE_16x16;
+}
+inline bool pf_compiled(int p, int id) { return p <= 4 || id == TILE_16x16; }
+inline bool rat_compiled(int p, int id) {
+  return id == TILE_16x16 || (p <= 4 && id == TILE_32x16);
+}
+
 template <int P>
 struct Launch {
   static constexpr int NQ = P + 1;
 
-  template <typename T, int TX, int TY, bool EBE>
+  template <typename T, int TX, int TY, bool EBE, bool RAT = false>
   static hipError_t apply_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE>), dim3(ntiles), dim3(TX * TY), 0, s, a);
+    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE, RAT>), dim3(ntiles), dim3(TX * TY), 0, s,
+                       a);
     return hipGetLastError();
   }
   template <typename T, bool EBE>
   static hipError_t apply_tile(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {
+    if constexpr (P <= 4) {
+      if (tile == TILE_32x32) return apply_t<T, 32, 32, EBE>(a, ntiles, s);
+      if (tile == TILE_64x8) return apply_t<T, 64, 8, EBE>(a, ntiles, s);
+    }
     switch (tile) {
-      case TILE_32x32: return apply_t<T, 32, 32, EBE>(a, ntiles, s);
       case TILE_32x16: return apply_t<T, 32, 16, EBE>(a, ntiles, s);
       case TILE_16x16: return apply_t<T, 16, 16, EBE>(a, ntiles, s);
-      case TILE_64x8: return apply_t<T, 64, 8, EBE>(a, ntiles, s);
     }
     return hipErrorInvalidValue;
   }
@@ -62,12 +74,14 @@ struct Launch {
   }
   template <typename T>
   static hipError_t apply_pf_tile(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {
-    switch (tile) {
-      case TILE_32x32: return apply_pf_t<T, 32, 32>(a, ntiles, s);
-      case TILE_32x16: return apply_pf_t<T, 32, 16>(a, ntiles, s);
-      case TILE_16x16: return apply_pf_t<T, 16, 16>(a, ntiles, s);
-      case TILE_64x8: return apply_pf_t<T, 64, 8>(a, ntiles, s);
+    if constexpr (P <= 4) {
+      switch (tile) {
+        case TILE_32x32: return apply_pf_t<T, 32, 32>(a, ntiles, s);
+        case TILE_32x16: return apply_pf_t<T, 32, 16>(a, ntiles, s);
+        case TILE_64x8: return apply_pf_t<T, 64, 8>(a, ntiles, s);
+      }
     }
+    if (tile == TILE_16x16) return apply_pf_t<T, 16, 16>(a, ntiles, s);
     return hipErrorInvalidValue;
   }
   // variant: 0 one workgroup per tile, 1 persistent pipelined (SF only)
@@ -96,52 +110,81 @@ struct Launch {
   }
 
   static hipError_t diag(double *d, const double *G, const double *tabx, const double *taby,
-                         const uint8_t *mask, int nex, int ney, int nfx, int nfy,
-                         const Layout &L, hipStream_t s) {
+                         const double *wts, const uint8_t *mask, int nex, int ney, int nfx,
+                         int nfy, const Layout &L, hipStream_t s) {
     const int bs = 256, nf = nfx * nfy;
     hipLaunchKernelGGL((k_diag<P, NQ>), dim3((nf + bs - 1) / bs), dim3(bs), 0, s, d, G, tabx,
-                       taby, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx);
+                       taby, wts, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx);
     return hipGetLastError();
   }
 
   template <typename T>
   static hipError_t geometry(T *G, const double *tabx, const double *taby, const double *cp,
-                             const double *wq, const double *kappa, int nex, int ney, int nfx,
-                             const Layout &L, hipStream_t s) {
+                             const double *wts, const double *wq, const double *kappa, int nex,
+                             int ney, int nfx, const Layout &L, hipStream_t s) {
     const int nt = L.ntx * L.nty;
     int bs = L.tx * L.ty;
     if (bs > 256) bs = 256;
-    hipLaunchKernelGGL((k_geometry<T, P, NQ>), dim3(nt), dim3(bs), 0, s, G, tabx, taby, cp, wq,
-                       kappa, nex, ney, nfx, L.tx, L.ty, L.ntx);
+    hipLaunchKernelGGL((k_geometry<T, P, NQ>), dim3(nt), dim3(bs), 0, s, G, tabx, taby, cp, wts,
+                       wq, kappa, nex, ney, nfx, L.tx, L.ty, L.ntx);
     return hipGetLastError();
   }
 
   static hipError_t load(double *F, double *fJ, const double *tabx, const double *taby,
-                         const double *cp, const double *wq, const uint8_t *mask, int nex,
-                         int ney, int nfx, int nfy, int kind, double value, hipStream_t s) {
+                         const double *cp, const double *wts, const double *wq,
+                         const uint8_t *mask, int nex, int ney, int nfx, int nfy, int kind,
+                         double value, hipStream_t s) {
     const int bs = 256;
     const int ne = nex * ney, nf = nfx * nfy;
     hipLaunchKernelGGL((k_load_qp<P, NQ>), dim3((ne + bs - 1) / bs), dim3(bs), 0, s, fJ, tabx,
-                       taby, cp, wq, nex, ney, nfx, kind, value);
+                       taby, cp, wts, wq, nex, ney, nfx, kind, value);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_load_gather<P, NQ>), dim3((nf + bs - 1) / bs), dim3(bs), 0, s, F, fJ,
-                       tabx, taby, mask, nex, ney, nfx, nfy);
+                       tabx, taby, wts, mask, nex, ney, nfx, nfy);
     return hipGetLastError();
   }
 
   static hipError_t kappa_grad(double *out, const double *x, const double *g, const double *G0,
-                               const double *tabx, const double *taby, const uint8_t *mask,
-                               int nex, int ney, int nfx, int nfy, const Layout &L,
-                               hipStream_t s) {
+                               const double *tabx, const double *taby, const double *wts,
+                               const uint8_t *mask, int nex, int ney, int nfx, int nfy,
+                               const Layout &L, hipStream_t s) {
     const int bs = 256, ne = nex * ney;
     hipLaunchKernelGGL((k_kappa_grad<P, NQ>), dim3((ne + bs - 1) / bs), dim3(bs), 0, s, out, x,
-                       g, G0, tabx, taby, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx);
+                       g, G0, tabx, taby, wts, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx);
     return hipGetLastError();
   }
 };
 
-// per-degree entry points (defined in kern_p<P>.hip)
+// General rational meshes (per-function weights): one workgroup per tile,
+// 16x16 tiles (and 32x16 for p <= 4); instantiated in kern_p<P>_rat.hip.
+template <int P>
+struct LaunchRat {
+  template <typename T, int TX, int TY, bool EBE>
+  static hipError_t run_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
+    hipLaunchKernelGGL((k_apply<T, P, P + 1, TX, TY, EBE, true>), dim3(ntiles), dim3(TX * TY), 0,
+                       s, a);
+    return hipGetLastError();
+  }
+  template <typename T, bool EBE>
+  static hipError_t run_tile(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {
+    if constexpr (P <= 4)
+      if (tile == TILE_32x16) return run_t<T, 32, 16, EBE>(a, ntiles, s);
+    if (tile == TILE_16x16) return run_t<T, 16, 16, EBE>(a, ntiles, s);
+    return hipErrorInvalidValue;
+  }
+  static hipError_t apply(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles,
+                          hipStream_t s) {
+    if (f64) return ebe ? run_tile<double, true>(a, tile, ntiles, s)
+                        : run_tile<double, false>(a, tile, ntiles, s);
+    return ebe ? run_tile<float, true>(a, tile, ntiles, s)
+               : run_tile<float, false>(a, tile, ntiles, s);
+  }
+};
+
+struct BlockArgs;
+
+// per-degree entry points (kern_p<P>.hip, kern_p<P>_rat.hip, kern_p<P>_block.hip)
 struct Ops {
   hipError_t (*apply)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles, int variant,
                       hipStream_t);
@@ -149,20 +192,23 @@ struct Ops {
                        const Layout &, int, int, const int *, hipStream_t);
   hipError_t (*seam32)(float *, const float *, const uint8_t *, int, int, int, int,
                        const Layout &, int, int, const int *, hipStream_t);
-  hipError_t (*diag)(double *, const double *, const double *, const double *, const uint8_t *,
-                     int, int, int, int, const Layout &, hipStream_t);
+  hipError_t (*diag)(double *, const double *, const double *, const double *, const double *,
+                     const uint8_t *, int, int, int, int, const Layout &, hipStream_t);
   hipError_t (*geom64)(double *, const double *, const double *, const double *,
-                       const double *, const double *, int, int, int, const Layout &,
-                       hipStream_t);
+                       const double *, const double *, const double *, int, int, int,
+                       const Layout &, hipStream_t);
   hipError_t (*geom32)(float *, const double *, const double *, const double *,
-                       const double *, const double *, int, int, int, const Layout &,
-                       hipStream_t);
+                       const double *, const double *, const double *, int, int, int,
+                       const Layout &, hipStream_t);
   hipError_t (*load)(double *, double *, const double *, const double *, const double *,
-                     const double *, const uint8_t *, int, int, int, int, int, double,
-                     hipStream_t);
+                     const double *, const double *, const uint8_t *, int, int, int, int, int,
+                     double, hipStream_t);
   hipError_t (*kappa_grad)(double *, const double *, const double *, const double *,
-                           const double *, const double *, const uint8_t *, int, int, int,
-                           int, const Layout &, hipStream_t);
+                           const double *, const double *, const double *, const uint8_t *, int,
+                           int, int, int, const Layout &, hipStream_t);
+  hipError_t (*apply_rat)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles,
+                          hipStream_t);
+  hipError_t (*block)(const BlockArgs &, int cfg, hipStream_t);
 };
 
 template <int P>
@@ -176,6 +222,8 @@ Ops make_ops() {
   o.load = &Launch<P>::load;
   o.diag = &Launch<P>::diag;
   o.kappa_grad = &Launch<P>::kappa_grad;
+  o.apply_rat = nullptr;
+  o.block = nullptr;
   return o;
 }
 
@@ -184,10 +232,25 @@ const Ops *ops_for_degree(int p);
 
 }  // namespace igak
 
-#define IGAK_DEFINE_DEGREE(P)                      \
-  namespace igak {                                 \
-  const Ops *ops_p##P() {                          \
-    static const Ops o = make_ops<P>();            \
-    return &o;                                     \
-  }                                                \
+#define IGAK_DEFINE_DEGREE(P)                                                             \
+  namespace igak {                                                                        \
+  hipError_t apply_rat_p##P(const ApplyArgs &, bool, bool, int, int, hipStream_t);        \
+  hipError_t block_p##P(const BlockArgs &, int, hipStream_t);                             \
+  const Ops *ops_p##P() {                                                                 \
+    static const Ops o = [] {                                                             \
+      Ops r = make_ops<P>();                                                              \
+      r.apply_rat = &apply_rat_p##P;                                                      \
+      r.block = &block_p##P;                                                              \
+      return r;                                                                           \
+    }();                                                                                  \
+    return &o;                                                                            \
+  }                                                                                       \
+  }
+
+#define IGAK_DEFINE_RAT(P)                                                                \
+  namespace igak {                                                                        \
+  hipError_t apply_rat_p##P(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles, \
+                            hipStream_t s) {                                              \
+    return LaunchRat<P>::apply(a, f64, ebe, tile, ntiles, s);                             \
+  }                                                                                       \
   }

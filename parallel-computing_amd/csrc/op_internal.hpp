@@ -23,13 +23,16 @@ struct iga_op {
   int device = 0;
   int p = 0, nq = 0, nex = 0, ney = 0, nfx = 0, nfy = 0;
   int nfunc = 0, nelem = 0, nqp = 0, nloc = 0, ndir = 0;
-  int rational = 0;
+  int rational = 0;                 // 0 polynomial, 1 tensor-product weights, 2 general
+  int nc = 3;                       // G_q components (5 on general rational meshes)
   bool ring = true;                 // Dirichlet set is the boundary ring
   const igak::Ops *ops = nullptr;
   uint8_t *dirmask = nullptr;       // device, only when !ring
   double *tabx64 = nullptr, *taby64 = nullptr;
   float *tabx32 = nullptr, *taby32 = nullptr;
   double *cp = nullptr;             // device [nfunc][2]
+  double *wts64 = nullptr;          // per-function weights, general rational meshes only
+  float *wts32 = nullptr;
   double *wq = nullptr;             // device Gauss weights on [0,1]
   igak::Layout L64{}, L32{};
   int tile64 = 0, tile32 = 0;       // igak::TileId
@@ -46,6 +49,8 @@ struct iga_op {
   int64_t bytes = 0;
   double setup_ms = 0.0;
   std::vector<void *> allocs;
+  // block apply: tuned configuration per number of right-hand sides
+  std::vector<std::pair<int, int>> block_tuned;  // (k, cfg)
   // k_apply timing (iga_op_profile)
   bool prof_on = false;
   int prof_cap = 0, prof_used = 0;
@@ -60,4 +65,7 @@ int set_device(int dev);
 // y = A x with an explicit layout / precision (used by the solver)
 int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
               int mask_in, int mask_out, hipStream_t s, const int *skip);
+// Y = A X for k FP32 columns (leading dimension ld), the configuration
+// tuned or defaulted for k; no device switch (callable under graph capture)
+int apply_block_raw(iga_op *op, int k, int ld, const float *X, float *Y, hipStream_t s);
 }  // namespace igai

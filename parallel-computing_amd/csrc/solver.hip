@@ -147,9 +147,21 @@ __global__ void k_cg_init_fin(const double *part, int nb, double *sc, int *ic, d
 }
 
 // K1: partials of p.Ap; bookkeeping for K3
+// Column j = blockIdx.y of a block solve: vectors at j * ld, scalars, flags
+// and partial slots of instance j (ld = 0 and one column otherwise).
+#define CG_COLUMN(ld)                                 \
+  const int col_ = blockIdx.y;                        \
+  const int64_t voff_ = (int64_t)col_ * (ld);         \
+  part += (size_t)col_ * P_NP * RG;                   \
+  sc += (size_t)col_ * S_NS;                          \
+  ic += (size_t)col_ * I_NI;
+
 template <typename V>
 __global__ void k_cg_pap(const V *p, const V *Ap, int64_t n, double *part, double *sc, int *ic,
-                         const int *outer_done) {
+                         const int *outer_done, int64_t ld) {
+  CG_COLUMN(ld)
+  p += voff_;
+  Ap += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
   double s = 0.0;
   GRID_LOOP(i, n) s += (double)p[i] * (double)Ap[i];
@@ -166,7 +178,15 @@ __global__ void k_cg_pap(const V *p, const V *Ap, int64_t n, double *part, doubl
 template <typename V>
 __global__ void k_cg_update(V *u, V *r, const V *p, const V *Ap, V *z, V *rold,
                             const double *dinv, int local_pc, int prec_norm, int64_t n,
-                            double *part, double *sc, int *ic, int nb, const int *outer_done) {
+                            double *part, double *sc, int *ic, int nb, const int *outer_done,
+                            int64_t ld) {
+  CG_COLUMN(ld)
+  u += voff_;
+  r += voff_;
+  p += voff_;
+  Ap += voff_;
+  z += voff_;
+  if (rold) rold += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
   const double pap = fold(part + P_PAP * RG, nb);
   const double a = sc[S_RZ] / pap;
@@ -212,7 +232,10 @@ __global__ void k_cg_rz(const V *r, const V *z, const V *rold, int64_t n, double
 // K3: convergence test, beta, p = z + beta p
 template <typename V>
 __global__ void k_cg_pupdate(V *p, const V *z, int64_t n, const double *part, double *sc,
-                             int *ic, int nb, int flexible, const int *outer_done) {
+                             int *ic, int nb, int flexible, const int *outer_done, int64_t ld) {
+  CG_COLUMN(ld)
+  p += voff_;
+  z += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
   const double rr = fold(part + P_RR * RG, nb);
   const double rz = fold(part + P_RZ * RG, nb);
@@ -448,6 +471,16 @@ struct iga_solver {
   int gen = -1;  // operator generation the Jacobi diagonal was built for
   // graphs of one chunk of iterations (FP64 PCG, FP32 inner PCG)
   hipGraphExec_t g64 = nullptr, g32 = nullptr;
+  // block solve (k columns of nfunc): FP64 iterates, FP32 inner vectors,
+  // per-column scalars / flags / partial slots, graph of one chunk
+  int bk = 0;
+  double *bx64 = nullptr;
+  float *bd32 = nullptr, *br32 = nullptr, *bp32 = nullptr, *bAp32 = nullptr, *bz32 = nullptr;
+  double *bsc = nullptr, *bpart = nullptr;
+  int *bic = nullptr;
+  std::vector<void *> ballocs;
+  hipGraphExec_t gblk = nullptr;
+  int gblk_k = 0, gblk_cfg = -1;
 };
 
 namespace {
@@ -486,10 +519,10 @@ int fem_pc(iga_solver *s, const V *r, V *z, V *rin, V *w, V *pp, V *q, const V *
   for (int it = 0; it < s->o.pc_inner_maxit; ++it) {
     hipLaunchKernelGGL((k_spmv<V>), dim3((s->fem_rows + bs - 1) / bs), dim3(bs), 0, st, s->rowptr,
                        s->col, val, pp, q, s->fem_rows, ic + I_DONE, od);
-    LAUNCH((k_cg_pap<V>), nb, pp, q, n, s->part_in, sc, ic, od);
+    LAUNCH((k_cg_pap<V>), nb, pp, q, n, s->part_in, sc, ic, od, (int64_t)0);
     LAUNCH((k_cg_update<V>), nb, z, rin, pp, q, w, (V *)nullptr, s->fdinv, 1, 1, n, s->part_in,
-           sc, ic, nb, od);
-    LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od);
+           sc, ic, nb, od, (int64_t)0);
+    LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od, (int64_t)0);
   }
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
@@ -513,18 +546,18 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
   int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE);
   if (rc) return rc;
-  LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr);
+  LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr, (int64_t)0);
   const bool loc = local_pc(s);
   const double *dinv = s->o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
   LAUNCH((k_cg_update<V>), nb, u, r, p, Ap, z, loc ? (V *)nullptr : rold, dinv, loc ? 1 : 0, 0,
-         n, s->part, s->sc, s->ic, nb, (const int *)nullptr);
+         n, s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0);
   if (!loc) {
     rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
     LAUNCH((k_cg_rz<V>), nb, r, z, rold, n, s->part, s->ic);
   }
   LAUNCH((k_cg_pupdate<V>), nb, p, z, n, s->part, s->sc, s->ic, nb, loc ? 0 : 1,
-         (const int *)nullptr);
+         (const int *)nullptr, (int64_t)0);
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
@@ -625,6 +658,98 @@ int outer_residual(iga_solver *s, const double *f, const double *u, double *r, d
   return IGA_OK;
 }
 
+// ---------------------------------------------------------------- block solve
+// k independent systems, each an FP32 PCG with FP64 reliable updates (as
+// PCG_F32_REFINE), advanced in lockstep: one multi-RHS apply per iteration
+// and the CG kernels with one grid row (blockIdx.y) per column.  A column
+// that asks for a reliable update, converges or stops is frozen by its own
+// done flag until the host has handled it between chunks.
+
+int block_buffers(iga_solver *s, int k) {
+  if (k <= s->bk) return IGA_OK;
+  for (void *p : s->ballocs) (void)hipFree(p);
+  s->ballocs.clear();
+  if (s->gblk) {
+    (void)hipGraphExecDestroy(s->gblk);
+    s->gblk = nullptr;
+  }
+  s->bk = 0;
+  const size_t nk = (size_t)s->n * k;
+  auto al = [&](void **p, size_t bytes) {
+    if (hipMalloc(p, bytes ? bytes : 256) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return IGA_ERR_OUT_OF_MEMORY;
+    }
+    s->ballocs.push_back(*p);
+    return IGA_OK;
+  };
+  int rc = 0;
+  rc = rc ? rc : al((void **)&s->bx64, nk * 8);
+  rc = rc ? rc : al((void **)&s->bd32, nk * 4);
+  rc = rc ? rc : al((void **)&s->br32, nk * 4);
+  rc = rc ? rc : al((void **)&s->bp32, nk * 4);
+  rc = rc ? rc : al((void **)&s->bAp32, nk * 4);
+  rc = rc ? rc : al((void **)&s->bz32, nk * 4);
+  rc = rc ? rc : al((void **)&s->bsc, (size_t)k * S_NS * 8);
+  rc = rc ? rc : al((void **)&s->bic, (size_t)k * I_NI * 4);
+  rc = rc ? rc : al((void **)&s->bpart, (size_t)k * P_NP * RG * 8);
+  if (rc) return rc;
+  s->bk = k;
+  return IGA_OK;
+}
+
+int bcg_chunk(iga_solver *s, int k, hipStream_t st) {
+  const int64_t n = s->n;
+  const int nb = s->nb;
+  const double *dinv = s->o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
+  const dim3 grid(nb, k);
+  for (int it = 0; it < CHUNK; ++it) {
+    int rc = igai::apply_block_raw(s->op, k, (int)n, s->bp32, s->bAp32, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_cg_pap<float>), grid, dim3(RB), 0, st, s->bp32, s->bAp32, n, s->bpart,
+                       s->bsc, s->bic, (const int *)nullptr, n);
+    hipLaunchKernelGGL((k_cg_update<float>), grid, dim3(RB), 0, st, s->bd32, s->br32, s->bp32,
+                       s->bAp32, s->bz32, (float *)nullptr, dinv, 1, 0, n, s->bpart, s->bsc,
+                       s->bic, nb, (const int *)nullptr, n);
+    hipLaunchKernelGGL((k_cg_pupdate<float>), grid, dim3(RB), 0, st, s->bp32, s->bz32, n,
+                       s->bpart, s->bsc, s->bic, nb, 0, (const int *)nullptr, n);
+  }
+  return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+}
+
+int bcg_run_chunk(iga_solver *s, int k, hipStream_t st) {
+  int cfg = -1;
+  (void)iga_op_get_block_config(s->op, k, &cfg);
+  if (!s->o.use_graph) return bcg_chunk(s, k, st);
+  if (s->gblk && (s->gblk_k != k || s->gblk_cfg != cfg)) {
+    (void)hipGraphExecDestroy(s->gblk);
+    s->gblk = nullptr;
+  }
+  if (!s->gblk) {
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return IGA_ERR_HIP;
+    hipGraph_t g;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return IGA_ERR_HIP;
+    }
+    int rc = bcg_chunk(s, k, cs);
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
+    e = hipGraphInstantiate(&s->gblk, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      s->gblk = nullptr;
+      return IGA_ERR_HIP;
+    }
+    s->gblk_k = k;
+    s->gblk_cfg = cfg;
+  }
+  return hipGraphLaunch(s->gblk, st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+}
+
 }  // namespace
 
 extern "C" {
@@ -653,7 +778,9 @@ int iga_solver_destroy(iga_solver *s) {
   (void)hipSetDevice(s->device);
   if (s->g64) (void)hipGraphExecDestroy(s->g64);
   if (s->g32) (void)hipGraphExecDestroy(s->g32);
+  if (s->gblk) (void)hipGraphExecDestroy(s->gblk);
   for (void *p : s->allocs) (void)hipFree(p);
+  for (void *p : s->ballocs) (void)hipFree(p);
   delete s;
   return IGA_OK;
 }
@@ -761,8 +888,8 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
   if (s->gen != s->op->gen) {
     // Jacobi of A for the operator's current G_q
     iga_op *op = s->op;
-    if (op->ops->diag(s->dinv, op->G64, op->tabx64, op->taby64, op->dirmask, op->nex, op->ney,
-                      op->nfx, op->nfy, op->L64, st) != hipSuccess)
+    if (op->ops->diag(s->dinv, op->G64, op->tabx64, op->taby64, op->wts64, op->dirmask, op->nex,
+                      op->ney, op->nfx, op->nfy, op->L64, st) != hipSuccess)
       return IGA_ERR_HIP;
     LAUNCH(k_inv, nb, s->dinv, n);
     s->gen = op->gen;
@@ -857,6 +984,137 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
   stt.status = status;
   if (stats) *stats = stt;
   return status;
+}
+
+int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double *U,
+                           void *stream, iga_solve_stats *stats) {
+  if (!s || k < 1 || !F || !U || ld < s->n) return IGA_ERR_INVALID_ARGUMENT;
+  const iga_solver_opts &o = s->o;
+  if (o.kind != IGA_SOLVE_PCG_F32_REFINE || !local_pc(s) || s->op->rational == 2 ||
+      !s->op->G32)
+    return IGA_ERR_UNSUPPORTED;
+  int rc = igai::set_device(s->device);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = s->n;
+  const int nb = s->nb;
+  rc = block_buffers(s, k);
+  if (rc) return rc;
+  const double *dinv = o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
+  if (s->gen != s->op->gen) {
+    iga_op *op = s->op;
+    if (op->ops->diag(s->dinv, op->G64, op->tabx64, op->taby64, op->wts64, op->dirmask, op->nex,
+                      op->ney, op->nfx, op->nfy, op->L64, st) != hipSuccess)
+      return IGA_ERR_HIP;
+    LAUNCH(k_inv, nb, s->dinv, n);
+    s->gen = op->gen;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
+  const auto t0 = std::chrono::steady_clock::now();
+  const double RU2 = o.inner_rtol * o.inner_rtol;
+  std::vector<double> ff(k), tol2(k), rrf(k);
+  std::vector<int> status(k, IGA_ERR_NOT_CONVERGED), outer(k, 0), iters(k, 0), fin(k, 0);
+  auto colp = [&](int j) { return s->bpart + (size_t)j * P_NP * RG; };
+  auto colsc = [&](int j) { return s->bsc + (size_t)j * S_NS; };
+  auto colic = [&](int j) { return s->bic + (size_t)j * I_NI; };
+  if (hipMemsetAsync(s->bx64, 0, (size_t)k * n * 8, st) != hipSuccess) return IGA_ERR_HIP;
+  int running = 0;
+  for (int j = 0; j < k; ++j) {
+    const double *fj = F + (size_t)j * ld;
+    double h[2];
+    rc = outer_residual(s, fj, s->bx64 + (size_t)j * n, s->r, h, st);
+    if (rc) return rc;
+    ff[j] = h[1];
+    rrf[j] = h[0];
+    const double tol = std::fmax(o.rtol * std::sqrt(ff[j]), o.abstol);
+    tol2[j] = tol * tol;
+    LAUNCH((k_cg_init<float, double>), nb, s->r, 1.0, s->bd32 + (size_t)j * n,
+           s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, s->bp32 + (size_t)j * n, dinv, 1,
+           n, colp(j));
+    const double rr = h[0];
+    const bool done = !(rr == rr) || rr <= tol2[j];
+    hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, colp(j), nb, colsc(j), colic(j),
+                       done ? 1.0 : tol2[j] / rr, 0.0, o.divtol * o.divtol * ff[j] / rr,
+                       o.maxit, RU2);
+    if (!(rr == rr)) {
+      status[j] = IGA_ERR_DIVERGED;
+      fin[j] = 1;
+    } else if (rr <= tol2[j]) {
+      status[j] = IGA_OK;
+      fin[j] = 1;
+    } else {
+      ++running;
+    }
+  }
+  // columns final at the start carry a nonzero done flag (k_cg_init_fin with
+  // rtol2 = 1, or a non-finite residual): the chunk kernels skip them
+  std::vector<int> hic((size_t)k * I_NI);
+  while (running) {
+    rc = bcg_run_chunk(s, k, st);
+    if (rc) return rc;
+    if (hipMemcpyAsync(hic.data(), s->bic, hic.size() * 4, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return IGA_ERR_HIP;
+    running = 0;
+    for (int j = 0; j < k; ++j) {
+      if (fin[j]) continue;
+      const int code = hic[(size_t)j * I_NI + I_DONE];
+      iters[j] = hic[(size_t)j * I_NI + I_ITER];
+      if (code == 0) {
+        ++running;
+        continue;
+      }
+      if (code == 3) {
+        status[j] = IGA_ERR_DIVERGED;
+        fin[j] = 1;
+        continue;
+      }
+      // reliable update of column j with the FP64 operator
+      ++outer[j];
+      double *xj = s->bx64 + (size_t)j * n;
+      LAUNCH(k_ru_accum, nb, xj, s->bd32 + (size_t)j * n, n);
+      double h[2];
+      rc = outer_residual(s, F + (size_t)j * ld, xj, s->r, h, st);
+      if (rc) return rc;
+      rrf[j] = h[0];
+      if (!(h[0] == h[0]) || h[0] > o.divtol * o.divtol * ff[j]) {
+        status[j] = IGA_ERR_DIVERGED;
+        fin[j] = 1;
+      } else if (h[0] <= tol2[j]) {
+        status[j] = IGA_OK;
+        fin[j] = 1;
+      } else if (code == 2) {
+        status[j] = IGA_ERR_NOT_CONVERGED;
+        fin[j] = 1;
+      } else {
+        LAUNCH(k_ru_reset, nb, s->r, s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, dinv, 1,
+               n, colp(j));
+        hipLaunchKernelGGL(k_ru_fin, dim3(1), dim3(RB), 0, st, colp(j), nb, colsc(j), colic(j),
+                           RU2 * h[0]);
+        ++running;
+      }
+    }
+  }
+  for (int j = 0; j < k; ++j)
+    if (hipMemcpyAsync(U + (size_t)j * ld, s->bx64 + (size_t)j * n, n * 8,
+                       hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return IGA_ERR_HIP;
+  if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  int first_bad = IGA_OK;
+  for (int j = 0; j < k; ++j) {
+    if (stats) {
+      stats[j].iterations = iters[j];
+      stats[j].outer_iterations = outer[j];
+      stats[j].rel_residual = ff[j] > 0 ? std::sqrt(rrf[j] / ff[j]) : std::sqrt(rrf[j]);
+      stats[j].solve_ms = ms;
+      stats[j].status = status[j];
+    }
+    if (status[j] != IGA_OK && first_bad == IGA_OK) first_bad = status[j];
+  }
+  return first_bad;
 }
 
 int iga_dot_f64(const double *a, const double *b, int64_t n, double *out_host, void *stream) {

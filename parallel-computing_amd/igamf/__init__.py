@@ -24,6 +24,7 @@ GEOM_UNIT_SQUARE, GEOM_QUARTER_ANNULUS, GEOM_PERTURBED_SQUARE = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_FEM_CG, PC_FEM_CHEB = 0, 1, 2, 3
 SOLVE_PCG_F64, SOLVE_PCG_F32_REFINE = 0, 1
 LOAD_REF_POLY, LOAD_CONST, LOAD_SINSIN = 0, 1, 2
+IGA_ERR_DIVERGED, IGA_ERR_NOT_CONVERGED = 10, 11
 
 
 class IgaError(RuntimeError):
@@ -58,7 +59,9 @@ class OpInfo(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("nfunc", "nfx", "nfy", "nelem", "nqp", "nloc", "ndir",
                                        "rational", "tile_x_f64", "tile_y_f64", "tile_x_f32",
                                        "tile_y_f32")] + [("device_bytes", C.c_int64),
-                                                         ("setup_ms", C.c_double)]
+                                                         ("setup_ms", C.c_double),
+                                                         ("variant_f64", C.c_int),
+                                                         ("variant_f32", C.c_int)]
 
 
 class SolverOpts(C.Structure):
@@ -110,6 +113,13 @@ def _lib():
         L.iga_op_set_kappa.argtypes = [VP, VP, VP]
         L.iga_op_apply_backward.argtypes = [VP, VP, VP, VP, VP, VP]
         L.iga_op_apply_block.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP]
+        L.iga_block_config_count.argtypes = []
+        L.iga_block_config_desc.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int)]
+        L.iga_op_set_block_config.argtypes = [VP, C.c_int, C.c_int]
+        L.iga_op_get_block_config.argtypes = [VP, C.c_int, C.POINTER(C.c_int)]
+        L.iga_op_tune_block.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP, C.c_int,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.iga_fem_reference_nodes.argtypes = [C.c_int] * 4 + [C.c_double] * 2 + [VP]
         L.iga_fem_assemble.argtypes = [C.c_int, C.c_int, VP, C.c_int, C.POINTER(VP)]
         L.iga_fem_view.argtypes = [VP, C.POINTER(CsrView)]
@@ -118,6 +128,8 @@ def _lib():
         L.iga_solver_opts_default.restype = None
         L.iga_solver_create.argtypes = [VP, VP, C.POINTER(SolverOpts), C.POINTER(VP)]
         L.iga_solver_solve.argtypes = [VP, VP, VP, VP, C.POINTER(SolveStats)]
+        L.iga_solver_solve_block.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP,
+                                             C.POINTER(SolveStats)]
         L.iga_solver_destroy.argtypes = [VP]
         L.iga_dot_f64.argtypes = [VP, VP, C.c_int64, C.POINTER(C.c_double), VP]
         _LIB = L
@@ -233,7 +245,7 @@ class Operator:
     """The matrix-free operator y = A x on one GPU (iga_op)."""
 
     def __init__(self, mesh, device=0, want_f64=True, want_f32=True, tile_f64=None,
-                 tile_f32=None, kappa=None, variant_f64=0, variant_f32=0):
+                 tile_f32=None, kappa=None, variant_f64=-1, variant_f32=-1):
         import torch
         self.mesh = mesh
         o = OpOpts()
@@ -260,6 +272,7 @@ class Operator:
         self.nfunc = info.nfunc
         self.nelem = info.nelem
         self.nqp = info.nqp
+        self.p = mesh.desc.p
 
     def create_vecs(self, dtype=None):
         import torch
@@ -310,21 +323,60 @@ class Operator:
                                            _stream_ptr(stream)), "apply_backward")
         return gx, gk
 
-    def apply_block(self, X, Y=None, stream=None):
+    def _block_args(self, X, Y):
         import torch
-        assert X.dtype == torch.float32 and X.is_cuda
-        k, n = X.shape  # row-major [k][n] == column-major n x k with ld = n
-        assert n == self.nfunc
+        if X.dtype != torch.float32 or not X.is_cuda or X.dim() != 2:
+            raise ValueError("X must be a 2-D float32 CUDA tensor [k, nfunc]")
+        k, n = X.shape
+        if n != self.nfunc or X.stride(1) != 1:
+            raise ValueError("X rows must be contiguous vectors of length nfunc")
         if Y is None:
             Y = torch.empty_like(X)
-        check(_lib().iga_op_apply_block(self._h, k, n, X.data_ptr(), Y.data_ptr(),
+        if Y.shape != X.shape or Y.stride() != X.stride() or Y.dtype != X.dtype:
+            raise ValueError("Y must match X in shape, strides and dtype")
+        return k, X.stride(0), Y
+
+    def apply_block(self, X, Y=None, stream=None):
+        """Y[j] = A X[j] for the k rows of X (FP32); a row stride above
+        nfunc is the leading dimension."""
+        k, ld, Y = self._block_args(X, Y)
+        check(_lib().iga_op_apply_block(self._h, k, ld, X.data_ptr(), Y.data_ptr(),
                                         _stream_ptr(stream)), "apply_block")
         return Y
+
+    def block_config(self, k):
+        c = C.c_int()
+        check(_lib().iga_op_get_block_config(self._h, k, C.byref(c)), "get_block_config")
+        return c.value
+
+    def set_block_config(self, k, cfg):
+        check(_lib().iga_op_set_block_config(self._h, k, cfg), "set_block_config")
+
+    def tune_block(self, X, Y=None, reps=5, stream=None):
+        """Time every block configuration for k = X.shape[0]; the fastest is
+        used by later apply_block calls with this k.  Returns (cfg, ms)."""
+        k, ld, Y = self._block_args(X, Y)
+        cfg, ms = C.c_int(), C.c_double()
+        check(_lib().iga_op_tune_block(self._h, k, ld, X.data_ptr(), Y.data_ptr(),
+                                       _stream_ptr(stream), reps, C.byref(cfg), C.byref(ms)),
+              "tune_block")
+        return cfg.value, ms.value
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _LIB is not None:
             _LIB.iga_op_destroy(self._h)
             self._h = None
+
+
+def block_configs():
+    """[(tile_x, tile_y, rhs_chunk)] of the compiled block-apply configurations."""
+    L = _lib()
+    out = []
+    for i in range(L.iga_block_config_count()):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(L.iga_block_config_desc(i, C.byref(a), C.byref(b), C.byref(c)), "block_config_desc")
+        out.append((a.value, b.value, c.value))
+    return out
 
 
 # --------------------------------------------------------------------------- FEM-PC
@@ -392,7 +444,10 @@ class Solver:
     def __init__(self, op, fem=None, **kw):
         o = SolverOpts()
         _lib().iga_solver_opts_default(C.byref(o))
+        names = {f[0] for f in SolverOpts._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise TypeError(f"unknown solver option {k!r}")
             setattr(o, k, v)
         h = VP()
         check(_lib().iga_solver_create(op._h, None if fem is None else fem._h, C.byref(o),
@@ -401,16 +456,38 @@ class Solver:
         self.op = op
         self.opts = o
 
-    def solve(self, f, u=None, stream=None):
+    def solve(self, f, u=None, stream=None, check_status=True):
+        """Solve A u = f; returns (u, stats).  A solve that does not converge
+        raises IgaError unless check_status is False (stats.status then
+        carries the code)."""
         import torch
         if u is None:
             u = torch.zeros_like(f)
         st = SolveStats()
         rc = _lib().iga_solver_solve(self._h, f.data_ptr(), u.data_ptr(), _stream_ptr(stream),
                                      C.byref(st))
-        if rc not in (0,):
+        if rc != 0 and (check_status or rc not in (IGA_ERR_NOT_CONVERGED, IGA_ERR_DIVERGED)):
             raise IgaError(rc, "solver_solve")
         return u, st
+
+    def solve_block(self, F, U=None, stream=None, check_status=True):
+        """Solve A U[j] = F[j] for the k rows of F (FP64 [k, nfunc], rows
+        contiguous; a row stride above nfunc is the leading dimension).
+        Returns (U, [stats per row])."""
+        import torch
+        if F.dtype != torch.float64 or F.dim() != 2 or F.stride(1) != 1:
+            raise ValueError("F must be a 2-D float64 CUDA tensor with contiguous rows")
+        k = F.shape[0]
+        if U is None:
+            U = torch.zeros_like(F)
+        if U.shape != F.shape or U.stride() != F.stride():
+            raise ValueError("U must match F in shape and strides")
+        st = (SolveStats * k)()
+        rc = _lib().iga_solver_solve_block(self._h, k, F.stride(0), F.data_ptr(), U.data_ptr(),
+                                           _stream_ptr(stream), st)
+        if rc != 0 and (check_status or rc not in (IGA_ERR_NOT_CONVERGED, IGA_ERR_DIVERGED)):
+            raise IgaError(rc, "solver_solve_block")
+        return U, list(st)
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _LIB is not None:

@@ -39,7 +39,15 @@ CASES = [
     ("annulus_p5_n12", dict(kind=igamf.GEOM_QUARTER_ANNULUS, p=5, nex=12)),
     ("square_p1_n8", dict(kind=igamf.GEOM_UNIT_SQUARE, p=1, nex=8)),
     ("square_p6_n9", dict(kind=igamf.GEOM_UNIT_SQUARE, p=6, nex=9)),
+    # per-function NURBS weights (not a tensor product): the general rational kernels
+    ("rational_p2_n20", dict(kind=igamf.GEOM_PERTURBED_SQUARE, p=2, nex=20, perturb=0.1,
+                             w_lo=0.8, w_hi=1.2, seed=3)),
+    ("rational_p3_ragged", dict(kind=igamf.GEOM_PERTURBED_SQUARE, p=3, nex=37, ney=21,
+                                perturb=0.1, w_lo=0.8, w_hi=1.2, seed=4)),
+    ("rational_p5_n12", dict(kind=igamf.GEOM_PERTURBED_SQUARE, p=5, nex=12, perturb=0.1,
+                             w_lo=0.8, w_hi=1.2, seed=5)),
 ]
+RATIONAL = [c for c in CASES if c[0].startswith("rational")]
 
 
 @pytest.fixture(scope="module")
@@ -91,6 +99,34 @@ def test_pipelined_variant_and_tiles(tile):
             assert rel(y32, y_ref) < TOL32, (tile, variant, asm, rel(y32, y_ref))
 
 
+def test_general_rational_mode_and_tiles():
+    """Non-tensor weights select the general rational kernels (5 geometry
+    components per point); both of their tiles agree with the oracle; the
+    persistent variant is not offered for them."""
+    m = igamf.Mesh.build(**RATIONAL[1][1])
+    om = oracle_mesh(m)
+    x = igamf.rng_normal(6, om.nfunc)
+    y_ref = O.apply_mf(om, x)
+    xd = torch.from_numpy(x).cuda()
+    for tile in ((16, 16), (32, 16)):
+        op = igamf.Operator(m, tile_f64=tile, tile_f32=tile)
+        assert op.info.rational == 2 and op.info.variant_f32 == 0
+        for kern in (igamf.SF, igamf.EBE):
+            y64 = op.apply(xd, kernel=kern).cpu().numpy()
+            assert rel(y64, y_ref) < TOL64, (tile, kern, rel(y64, y_ref))
+            y32 = op.apply(xd.float(), kernel=kern).double().cpu().numpy()
+            assert rel(y32, y_ref) < TOL32, (tile, kern, rel(y32, y_ref))
+    for bad in (dict(variant_f32=1), dict(tile_f64=(64, 8))):
+        with pytest.raises(igamf.IgaError) as e:
+            igamf.Operator(m, **bad)
+        assert e.value.status == 6
+    # the multi-RHS path has no general rational kernel
+    op = igamf.Operator(m)
+    with pytest.raises(igamf.IgaError) as e:
+        op.apply_block(torch.zeros(2, op.nfunc, device="cuda"))
+    assert e.value.status == 6
+
+
 def test_gather_bitwise_reproducible(built):
     m, om, op = built("annulus_p3_n24", CASES[3][1])
     x = torch.from_numpy(igamf.rng_normal(5, op.nfunc)).cuda()
@@ -121,7 +157,8 @@ def test_golden_reference_vectors(built):
     assert rel(F, g["F_ref"]) < TOL64
 
 
-@pytest.mark.parametrize("name,kw", CASES[:5], ids=[c[0] for c in CASES[:5]])
+@pytest.mark.parametrize("name,kw", CASES[:5] + RATIONAL,
+                         ids=[c[0] for c in CASES[:5] + RATIONAL])
 def test_load_matches_oracle(built, name, kw):
     m, om, op = built(name, kw)
     for kind, val in ((igamf.LOAD_REF_POLY, 0.0), (igamf.LOAD_CONST, 1.0),

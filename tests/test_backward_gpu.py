@@ -75,6 +75,29 @@ def test_kappa_apply_and_backward_match_oracle(case):
     assert torch.equal(op2.apply(xd), op.apply(xd))
 
 
+def test_kappa_backward_general_rational():
+    """kappa, grad_x and grad_kappa on a mesh with per-function weights."""
+    m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, 3, 14, perturb=0.1, w_lo=0.8, w_hi=1.2,
+                         seed=8)
+    kappa = kappa_field(m, 2)
+    op = igamf.Operator(m, kappa=kappa)
+    assert op.info.rational == 2
+    om = _om(m, kappa)
+    x = igamf.rng_normal(13, op.nfunc)
+    g = igamf.rng_normal(14, op.nfunc)
+    xd, gd = torch.from_numpy(x).cuda(), torch.from_numpy(g).cuda()
+    assert rel(op.apply(xd).cpu().numpy(), O.apply_mf(om, x)) < 1e-12
+    gx, gk = op.backward(xd, gd)
+    assert rel(gk.cpu().numpy(), O.kappa_grad(om, x, g)) < 1e-12
+    Ke = O.element_matrices(om)
+    ien = om.IEN.reshape(om.nelem, om.nloc)
+    gm = g * (om.ID >= 0)
+    ref = np.zeros(om.nfunc)
+    for e in range(om.nelem):
+        ref[ien[e]] += Ke[e] @ gm[ien[e]]
+    assert rel(gx.cpu().numpy(), ref) < 1e-12
+
+
 def test_autograd_apply_gradcheck(case):
     m, kappa = case
     op = igamf.Operator(m)

@@ -241,6 +241,52 @@ def test_apply_invariants(kind):
     assert rel(yc, O.apply_mf(m, x)) < 1e-14
 
 
+def _general_weights_mesh(p, n, seed, scale=1.0):
+    """Unit square with interior control points moved and per-function
+    weights U[0.8, 1.2] that are not a tensor product."""
+    m = O.mesh_unit_square(p, n)
+    cp = m.CP.reshape(-1, 2).copy()
+    interior = m.ID >= 0
+    cp[interior] += (O.rng_uniform(seed, 2 * m.nfunc, -0.1, 0.1).reshape(-1, 2)[interior] / n)
+    W = O.rng_uniform(seed + 1, m.nfunc, 0.8, 1.2) * scale
+    return O.Mesh(m.p, m.q, m.nex, m.ney, cp, W, m.IEN, m.ID, m.Dir, m.C1, m.C2, m.h1, m.h2)
+
+
+@pytest.mark.parametrize("p,n", [(2, 9), (3, 7)])
+def test_general_weights_invariants(p, n):
+    """Per-function NURBS weights: partition of unity (A 1 = 0 on interior
+    rows), symmetry, reproduction of the physical coordinates (which are in
+    the rational span), invariance under a global weight scale, and the
+    assembled matrix equal to the apply.  A dropped grad-W term in the
+    quotient rule breaks the first and third."""
+    m = _general_weights_mesh(p, n, 5)
+    w = m.W.reshape(n + p, n + p)
+    assert abs(w[0, 0] * w[1, 1] - w[0, 1] * w[1, 0]) > 1e-3  # not a tensor product
+    cp = m.CP.reshape(-1, 2)
+    interior = m.ID >= 0
+    u = O.rng_normal(1, m.nfunc) * interior
+    v = O.rng_normal(2, m.nfunc) * interior
+    Au, Av = O.apply_mf(m, u), O.apply_mf(m, v)
+    s = np.abs(Au).max()
+    assert np.abs(O.apply_mf(m, np.ones(m.nfunc))).max() < 1e-12 * s
+    assert abs(v @ Au - u @ Av) < 1e-12 * abs(v @ Au)
+    # coordinates: exact in the continuum; only the quadrature error of the
+    # rational integrand remains (about 2e-5 here; a wrong Jacobian is O(1))
+    err = [np.abs(O.apply_mf(m, cp[:, c].copy())).max() / s for c in (0, 1)]
+    assert max(err) < 1e-3, err
+    m2 = _general_weights_mesh(p, n, 5, scale=2.5)
+    assert rel(O.apply_mf(m2, u), Au) < 1e-13
+    x = O.rng_normal(3, m.nfunc)
+    assert rel(O.assembled_matrix(m) @ x, O.apply_mf(m, x)) < 1e-13
+    # the load vector integrates R_A: sum over all functions of the load of
+    # f = 1 is the area (partition of unity)
+    free = O.Mesh(m.p, m.q, m.nex, m.ney, m.CP, m.W, m.IEN, np.arange(m.nfunc),
+                  np.zeros(0, np.int32), m.C1, m.C2, m.h1, m.h2)
+    area = O.load(free, O.LOAD_CONST, 1.0)
+    jac_area = sum(O.element_qp(m, ex, ey)[3].sum() for ex in range(n) for ey in range(n))
+    assert abs(area.sum() - jac_area) < 1e-12 * jac_area
+
+
 def test_annulus_geometry_exact():
     m = O.mesh_quarter_annulus(3, 6, 10, 1.0, 2.0)
     area = 0.0

@@ -81,10 +81,14 @@ def _oracle_mesh(m):
                   a["Dir"], a["C1"], a["C2"], a["h1"], a["h2"])
 
 
-@pytest.mark.parametrize("geom", ["square", "annulus"])
+@pytest.mark.parametrize("geom", ["square", "annulus", "rational"])
 def test_pcg_solves(geom):
     if geom == "square":
         m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 2, 32)
+        kind, val = igamf.LOAD_SINSIN, 2 * np.pi ** 2
+    elif geom == "rational":
+        m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, 3, 20, perturb=0.1, w_lo=0.8,
+                             w_hi=1.2, seed=9)
         kind, val = igamf.LOAD_SINSIN, 2 * np.pi ** 2
     else:
         m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 24)
@@ -160,3 +164,35 @@ def test_iteration_limit_and_bad_options():
     with pytest.raises(igamf.IgaError) as e:
         igamf.Solver(op, None, pc=igamf.PC_FEM_CG)  # FEM preconditioner without a matrix
     assert e.value.status == 1
+
+
+def test_block_solve_matches_single_solves():
+    """k systems in one block solve (FP32 PCG + FP64 reliable updates per
+    column on the multi-RHS apply) against single FP64 solves; a zero column
+    converges at once; a leading dimension above nfunc; unsupported kinds."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 41, 29)
+    op = igamf.Operator(m)
+    n, k = op.nfunc, 6
+    mask = torch.from_numpy(m.arrays()["ID"] >= 0).cuda()
+    ld = n + 5
+    buf = torch.zeros(k, ld, dtype=torch.float64, device="cuda")
+    for j in range(k - 1):
+        buf[j, :n] = torch.from_numpy(igamf.rng_normal(40 + j, n)).cuda() * mask
+    F = buf[:, :n]  # last column stays zero
+    s = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F32_REFINE, pc=igamf.PC_JACOBI, rtol=1e-9)
+    U, stats = s.solve_block(F, torch.full((k, ld), 5.0, dtype=torch.float64,
+                                           device="cuda")[:, :n])
+    s64 = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F64, pc=igamf.PC_JACOBI, rtol=1e-11)
+    for j in range(k - 1):
+        assert stats[j].status == 0 and stats[j].rel_residual <= 1e-9 * 1.0001, (j, stats[j])
+        assert stats[j].outer_iterations >= 1
+        u, _ = s64.solve(F[j].contiguous())
+        assert (U[j] - u).abs().max().item() < 1e-7 * u.abs().max().item(), j
+    assert stats[k - 1].status == 0 and stats[k - 1].iterations == 0
+    assert U[k - 1].abs().max().item() == 0.0
+    # repeated block solve (graph replay) gives the same result
+    U2, _ = s.solve_block(F)
+    assert torch.equal(U, U2)
+    with pytest.raises(igamf.IgaError) as e:
+        s64.solve_block(F)
+    assert e.value.status == 6
