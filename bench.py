@@ -128,10 +128,14 @@ def headline(a, out):
     achieved = alg_bytes32 / (k_avg * 1e-3) / 1e9
     gdofs = n_int / (med32 * 1e-3) / 1e9
     variant = info.variant_f32
-    kname = (f"k_apply_pf<float, {p}, {p + 1}, {info.tile_x_f32}, {info.tile_y_f32}>"
-             if variant == 1 and a.kernel == "sf" else
-             f"k_apply<float, {p}, {p + 1}, {info.tile_x_f32}, {info.tile_y_f32}, "
-             f"{'true' if a.kernel == 'ebe' else 'false'}, false>")
+    tx, ty = info.tile_x_f32, info.tile_y_f32
+    if variant == 1 and a.kernel == "sf":
+        kname = f"k_apply_pf<float, {p}, {p + 1}, {tx}, {ty}>"
+    else:
+        minw = 4 if (variant == 2 and a.kernel == "sf") else 1
+        split = 2 if (variant == 3 and a.kernel == "sf") else 1
+        ebe = "true" if a.kernel == "ebe" else "false"
+        kname = f"k_apply<float, {p}, {p + 1}, {tx}, {ty}, {ebe}, false, {minw}, {split}>"
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:

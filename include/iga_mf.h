@@ -136,13 +136,23 @@ typedef struct {
   int want_f64;     /* keep FP64 G_q (default 1) */
   int want_f32;     /* keep FP32 G_q (default 1) */
   int tile_x_f64, tile_y_f64; /* element tile of the FP64 kernels, 0 = default
-                                 (16x16); 32x16 for every degree, 32x32 and
-                                 64x8 for p <= 4; general rational meshes
-                                 16x16, and 32x16 for p <= 4 */
-  int tile_x_f32, tile_y_f32; /* element tile of the FP32 kernels, as above */
+                                 (16x16; 16x8 with variant 3); 16x16 and 32x16
+                                 for every degree, 32x32 and 64x8 for p <= 4;
+                                 general rational meshes 16x16, and 32x16 for
+                                 p <= 4 */
+  int tile_x_f32, tile_y_f32; /* element tile of the FP32 kernels, same
+                                 choices and default */
   int variant_f64, variant_f32; /* 0: one workgroup per tile; 1: persistent
                                    workgroups prefetching the next tile (SF);
-                                   -1 (default): 1 for p <= 4, else 0 */
+                                   2: one workgroup per tile with a register
+                                   budget of four waves per SIMD (FP32 SF,
+                                   16x16, p <= 4; EBE applies use 0);
+                                   3: two threads per element (SF) on 16x8
+                                   tiles, for high degrees and FP64; general
+                                   rational meshes accept 0 and 3;
+                                   -1 (default): 3 for FP32 at p >= 5 and for
+                                   FP64 at p = 3 (not general rational),
+                                   else 0 */
 } iga_op_opts;
 
 typedef struct {
@@ -183,6 +193,11 @@ int iga_op_profile_read(iga_op *op, int *count, double *total_ms, double *min_ms
  * 2 f = value*sin(pi x)sin(pi y). */
 int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream);
 
+/* Physical coordinates of every quadrature point, xq[e][q][2] (FP64,
+ * device, E * nqp * 2 values), elements and points in the reference's order
+ * (the x_q of matrix_free/src/ElementMF.cpp:106-207): for building a
+ * per-point kappa field on the device. */
+int iga_op_qp_coords(iga_op *op, double *xq, void *stream);
 /* ---- per-qp conductivity and its gradient (extension; kappa = 1 is the
  * reference).  kappa: [E][nqp] FP64 device; G_q is rebuilt on the device. */
 int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream);

@@ -155,8 +155,8 @@ int dev_alloc(iga_op *op, void **p, size_t bytes) {
   return IGA_OK;
 }
 
-int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
-              int mask_in, int mask_out, hipStream_t s, const int *skip) {
+int apply_raw_impl(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
+                   int mask_in, int mask_out, hipStream_t s, const int *skip, bool seam) {
   const igak::Layout &L = f64 ? op->L64 : op->L32;
   igak::ApplyArgs a;
   a.x = x;
@@ -198,7 +198,7 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
     IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used + 1], s));
     ++op->prof_used;
   }
-  if (gather) {
+  if (gather && seam) {
     if (f64)
       IGA_HIP_TRY(op->ops->seam64((double *)y, op->halo64, op->dirmask, op->nex, op->ney,
                                   op->nfx, op->nfy, L, 1, mask_out, skip, s));
@@ -206,6 +206,27 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
       IGA_HIP_TRY(op->ops->seam32((float *)y, op->halo32, op->dirmask, op->nex, op->ney,
                                   op->nfx, op->nfy, L, 1, mask_out, skip, s));
   }
+  return IGA_OK;
+}
+
+int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
+              int mask_in, int mask_out, hipStream_t s, const int *skip) {
+  return apply_raw_impl(op, f64, ebe, gather, x, y, mask_in, mask_out, s, skip, true);
+}
+
+int apply_gather_dot(iga_op *op, bool f64, const void *x, void *y, const void *pv,
+                     double *part, int grid, hipStream_t s, const int *skip) {
+  int rc = apply_raw_impl(op, f64, false, true, x, y, 0, 1, s, skip, false);
+  if (rc) return rc;
+  const igak::Layout &L = f64 ? op->L64 : op->L32;
+  if (f64)
+    IGA_HIP_TRY(op->ops->seam_dot64((double *)y, op->halo64, (const double *)pv, op->dirmask,
+                                    op->nex, op->ney, op->nfx, op->nfy, L, 1, part, grid, skip,
+                                    s));
+  else
+    IGA_HIP_TRY(op->ops->seam_dot32((float *)y, op->halo32, (const float *)pv, op->dirmask,
+                                    op->nex, op->ney, op->nfx, op->nfy, L, 1, part, grid, skip,
+                                    s));
   return IGA_OK;
 }
 
@@ -365,9 +386,16 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   op->ops = ops;
   op->want64 = o.want_f64 != 0;
   op->want32 = o.want_f32 != 0;
-  op->variant64 = o.variant_f64 < 0 ? (p <= 4 ? 1 : 0) : o.variant_f64;
-  op->variant32 = o.variant_f32 < 0 ? (p <= 4 ? 1 : 0) : o.variant_f32;
-  if (o.variant_f64 < -1 || op->variant64 > 1 || o.variant_f32 < -1 || op->variant32 > 1) {
+  // defaults measured on MI355X (quarter annulus 1024^2 / 512^2, p = 2..5):
+  // one workgroup per 16x16 tile, except two threads per element where the
+  // per-thread state is large -- FP32 at p >= 5 and FP64 at p = 3
+  auto dflt_variant = [&](bool f64) {
+    if (f64) return (p == 3 && rational != 2) ? 3 : 0;
+    return p >= 5 ? 3 : 0;
+  };
+  op->variant64 = o.variant_f64 < 0 ? dflt_variant(true) : o.variant_f64;
+  op->variant32 = o.variant_f32 < 0 ? dflt_variant(false) : o.variant_f32;
+  if (o.variant_f64 < -1 || op->variant64 > 3 || o.variant_f32 < -1 || op->variant32 > 3) {
     delete op;
     return IGA_ERR_INVALID_ARGUMENT;
   }
@@ -379,27 +407,32 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   if (rc) return fail(rc);
 
   // tiles
-  auto pick = [&](int tx, int ty, int dflt, int &id, igak::Layout &L) {
+  // the split-element variant (3) runs on 16x8 tiles, and only it does
+  auto pick = [&](int tx, int ty, int dflt, int variant, int &id, igak::Layout &L) {
+    if (variant == 3) dflt = igak::TILE_16x8;
     id = (tx || ty) ? igak::tile_id(tx, ty) : dflt;
     if (id < 0 || !igak::tile_compiled(p, id)) return false;
+    if ((id == igak::TILE_16x8) != (variant == 3)) return false;
     igak::tile_shape(id, L.tx, L.ty);
     if (L.tx < p + 1 || L.ty < p + 1) return false;
     L.ntx = ceil_div(nex, L.tx);
     L.nty = ceil_div(ney, L.ty);
     return true;
   };
-  if (!pick(o.tile_x_f64, o.tile_y_f64, igak::TILE_16x16, op->tile64, op->L64) ||
-      !pick(o.tile_x_f32, o.tile_y_f32, igak::TILE_16x16, op->tile32, op->L32))
+  // default tile 16x16 (16x8 for the split-element variant)
+  if (!pick(o.tile_x_f64, o.tile_y_f64, igak::TILE_16x16, op->variant64, op->tile64, op->L64) ||
+      !pick(o.tile_x_f32, o.tile_y_f32, igak::TILE_16x16, op->variant32, op->tile32, op->L32))
     return fail(IGA_ERR_UNSUPPORTED);
   if (rational == 2) {
-    // general rational kernels: one workgroup per tile, few tile shapes
+    // general rational kernels: variants 0 and 3, few tile shapes
     for (int id : {op->tile64, op->tile32})
       if (!igak::rat_compiled(p, id)) return fail(IGA_ERR_UNSUPPORTED);
-    if (o.variant_f64 == 1 || o.variant_f32 == 1) return fail(IGA_ERR_UNSUPPORTED);
-    op->variant64 = op->variant32 = 0;
+    if (o.variant_f64 == 1 || o.variant_f64 == 2 || o.variant_f32 == 1 || o.variant_f32 == 2)
+      return fail(IGA_ERR_UNSUPPORTED);
   }
   if ((op->variant64 == 1 && !igak::pf_compiled(p, op->tile64)) ||
-      (op->variant32 == 1 && !igak::pf_compiled(p, op->tile32)))
+      (op->variant32 == 1 && !igak::pf_compiled(p, op->tile32)) || op->variant64 == 2 ||
+      (op->variant32 == 2 && !igak::w4_compiled(p, op->tile32, false)))
     return fail(IGA_ERR_UNSUPPORTED);
 
   // tables and Gauss rule (host FP64)
@@ -697,6 +730,15 @@ int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream) {
                             op->dirmask,
                             op->nex, op->ney, op->nfx, op->nfy, kind, value,
                             (hipStream_t)stream));
+  return IGA_OK;
+}
+
+int iga_op_qp_coords(iga_op *op, double *xq, void *stream) {
+  if (!op || !xq) return IGA_ERR_INVALID_ARGUMENT;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  IGA_HIP_TRY(op->ops->qp_coords(xq, op->tabx64, op->taby64, op->cp, op->wts64, op->nex,
+                                 op->ney, op->nfx, (hipStream_t)stream));
   return IGA_OK;
 }
 

@@ -83,6 +83,17 @@ __device__ __forceinline__ void atomic_add(T *p, T v) {
   atomicAdd(p, v);
 }
 
+// the same with power-of-two tiles of size 1 << sh
+template <int P>
+__device__ __forceinline__ void touching_tiles_sh(int g, int nel, int sh, int &lo, int &hi) {
+  int e0 = g - P;
+  if (e0 < 0) e0 = 0;
+  int e1 = g;
+  if (e1 > nel - 1) e1 = nel - 1;
+  lo = e0 >> sh;
+  hi = e1 >> sh;
+}
+
 // --------------------------------------------------------------------------
 // G_q storage: per tile and component c (G11, G12, G22; on general rational
 // meshes also a_xi, a_eta, so NC = 5) a block of NQ2*NT values; the first
@@ -138,20 +149,26 @@ __device__ __forceinline__ void load_gq(const T *G, int tile, int t, int NT, T (
 // Sum-factorised action ye = -sum_q gradR^T G_q gradR X.  The eta loop fuses
 // the forward eta contraction, the quadrature-point operator and the backward
 // eta contraction, so only one table row is live at a time.
-template <typename T, int P, int NQ, int LX, int LY, bool RAT = false, typename GL>
+// NI quadrature columns i0 .. i0 + NI - 1 (all of them by default): with
+// NI < NQ two threads share an element, each contracting its half of the
+// columns; the element vector is then the sum of the two threads' ye.
+template <typename T, int P, int NQ, int LX, int LY, bool RAT = false, int NI = NQ,
+          typename GL>
 __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
                                            const T (*tbx)[LX], const T (*tby)[LY],
                                            int lx, int ly, const GL &gl,
-                                           T (&ye)[P + 1][P + 1]) {
+                                           T (&ye)[P + 1][P + 1], int i0 = 0) {
   constexpr int NB = P + 1, TAB = NQ * NB;
-  T t0[NB][NQ], t1[NB][NQ];
+  T t0[NB][NI], t1[NB][NI];
 #pragma unroll
-  for (int i = 0; i < NQ; ++i) {
+  for (int il = 0; il < NI; ++il) {
+    const int i = i0 + il;
+    const bool in = NI == NQ || i < NQ;
     T n1[NB], d1[NB];
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
-      n1[a] = tbx[i * NB + a][lx];
-      d1[a] = tbx[TAB + i * NB + a][lx];
+      n1[a] = in ? tbx[i * NB + a][lx] : T(0);
+      d1[a] = in ? tbx[TAB + i * NB + a][lx] : T(0);
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -161,17 +178,17 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
         s0 += n1[a] * X[b][a];
         s1 += d1[a] * X[b][a];
       }
-      t0[b][i] = s0;
-      t1[b][i] = s1;
+      t0[b][il] = s0;
+      t1[b][il] = s1;
     }
   }
-  T s0[NB][NQ], s1[NB][NQ];
+  T s0[NB][NI], s1[NB][NI];
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-      s0[b][i] = T(0);
-      s1[b][i] = T(0);
+    for (int il = 0; il < NI; ++il) {
+      s0[b][il] = T(0);
+      s1[b][il] = T(0);
     }
 #pragma unroll
   for (int j = 0; j < NQ; ++j) {
@@ -182,13 +199,15 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
       d2[b] = tby[TAB + j * NB + b][ly];
     }
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
+    for (int il = 0; il < NI; ++il) {
+      const int i = i0 + il;
+      if (NI != NQ && i >= NQ) continue;
       T ux = T(0), uy = T(0), uv = T(0);
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        ux += n2[b] * t1[b][i];
-        uy += d2[b] * t0[b][i];
-        if (RAT) uv += n2[b] * t0[b][i];
+        ux += n2[b] * t1[b][il];
+        uy += d2[b] * t0[b][il];
+        if (RAT) uv += n2[b] * t0[b][il];
       }
       const int q = j * NQ + i;
       T ax = T(0), ay = T(0);
@@ -205,8 +224,8 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
       const T fv = RAT ? -(ax * fx + ay * fy) : T(0);
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        s0[b][i] += n2[b] * fx;
-        s1[b][i] += RAT ? d2[b] * fy + n2[b] * fv : d2[b] * fy;
+        s0[b][il] += n2[b] * fx;
+        s1[b][il] += RAT ? d2[b] * fy + n2[b] * fv : d2[b] * fy;
       }
     }
   }
@@ -215,22 +234,21 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
 #pragma unroll
     for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
 #pragma unroll
-  for (int i = 0; i < NQ; ++i) {
+  for (int il = 0; il < NI; ++il) {
+    const int i = i0 + il;
+    const bool in = NI == NQ || i < NQ;
     T n1[NB], d1[NB];
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
-      n1[a] = tbx[i * NB + a][lx];
-      d1[a] = tbx[TAB + i * NB + a][lx];
+      n1[a] = in ? tbx[i * NB + a][lx] : T(0);
+      d1[a] = in ? tbx[TAB + i * NB + a][lx] : T(0);
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int a = 0; a < NB; ++a) ye[b][a] += d1[a] * s0[b][i] + n1[a] * s1[b][i];
+      for (int a = 0; a < NB; ++a) ye[b][a] += d1[a] * s0[b][il] + n1[a] * s1[b][il];
   }
 }
-
-// Element-by-element action: per quadrature point the full reference
-// gradient of every local function (LocalAssemblyMF.cpp:31-67 shape).
 template <typename T, int P, int NQ, int LX, int LY, bool RAT = false, typename GL>
 __device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
                                             const T (*tbx)[LX], const T (*tby)[LY],
@@ -296,9 +314,9 @@ __device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
 // counter completes in issue order, so the LDS stores wait for the x and
 // table loads alone while the G_q loads stay in flight.
 // --------------------------------------------------------------------------
-template <typename T, int P, int NQ, int TX, int TY, bool RAT>
+template <typename T, int P, int NQ, int TX, int TY, bool RAT, int NTH = TX * TY>
 struct TileStage {
-  static constexpr int NB = P + 1, NT = TX * TY, FX = TX + P, FY = TY + P, NFP = FX * FY;
+  static constexpr int NB = P + 1, NT = NTH, FX = TX + P, FY = TY + P, NFP = FX * FY;
   static constexpr int TAB = NQ * NB, NOWN = (NFP + NT - 1) / NT;
   static constexpr int NTBX = (2 * TAB * TX + NT - 1) / NT, NTBY = (2 * TAB * TY + NT - 1) / NT;
   T xv[NOWN];
@@ -487,18 +505,26 @@ __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (
 // --------------------------------------------------------------------------
 // the apply kernel
 // --------------------------------------------------------------------------
-template <typename T, int P, int NQ, int TX, int TY, bool EBE, bool RAT = false>
-__global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
-  constexpr int NB = P + 1, NT = TX * TY, FX = TX + P, FY = TY + P;
+// MINW: waves per SIMD the register budget must allow (1 = compiler's
+// choice; 4 caps a 256-thread tile at 128 VGPRs and streams G_q on demand).
+// SPLIT = 2: two threads per element (sum-factorised only), each
+// contracting half of the quadrature columns; halves the per-thread state
+// of high degrees and FP64.
+template <typename T, int P, int NQ, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1,
+          int SPLIT = 1>
+__global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
+  static_assert(SPLIT == 1 || !EBE, "split elements are sum-factorised");
+  constexpr int NB = P + 1, NE = TX * TY, NT = NE * SPLIT, FX = TX + P, FY = TY + P;
   constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY;
+  constexpr bool ROW16 = TX == 16 && SPLIT == 1;
   __shared__ T xs[NFP];
   __shared__ T tbx[2 * TAB][TX];
   __shared__ T tby[2 * TAB][TY];
-  __shared__ T bufm[(TX == 16 ? 2 : 1) * NB * NT];
-  T(*buf)[NT] = reinterpret_cast<T(*)[NT]>(bufm);
+  __shared__ T bufm[(ROW16 ? 2 : 1) * NB * NE];
+  T(*buf)[NE] = reinterpret_cast<T(*)[NE]>(bufm);
 
   if (A.skip && *A.skip) return;
-  const int t = threadIdx.x, lx = t % TX, ly = t / TX;
+  const int t = threadIdx.x, h = t % SPLIT, el = t / SPLIT, lx = el % TX, ly = el / TX;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % A.ntx, ty = tile / A.ntx;
   const int ex0 = tx * TX, ey0 = ty * TY;
@@ -506,18 +532,19 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
 
   // x footprint and tables first, then the geometry factors of this element
   // (held in registers unless they are many: high degree, general rational)
-  TileStage<T, P, NQ, TX, TY, RAT> stg;
+  TileStage<T, P, NQ, TX, TY, RAT, NT> stg;
   stg.load(A, ex0, ey0, t);
   constexpr int NC = RAT ? 5 : 3;
-  constexpr bool PREF = IGAK_PREFETCH_G && NC * NQ2 * sizeof(T) <= 320;
+  constexpr bool PREF =
+      IGAK_PREFETCH_G && MINW < 4 && SPLIT == 1 && NC * NQ2 * sizeof(T) <= 320;
   T g[PREF ? NC : 1][PREF ? NQ2 : 1];
   const T *Gt = static_cast<const T *>(A.G);
-  if constexpr (PREF) load_gq<T, NQ2, NC>(Gt, tile, t, NT, g);
+  if constexpr (PREF) load_gq<T, NQ2, NC>(Gt, tile, el, NE, g);
   auto gl = [&](int c, int q) {
     if constexpr (PREF)
       return g[c][q];
     else
-      return Gt[gq_index<T, NQ2, NC>(tile, c, q, t, NT)];
+      return Gt[gq_index<T, NQ2, NC>(tile, c, q, el, NE)];
   };
   const T *wf = static_cast<const T *>(A.wf);
   stg.store(A, ex0, ey0, t, xs, tbx, tby);
@@ -530,10 +557,18 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
+    constexpr int NI = (NQ + SPLIT - 1) / SPLIT;
     if (EBE)
       element_ebe<T, P, NQ, TX, TY, RAT>(X, tbx, tby, lx, ly, gl, ye);
     else
-      element_sf<T, P, NQ, TX, TY, RAT>(X, tbx, tby, lx, ly, gl, ye);
+      element_sf<T, P, NQ, TX, TY, RAT, NI>(X, tbx, tby, lx, ly, gl, ye, h * NI);
+    if constexpr (SPLIT == 2) {
+      // the pair (lanes 2e, 2e+1) adds its halves: quad_perm [1,0,3,2]
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NB; ++a) ye[b][a] += dpp_row<0xB1>(ye[b][a]);
+    }
     if (!valid) {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
@@ -544,7 +579,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
 
   T *y = static_cast<T *>(A.y);
   T *halo = static_cast<T *>(A.halo);
-  if constexpr (TX == 16) {
+  if constexpr (ROW16) {
     reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, tile, ex0, ey0, y, halo, wf);
     return;
   }
@@ -556,8 +591,10 @@ __global__ __launch_bounds__(TX *TY) void k_apply(ApplyArgs A) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     if (b > 0) __syncthreads();
+    if (h == 0) {
 #pragma unroll
-    for (int a = 0; a < NB; ++a) buf[a][t] = ye[b][a];
+      for (int a = 0; a < NB; ++a) buf[a][el] = ye[b][a];
+    }
     __syncthreads();
 #pragma unroll
     for (int o = 0; o < NOWN; ++o) {
@@ -688,31 +725,31 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
 
 // Seam functions (touched by more than one tile).  mode 0: zero them before
 // a scatter-add apply; mode 1: sum the halo slots of the touching tiles in
-// (tile row, tile column) order.
+// (tile row, tile column) order.  Grid: x along a seam line, y over the
+// lines -- the P columns right of each vertical tile boundary (all rows),
+// then the P rows below each horizontal one (functions not already on a
+// vertical seam).  Tile sizes are powers of two (shifts sx, sy).
 template <typename T, int P>
-__global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex,
-                       int ney, int nfx, int nfy, int TX, int TY, int ntx,
-                       int nty, int mode, int mask_out, const int *skip) {
+__global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex, int ney, int nfx,
+                       int nfy, int sx, int sy, int ntx, int nty, int mode, int mask_out,
+                       const int *skip) {
   if (skip && *skip) return;
-  const int64_t nv = (int64_t)(ntx - 1) * P * nfy;
-  const int64_t nh = (int64_t)(nty - 1) * P * nfx;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv + nh) return;
+  const int TX = 1 << sx, TY = 1 << sy;
+  const int line = blockIdx.y, pos = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nvl = (ntx - 1) * P;
   int gx, gy;
-  if (i < nv) {
-    const int band = (int)(i / ((int64_t)P * nfy));
-    const int r = (int)(i % ((int64_t)P * nfy));
-    gx = (band + 1) * TX + r % P;
-    gy = r / P;
+  if (line < nvl) {
+    gx = (line / P + 1) * TX + line % P;
+    gy = pos;
+    if (gy >= nfy) return;
   } else {
-    const int64_t j = i - nv;
-    const int band = (int)(j / ((int64_t)P * nfx));
-    const int r = (int)(j % ((int64_t)P * nfx));
-    gy = (band + 1) * TY + r / nfx;
-    gx = r % nfx;
+    const int l = line - nvl;
+    gy = (l / P + 1) * TY + l % P;
+    gx = pos;
+    if (gx >= nfx) return;
     int xlo, xhi;
-    touching_tiles<P>(gx, nex, TX, xlo, xhi);
-    if (xlo != xhi) return;  // owned by the vertical-band pass
+    touching_tiles_sh<P>(gx, nex, sx, xlo, xhi);
+    if (xlo != xhi) return;  // owned by the vertical-seam lines
   }
   const int Ag = gy * nfx + gx;
   if (mode == 0 || (mask_out && is_dirichlet(dirmask, Ag, gx, gy, nfx, nfy))) {
@@ -721,8 +758,8 @@ __global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex,
   }
   const int FX = TX + P, FY = TY + P;
   int xlo, xhi, ylo, yhi;
-  touching_tiles<P>(gx, nex, TX, xlo, xhi);
-  touching_tiles<P>(gy, ney, TY, ylo, yhi);
+  touching_tiles_sh<P>(gx, nex, sx, xlo, xhi);
+  touching_tiles_sh<P>(gy, ney, sy, ylo, yhi);
   T acc = T(0);
   for (int ty = ylo; ty <= yhi; ++ty)
     for (int tx = xlo; tx <= xhi; ++tx) {
@@ -730,6 +767,57 @@ __global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex,
       acc += halo[(size_t)(ty * ntx + tx) * FX * FY + k];
     }
   y[Ag] = acc;
+}
+
+// Inverse-LM gather of the seam functions fused with the partial sums of
+// pv . y over all functions (the CG's p.Ap).  Workgroups take function rows
+// (grid-stride over rows, threads along the row: no integer division); tile
+// sizes are powers of two (shifts sx, sy).  Seam functions are summed from
+// the touching tiles' halo slots in (tile row, tile column) order and
+// stored, the others read back.  One partial per workgroup (256 threads,
+// fixed reduction tree) in part[].
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_seam_dot(T *y, const T *halo, const T *pv,
+                                                  const uint8_t *dirmask, int nex, int ney,
+                                                  int nfx, int nfy, int sx, int sy, int ntx,
+                                                  int mask_out, double *part, const int *skip) {
+  if (skip && *skip) return;
+  const int TX = 1 << sx, TY = 1 << sy, FX = TX + P, FY = TY + P;
+  double s = 0.0;
+  for (int gy = blockIdx.x; gy < nfy; gy += gridDim.x) {
+    int ylo, yhi;
+    touching_tiles_sh<P>(gy, ney, sy, ylo, yhi);
+    const size_t row = (size_t)gy * nfx;
+    for (int gx = threadIdx.x; gx < nfx; gx += 256) {
+      int xlo, xhi;
+      touching_tiles_sh<P>(gx, nex, sx, xlo, xhi);
+      const size_t i = row + gx;
+      T v;
+      if (xlo != xhi || ylo != yhi) {
+        if (mask_out && is_dirichlet(dirmask, (int)i, gx, gy, nfx, nfy)) {
+          v = T(0);
+        } else {
+          T acc = T(0);
+          for (int ty = ylo; ty <= yhi; ++ty)
+            for (int tx = xlo; tx <= xhi; ++tx)
+              acc += halo[(size_t)(ty * ntx + tx) * FX * FY + (gy - ty * TY) * FX +
+                          (gx - tx * TX)];
+          v = acc;
+        }
+        y[i] = v;
+      } else {
+        v = y[i];
+      }
+      s += (double)pv[i] * (double)v;
+    }
+  }
+  // workgroup sum: wavefront shuffles, then four wavefront sums in order
+  __shared__ double ws[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 // --------------------------------------------------------------------------
@@ -863,6 +951,26 @@ __global__ void k_load_qp(double *fJ, const double *tabx, const double *taby,
       double v = f * wq[i] * wq[j] * (m[2] * m[5] - m[3] * m[4]);
       if (wts) v /= m[6];
       fJ[(size_t)e * NQ2 + j * NQ + i] = v;
+    }
+}
+
+// physical coordinates of every quadrature point: xq[e][q][2] (FP64)
+template <int P, int NQ>
+__global__ void k_qp_coords(double *xq, const double *tabx, const double *taby,
+                            const double *cp, const double *wts, int nex, int ney, int nfx) {
+  constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nex * ney) return;
+  const int ex = e % nex, ey = e / nex;
+  const double *t1 = tabx + (size_t)ex * 2 * TAB;
+  const double *t2 = taby + (size_t)ey * 2 * TAB;
+  for (int j = 0; j < NQ; ++j)
+    for (int i = 0; i < NQ; ++i) {
+      double m[9];
+      map_q<P, NQ>(cp, wts, nfx, ex, ey, t1, t2, i, j, m);
+      const size_t o = ((size_t)e * NQ2 + j * NQ + i) * 2;
+      xq[o] = m[0];
+      xq[o + 1] = m[1];
     }
 }
 

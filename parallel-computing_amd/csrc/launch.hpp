@@ -8,9 +8,16 @@
 namespace igak {
 
 // element tiles with compiled kernels
-enum TileId { TILE_32x32 = 0, TILE_32x16 = 1, TILE_16x16 = 2, TILE_64x8 = 3, NTILE = 4 };
+enum TileId {
+  TILE_32x32 = 0,
+  TILE_32x16 = 1,
+  TILE_16x16 = 2,
+  TILE_64x8 = 3,
+  TILE_16x8 = 4,  // split-element variant (3) and its EBE companion only
+  NTILE = 5
+};
 inline void tile_shape(int id, int &tx, int &ty) {
-  static const int s[NTILE][2] = {{32, 32}, {32, 16}, {16, 16}, {64, 8}};
+  static const int s[NTILE][2] = {{32, 32}, {32, 16}, {16, 16}, {64, 8}, {16, 8}};
   tx = s[id][0];
   ty = s[id][1];
 }
@@ -25,21 +32,22 @@ inline int tile_id(int tx, int ty) {
 
 // compiled (tile, variant) pairs: high degrees spill at 512+ threads
 inline bool tile_compiled(int p, int id) {
-  return p <= 4 || id == TILE_32x16 || id == TILE_16x16;
+  return p <= 4 || id == TILE_32x16 || id == TILE_16x16 || id == TILE_16x8;
 }
 inline bool pf_compiled(int p, int id) { return p <= 4 || id == TILE_16x16; }
+inline bool w4_compiled(int p, int id, bool f64) { return p <= 4 && id == TILE_16x16 && !f64; }
 inline bool rat_compiled(int p, int id) {
-  return id == TILE_16x16 || (p <= 4 && id == TILE_32x16);
+  return id == TILE_16x16 || id == TILE_16x8 || (p <= 4 && id == TILE_32x16);
 }
 
 template <int P>
 struct Launch {
   static constexpr int NQ = P + 1;
 
-  template <typename T, int TX, int TY, bool EBE, bool RAT = false>
+  template <typename T, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1, int SPLIT = 1>
   static hipError_t apply_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE, RAT>), dim3(ntiles), dim3(TX * TY), 0, s,
-                       a);
+    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE, RAT, MINW, SPLIT>), dim3(ntiles),
+                       dim3(TX * TY * SPLIT), 0, s, a);
     return hipGetLastError();
   }
   template <typename T, bool EBE>
@@ -84,9 +92,22 @@ struct Launch {
     if (tile == TILE_16x16) return apply_pf_t<T, 16, 16>(a, ntiles, s);
     return hipErrorInvalidValue;
   }
-  // variant: 0 one workgroup per tile, 1 persistent pipelined (SF only)
+  // variant: 0 one workgroup per tile, 1 persistent pipelined (SF only),
+  // 2 one workgroup per tile at four waves per SIMD (FP32 SF, 16x16, p <= 4)
+  // 3 two threads per element (SF; 16x8 tiles, which the EBE apply of such
+  // an operator also uses)
   static hipError_t apply(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles,
                           int variant, hipStream_t s) {
+    if (tile == TILE_16x8) {
+      if (f64) return ebe ? apply_t<double, 16, 8, true>(a, ntiles, s)
+                          : apply_t<double, 16, 8, false, false, 1, 2>(a, ntiles, s);
+      return ebe ? apply_t<float, 16, 8, true>(a, ntiles, s)
+                 : apply_t<float, 16, 8, false, false, 1, 2>(a, ntiles, s);
+    }
+    if constexpr (P <= 4)
+      if (variant == 2 && !f64 && !ebe && tile == TILE_16x16)
+        return apply_t<float, 16, 16, false, false, 4>(a, ntiles, s);
+    if (variant == 2) variant = 0;  // EBE applies of a variant-2 operator
     if (variant == 1 && !ebe)
       return f64 ? apply_pf_tile<double>(a, tile, ntiles, s)
                  : apply_pf_tile<float>(a, tile, ntiles, s);
@@ -100,12 +121,23 @@ struct Launch {
   static hipError_t seam(T *y, const T *halo, const uint8_t *mask, int nex, int ney, int nfx,
                          int nfy, const Layout &L, int mode, int mask_out, const int *skip,
                          hipStream_t s) {
-    const int64_t n = (int64_t)(L.ntx - 1) * P * nfy + (int64_t)(L.nty - 1) * P * nfx;
-    if (n == 0) return hipSuccess;
-    const int bs = 256;
-    hipLaunchKernelGGL((k_seam<T, P>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, y,
-                       halo, mask, nex, ney, nfx, nfy, L.tx, L.ty, L.ntx, L.nty, mode, mask_out,
-                       skip);
+    const int lines = (L.ntx - 1) * P + (L.nty - 1) * P;
+    if (lines == 0) return hipSuccess;
+    const int bs = 256, len = nfx > nfy ? nfx : nfy;
+    const int sx = __builtin_ctz((unsigned)L.tx), sy = __builtin_ctz((unsigned)L.ty);
+    hipLaunchKernelGGL((k_seam<T, P>), dim3((len + bs - 1) / bs, lines), dim3(bs), 0, s, y, halo,
+                       mask, nex, ney, nfx, nfy, sx, sy, L.ntx, L.nty, mode, mask_out, skip);
+    return hipGetLastError();
+  }
+
+  template <typename T>
+  static hipError_t seam_dot(T *y, const T *halo, const T *pv, const uint8_t *mask, int nex,
+                             int ney, int nfx, int nfy, const Layout &L, int mask_out,
+                             double *part, int grid, const int *skip, hipStream_t s) {
+    // tile sizes are powers of two
+    const int sx = __builtin_ctz((unsigned)L.tx), sy = __builtin_ctz((unsigned)L.ty);
+    hipLaunchKernelGGL((k_seam_dot<T, P>), dim3(grid), dim3(256), 0, s, y, halo, pv, mask, nex,
+                       ney, nfx, nfy, sx, sy, L.ntx, mask_out, part, skip);
     return hipGetLastError();
   }
 
@@ -145,6 +177,15 @@ struct Launch {
     return hipGetLastError();
   }
 
+  static hipError_t qp_coords(double *xq, const double *tabx, const double *taby,
+                              const double *cp, const double *wts, int nex, int ney, int nfx,
+                              hipStream_t s) {
+    const int bs = 256, ne = nex * ney;
+    hipLaunchKernelGGL((k_qp_coords<P, NQ>), dim3((ne + bs - 1) / bs), dim3(bs), 0, s, xq, tabx,
+                       taby, cp, wts, nex, ney, nfx);
+    return hipGetLastError();
+  }
+
   static hipError_t kappa_grad(double *out, const double *x, const double *g, const double *G0,
                                const double *tabx, const double *taby, const double *wts,
                                const uint8_t *mask, int nex, int ney, int nfx, int nfy,
@@ -160,10 +201,10 @@ struct Launch {
 // 16x16 tiles (and 32x16 for p <= 4); instantiated in kern_p<P>_rat.hip.
 template <int P>
 struct LaunchRat {
-  template <typename T, int TX, int TY, bool EBE>
+  template <typename T, int TX, int TY, bool EBE, int SPLIT = 1>
   static hipError_t run_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<T, P, P + 1, TX, TY, EBE, true>), dim3(ntiles), dim3(TX * TY), 0,
-                       s, a);
+    hipLaunchKernelGGL((k_apply<T, P, P + 1, TX, TY, EBE, true, 1, SPLIT>), dim3(ntiles),
+                       dim3(TX * TY * SPLIT), 0, s, a);
     return hipGetLastError();
   }
   template <typename T, bool EBE>
@@ -171,6 +212,8 @@ struct LaunchRat {
     if constexpr (P <= 4)
       if (tile == TILE_32x16) return run_t<T, 32, 16, EBE>(a, ntiles, s);
     if (tile == TILE_16x16) return run_t<T, 16, 16, EBE>(a, ntiles, s);
+    if (tile == TILE_16x8) return EBE ? run_t<T, 16, 8, true>(a, ntiles, s)
+                                      : run_t<T, 16, 8, false, 2>(a, ntiles, s);
     return hipErrorInvalidValue;
   }
   static hipError_t apply(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles,
@@ -206,6 +249,13 @@ struct Ops {
   hipError_t (*kappa_grad)(double *, const double *, const double *, const double *,
                            const double *, const double *, const double *, const uint8_t *, int,
                            int, int, int, const Layout &, hipStream_t);
+  hipError_t (*seam_dot64)(double *, const double *, const double *, const uint8_t *, int, int,
+                           int, int, const Layout &, int, double *, int, const int *,
+                           hipStream_t);
+  hipError_t (*seam_dot32)(float *, const float *, const float *, const uint8_t *, int, int, int,
+                           int, const Layout &, int, double *, int, const int *, hipStream_t);
+  hipError_t (*qp_coords)(double *, const double *, const double *, const double *,
+                          const double *, int, int, int, hipStream_t);
   hipError_t (*apply_rat)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles,
                           hipStream_t);
   hipError_t (*block)(const BlockArgs &, int cfg, hipStream_t);
@@ -222,6 +272,9 @@ Ops make_ops() {
   o.load = &Launch<P>::load;
   o.diag = &Launch<P>::diag;
   o.kappa_grad = &Launch<P>::kappa_grad;
+  o.qp_coords = &Launch<P>::qp_coords;
+  o.seam_dot64 = &Launch<P>::template seam_dot<double>;
+  o.seam_dot32 = &Launch<P>::template seam_dot<float>;
   o.apply_rat = nullptr;
   o.block = nullptr;
   return o;

@@ -65,6 +65,10 @@ int set_device(int dev);
 // y = A x with an explicit layout / precision (used by the solver)
 int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
               int mask_in, int mask_out, hipStream_t s, const int *skip);
+// y = A x in inverse-LM gather form with the seam gather fused with the
+// per-workgroup partials of pv . y (grid workgroups, partials in part[])
+int apply_gather_dot(iga_op *op, bool f64, const void *x, void *y, const void *pv,
+                     double *part, int grid, hipStream_t s, const int *skip);
 // Y = A X for k FP32 columns (leading dimension ld), the configuration
 // tuned or defaulted for k; no device switch (callable under graph capture)
 int apply_block_raw(iga_op *op, int k, int ld, const float *X, float *Y, hipStream_t s);

@@ -107,15 +107,16 @@ __device__ __forceinline__ bool finite(double x) { return x == x && !isinf(x); }
 // ---------------------------------------------------------------- CG kernels
 // init: u = 0, r = f*fscale, z = dinv r (or r), p = z when use_z;
 // partials r.r and r.z
-template <typename V, typename F>
-__global__ void k_cg_init(const F *f, double fscale, V *u, V *r, V *z, V *p, const double *dinv,
-                          int use_z, int64_t n, double *part) {
+template <typename V, typename F, typename D>
+__global__ void k_cg_init(const F *__restrict__ f, double fscale, V *__restrict__ u,
+                          V *__restrict__ r, V *__restrict__ z, V *__restrict__ p,
+                          const D *__restrict__ dinv, int use_z, int64_t n, double *part) {
   double rr = 0.0, rz = 0.0;
   GRID_LOOP(i, n) {
     const V ri = (V)((double)f[i] * fscale);
     u[i] = V(0);
     r[i] = ri;
-    const V zi = dinv ? (V)((double)ri * dinv[i]) : ri;
+    const V zi = dinv ? (V)((double)ri * (double)dinv[i]) : ri;
     if (use_z) {
       z[i] = zi;
       p[i] = zi;
@@ -146,7 +147,7 @@ __global__ void k_cg_init_fin(const double *part, int nb, double *sc, int *ic, d
   }
 }
 
-// K1: partials of p.Ap; bookkeeping for K3
+// K1: partials of p.Ap
 // Column j = blockIdx.y of a block solve: vectors at j * ld, scalars, flags
 // and partial slots of instance j (ld = 0 and one column otherwise).
 #define CG_COLUMN(ld)                                 \
@@ -157,36 +158,52 @@ __global__ void k_cg_init_fin(const double *part, int nb, double *sc, int *ic, d
   ic += (size_t)col_ * I_NI;
 
 template <typename V>
-__global__ void k_cg_pap(const V *p, const V *Ap, int64_t n, double *part, double *sc, int *ic,
-                         const int *outer_done, int64_t ld) {
+__global__ void k_cg_pap(const V *__restrict__ p, const V *__restrict__ Ap, int64_t n,
+                         double *part, double *sc, int *ic, const int *outer_done, int64_t ld) {
   CG_COLUMN(ld)
   p += voff_;
   Ap += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
+  constexpr int U = 4;
   double s = 0.0;
-  GRID_LOOP(i, n) s += (double)p[i] * (double)Ap[i];
-  put_partial(part + P_PAP * RG, s);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc[S_RZ_OLD] = sc[S_RZ];
-    ic[I_NEXT] = ic[I_ITER] + 1;
+  for (int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x; b0 < n;
+       b0 += (int64_t)gridDim.x * RB * U) {
+    V pv[U], av[U];
+    if (b0 + (int64_t)(U - 1) * RB < n) {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        pv[k] = p[b0 + (int64_t)k * RB];
+        av[k] = Ap[b0 + (int64_t)k * RB];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b0 + (int64_t)k * RB;
+        pv[k] = i < n ? p[i] : V(0);
+        av[k] = i < n ? Ap[i] : V(0);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) s += (double)pv[k] * (double)av[k];
   }
+  put_partial(part + P_PAP * RG, s);
 }
 
 // K2: alpha; u += alpha p; r -= alpha Ap (rold keeps r for the flexible
 // beta); z = dinv r when the preconditioner is local; partials of r.r (or
 // z.z with prec_norm) and r.z
-template <typename V>
-__global__ void k_cg_update(V *u, V *r, const V *p, const V *Ap, V *z, V *rold,
-                            const double *dinv, int local_pc, int prec_norm, int64_t n,
-                            double *part, double *sc, int *ic, int nb, const int *outer_done,
-                            int64_t ld) {
+template <typename V, typename D, bool LOCAL, bool ROLD>
+__global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__restrict__ p,
+                            const V *__restrict__ Ap, V *__restrict__ z, V *__restrict__ rold,
+                            const D *__restrict__ dinv, int prec_norm, int64_t n, double *part,
+                            double *sc, int *ic, int nb, const int *outer_done, int64_t ld) {
   CG_COLUMN(ld)
   u += voff_;
   r += voff_;
   p += voff_;
   Ap += voff_;
   z += voff_;
-  if (rold) rold += voff_;
+  if (ROLD) rold += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
   const double pap = fold(part + P_PAP * RG, nb);
   const double a = sc[S_RZ] / pap;
@@ -194,24 +211,61 @@ __global__ void k_cg_update(V *u, V *r, const V *p, const V *Ap, V *z, V *rold,
     if (blockIdx.x == 0 && threadIdx.x == 0) ic[I_DONE] = 3;
     return;
   }
+  // four elements per thread and step, every load issued before any store
+  constexpr int U = 4;
   double rr = 0.0, rz = 0.0;
-  GRID_LOOP(i, n) {
-    u[i] = (V)((double)u[i] + a * (double)p[i]);
-    if (rold) rold[i] = r[i];
-    const V ri = (V)((double)r[i] - a * (double)Ap[i]);
-    r[i] = ri;
-    if (local_pc) {
-      const V zi = dinv ? (V)((double)ri * dinv[i]) : ri;
-      z[i] = zi;
-      rz += (double)ri * (double)zi;
-      rr += prec_norm ? (double)zi * (double)zi : (double)ri * (double)ri;
+  for (int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x; b0 < n;
+       b0 += (int64_t)gridDim.x * RB * U) {
+    V uu[U], pv[U], rv[U], av[U];
+    double dv[U];
+    if (b0 + (int64_t)(U - 1) * RB < n) {  // whole step in range: unconditional loads
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b0 + (int64_t)k * RB;
+        uu[k] = u[i];
+        pv[k] = p[i];
+        rv[k] = r[i];
+        av[k] = Ap[i];
+        dv[k] = (LOCAL && dinv) ? (double)dinv[i] : 1.0;
+      }
     } else {
-      rr += (double)ri * (double)ri;
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b0 + (int64_t)k * RB;
+        const bool in = i < n;
+        uu[k] = in ? u[i] : V(0);
+        pv[k] = in ? p[i] : V(0);
+        rv[k] = in ? r[i] : V(0);
+        av[k] = in ? Ap[i] : V(0);
+        dv[k] = (LOCAL && dinv && in) ? (double)dinv[i] : 1.0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t i = b0 + (int64_t)k * RB;
+      if (i >= n) continue;
+      u[i] = (V)((double)uu[k] + a * (double)pv[k]);
+      if (ROLD) rold[i] = rv[k];
+      const V ri = (V)((double)rv[k] - a * (double)av[k]);
+      r[i] = ri;
+      if (LOCAL) {
+        const V zi = (V)((double)ri * dv[k]);
+        z[i] = zi;
+        rz += (double)ri * (double)zi;
+        rr += prec_norm ? (double)zi * (double)zi : (double)ri * (double)ri;
+      } else {
+        rr += (double)ri * (double)ri;
+      }
     }
   }
   put_partial(part + P_RR * RG, rr);
-  if (local_pc) put_partial(part + P_RZ * RG, rz);
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHA] = a;
+  if (LOCAL) put_partial(part + P_RZ * RG, rz);
+  // bookkeeping for K3 (slots no workgroup of this kernel reads)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_ALPHA] = a;
+    sc[S_RZ_OLD] = sc[S_RZ];
+    ic[I_NEXT] = ic[I_ITER] + 1;
+  }
 }
 
 // after a preconditioner applied by its own kernels: partials of r.z and
@@ -231,8 +285,9 @@ __global__ void k_cg_rz(const V *r, const V *z, const V *rold, int64_t n, double
 
 // K3: convergence test, beta, p = z + beta p
 template <typename V>
-__global__ void k_cg_pupdate(V *p, const V *z, int64_t n, const double *part, double *sc,
-                             int *ic, int nb, int flexible, const int *outer_done, int64_t ld) {
+__global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t n,
+                             const double *part, double *sc, int *ic, int nb, int flexible,
+                             const int *outer_done, int64_t ld) {
   CG_COLUMN(ld)
   p += voff_;
   z += voff_;
@@ -255,7 +310,30 @@ __global__ void k_cg_pupdate(V *p, const V *z, int64_t n, const double *part, do
     if (st) ic[I_DONE] = st;
   }
   if (st && st != 4) return;  // a reliable update keeps the search direction
-  GRID_LOOP(i, n) p[i] = (V)((double)z[i] + b * (double)p[i]);
+  constexpr int U = 4;
+  for (int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x; b0 < n;
+       b0 += (int64_t)gridDim.x * RB * U) {
+    V zv[U], pv[U];
+    if (b0 + (int64_t)(U - 1) * RB < n) {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        zv[k] = z[b0 + (int64_t)k * RB];
+        pv[k] = p[b0 + (int64_t)k * RB];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b0 + (int64_t)k * RB;
+        zv[k] = i < n ? z[i] : V(0);
+        pv[k] = i < n ? p[i] : V(0);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t i = b0 + (int64_t)k * RB;
+      if (i < n) p[i] = (V)((double)zv[k] + b * (double)pv[k]);
+    }
+  }
 }
 
 // first preconditioner application of a non-local preconditioner:
@@ -287,14 +365,15 @@ __global__ void k_ru_accum(double *x64, float *x32, int64_t n) {
 }
 
 // r32 = r64 (true residual); z32 = dinv r32 when local; partials r.z
-__global__ void k_ru_reset(const double *r64, float *r32, float *z32, const double *dinv,
-                           int local_pc, int64_t n, double *part) {
+__global__ void k_ru_reset(const double *__restrict__ r64, float *__restrict__ r32,
+                           float *__restrict__ z32, const float *__restrict__ dinv, int local_pc,
+                           int64_t n, double *part) {
   double rz = 0.0;
   GRID_LOOP(i, n) {
     const float ri = (float)r64[i];
     r32[i] = ri;
     if (local_pc) {
-      const float zi = dinv ? (float)((double)ri * dinv[i]) : ri;
+      const float zi = dinv ? (float)((double)ri * (double)dinv[i]) : ri;
       z32[i] = zi;
       rz += (double)ri * (double)zi;
     }
@@ -451,6 +530,7 @@ struct iga_solver {
   // FP64 state
   double *r = nullptr, *p = nullptr, *Ap = nullptr, *z = nullptr, *rold = nullptr;
   double *Au = nullptr, *dinv = nullptr;  // Jacobi of A (inverse diagonal)
+  float *dinv32 = nullptr;                // the same, rounded, for the FP32 inner PCG
   double *x64 = nullptr;                  // FP64 PCG iterate (owned: graphs capture it)
   // FP32 inner state
   float *d32 = nullptr, *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *z32 = nullptr,
@@ -520,8 +600,8 @@ int fem_pc(iga_solver *s, const V *r, V *z, V *rin, V *w, V *pp, V *q, const V *
     hipLaunchKernelGGL((k_spmv<V>), dim3((s->fem_rows + bs - 1) / bs), dim3(bs), 0, st, s->rowptr,
                        s->col, val, pp, q, s->fem_rows, ic + I_DONE, od);
     LAUNCH((k_cg_pap<V>), nb, pp, q, n, s->part_in, sc, ic, od, (int64_t)0);
-    LAUNCH((k_cg_update<V>), nb, z, rin, pp, q, w, (V *)nullptr, s->fdinv, 1, 1, n, s->part_in,
-           sc, ic, nb, od, (int64_t)0);
+    LAUNCH((k_cg_update<V, double, true, false>), nb, z, rin, pp, q, w, (V *)nullptr, s->fdinv,
+           1, n, s->part_in, sc, ic, nb, od, (int64_t)0);
     LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od, (int64_t)0);
   }
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
@@ -544,15 +624,28 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
   const int nb = s->nb;
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
-  int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE);
-  if (rc) return rc;
-  LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr, (int64_t)0);
   const bool loc = local_pc(s);
-  const double *dinv = s->o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
-  LAUNCH((k_cg_update<V>), nb, u, r, p, Ap, z, loc ? (V *)nullptr : rold, dinv, loc ? 1 : 0, 0,
-         n, s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0);
+  if (gather && !ebe) {
+    // seam gather fused with the p.Ap partials
+    int rc = igai::apply_gather_dot(s->op, f64, p, Ap, p, s->part + P_PAP * RG, nb, st,
+                                    s->ic + I_DONE);
+    if (rc) return rc;
+  } else {
+    int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE);
+    if (rc) return rc;
+    LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr, (int64_t)0);
+  }
+  const V *dinv = s->o.pc != IGA_PC_JACOBI
+                      ? nullptr
+                      : (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32);
+  if (loc)
+    LAUNCH((k_cg_update<V, V, true, false>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n, s->part,
+           s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0);
+  else
+    LAUNCH((k_cg_update<V, V, false, true>), nb, u, r, p, Ap, z, rold, (const V *)nullptr, 0, n,
+           s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0);
   if (!loc) {
-    rc = apply_pc<V>(s, f64, r, z, st);
+    int rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
     LAUNCH((k_cg_rz<V>), nb, r, z, rold, n, s->part, s->ic);
   }
@@ -702,16 +795,16 @@ int block_buffers(iga_solver *s, int k) {
 int bcg_chunk(iga_solver *s, int k, hipStream_t st) {
   const int64_t n = s->n;
   const int nb = s->nb;
-  const double *dinv = s->o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
+  const float *dinv = s->o.pc == IGA_PC_JACOBI ? s->dinv32 : nullptr;
   const dim3 grid(nb, k);
   for (int it = 0; it < CHUNK; ++it) {
     int rc = igai::apply_block_raw(s->op, k, (int)n, s->bp32, s->bAp32, st);
     if (rc) return rc;
     hipLaunchKernelGGL((k_cg_pap<float>), grid, dim3(RB), 0, st, s->bp32, s->bAp32, n, s->bpart,
                        s->bsc, s->bic, (const int *)nullptr, n);
-    hipLaunchKernelGGL((k_cg_update<float>), grid, dim3(RB), 0, st, s->bd32, s->br32, s->bp32,
-                       s->bAp32, s->bz32, (float *)nullptr, dinv, 1, 0, n, s->bpart, s->bsc,
-                       s->bic, nb, (const int *)nullptr, n);
+    hipLaunchKernelGGL((k_cg_update<float, float, true, false>), grid, dim3(RB), 0, st, s->bd32,
+                       s->br32, s->bp32, s->bAp32, s->bz32, (float *)nullptr, dinv, 0, n, s->bpart,
+                       s->bsc, s->bic, nb, (const int *)nullptr, n);
     hipLaunchKernelGGL((k_cg_pupdate<float>), grid, dim3(RB), 0, st, s->bp32, s->bz32, n,
                        s->bpart, s->bsc, s->bic, nb, 0, (const int *)nullptr, n);
   }
@@ -831,6 +924,7 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   SA(s->part, (size_t)P_NP * RG * 8);
   SA(s->part_in, (size_t)P_NP * RG * 8);
   if (o.kind == IGA_SOLVE_PCG_F32_REFINE) {
+    SA(s->dinv32, n * 4);
     SA(s->d32, n * 4);
     SA(s->r32, n * 4);
     SA(s->p32, n * 4);
@@ -885,6 +979,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
   std::memset(&stt, 0, sizeof(stt));
   const bool loc = local_pc(s);
   const double *dinv = o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
+  const float *dinv32 = o.pc == IGA_PC_JACOBI ? s->dinv32 : nullptr;
   if (s->gen != s->op->gen) {
     // Jacobi of A for the operator's current G_q
     iga_op *op = s->op;
@@ -892,13 +987,15 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
                       op->ney, op->nfx, op->nfy, op->L64, st) != hipSuccess)
       return IGA_ERR_HIP;
     LAUNCH(k_inv, nb, s->dinv, n);
+    if (s->dinv32) LAUNCH((k_cast<float>), nb, s->dinv, s->dinv32, n);
     s->gen = op->gen;
   }
   if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
   const auto t0 = std::chrono::steady_clock::now();
   int status = IGA_OK;
   if (o.kind == IGA_SOLVE_PCG_F64) {
-    LAUNCH((k_cg_init<double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, dinv, loc ? 1 : 0,
+    LAUNCH((k_cg_init<double, double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, dinv,
+           loc ? 1 : 0,
            n, s->part);
     hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
                        o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit, 0.0);
@@ -933,7 +1030,8 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
     } else if (rr <= tol2) {
       status = IGA_OK;
     } else {
-      LAUNCH((k_cg_init<float, double>), nb, s->r, 1.0, s->d32, s->r32, s->z32, s->p32, dinv,
+      LAUNCH((k_cg_init<float, double, float>), nb, s->r, 1.0, s->d32, s->r32, s->z32, s->p32,
+             dinv32,
              loc ? 1 : 0, n, s->part);
       hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
                          tol2 / rr, 0.0, o.divtol * o.divtol * ff / rr, o.maxit, RU2);
@@ -958,7 +1056,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
         if (!(rr == rr) || rr > o.divtol * o.divtol * ff) { status = IGA_ERR_DIVERGED; break; }
         if (rr <= tol2) { status = IGA_OK; break; }
         if (done == 2) break;
-        LAUNCH(k_ru_reset, nb, s->r, s->r32, s->z32, dinv, loc ? 1 : 0, n, s->part);
+        LAUNCH(k_ru_reset, nb, s->r, s->r32, s->z32, dinv32, loc ? 1 : 0, n, s->part);
         if (!loc) {
           hipLaunchKernelGGL(k_ru_clear, dim3(1), dim3(64), 0, st, s->ic);
           rc = apply_pc<float>(s, false, s->r32, s->z32, st);
@@ -1000,13 +1098,14 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
   const int nb = s->nb;
   rc = block_buffers(s, k);
   if (rc) return rc;
-  const double *dinv = o.pc == IGA_PC_JACOBI ? s->dinv : nullptr;
+  const float *dinv32 = o.pc == IGA_PC_JACOBI ? s->dinv32 : nullptr;
   if (s->gen != s->op->gen) {
     iga_op *op = s->op;
     if (op->ops->diag(s->dinv, op->G64, op->tabx64, op->taby64, op->wts64, op->dirmask, op->nex,
                       op->ney, op->nfx, op->nfy, op->L64, st) != hipSuccess)
       return IGA_ERR_HIP;
     LAUNCH(k_inv, nb, s->dinv, n);
+    LAUNCH((k_cast<float>), nb, s->dinv, s->dinv32, n);
     s->gen = op->gen;
   }
   if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
@@ -1028,8 +1127,8 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
     rrf[j] = h[0];
     const double tol = std::fmax(o.rtol * std::sqrt(ff[j]), o.abstol);
     tol2[j] = tol * tol;
-    LAUNCH((k_cg_init<float, double>), nb, s->r, 1.0, s->bd32 + (size_t)j * n,
-           s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, s->bp32 + (size_t)j * n, dinv, 1,
+    LAUNCH((k_cg_init<float, double, float>), nb, s->r, 1.0, s->bd32 + (size_t)j * n,
+           s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, s->bp32 + (size_t)j * n, dinv32, 1,
            n, colp(j));
     const double rr = h[0];
     const bool done = !(rr == rr) || rr <= tol2[j];
@@ -1088,7 +1187,7 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
         status[j] = IGA_ERR_NOT_CONVERGED;
         fin[j] = 1;
       } else {
-        LAUNCH(k_ru_reset, nb, s->r, s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, dinv, 1,
+        LAUNCH(k_ru_reset, nb, s->r, s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, dinv32, 1,
                n, colp(j));
         hipLaunchKernelGGL(k_ru_fin, dim3(1), dim3(RB), 0, st, colp(j), nb, colsc(j), colic(j),
                            RU2 * h[0]);

@@ -110,6 +110,7 @@ def _lib():
         L.iga_op_load.argtypes = [VP, C.c_int, C.c_double, VP, VP]
         L.iga_op_profile.argtypes = [VP, C.c_int, C.c_int]
         L.iga_op_profile_read.argtypes = [VP, C.POINTER(C.c_int)] + [C.POINTER(C.c_double)] * 3
+        L.iga_op_qp_coords.argtypes = [VP, VP, VP]
         L.iga_op_set_kappa.argtypes = [VP, VP, VP]
         L.iga_op_apply_backward.argtypes = [VP, VP, VP, VP, VP, VP]
         L.iga_op_apply_block.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP]
@@ -309,6 +310,13 @@ class Operator:
               "op_load")
         return F
 
+    def qp_coords(self, stream=None):
+        """Physical quadrature-point coordinates, float64 CUDA [E, nqp, 2]."""
+        import torch
+        xq = torch.empty((self.nelem, self.nqp, 2), dtype=torch.float64, device=self.device)
+        check(_lib().iga_op_qp_coords(self._h, xq.data_ptr(), _stream_ptr(stream)), "qp_coords")
+        return xq
+
     def set_kappa(self, kappa, stream=None):
         assert kappa.dtype.is_floating_point and kappa.numel() == self.nelem * self.nqp
         k = kappa.contiguous().double()
@@ -477,9 +485,9 @@ class Solver:
         import torch
         if F.dtype != torch.float64 or F.dim() != 2 or F.stride(1) != 1:
             raise ValueError("F must be a 2-D float64 CUDA tensor with contiguous rows")
-        k = F.shape[0]
-        if U is None:
-            U = torch.zeros_like(F)
+        k, n = F.shape
+        if U is None:  # same leading dimension as F
+            U = torch.zeros((k, F.stride(0)), dtype=F.dtype, device=F.device)[:, :n]
         if U.shape != F.shape or U.stride() != F.stride():
             raise ValueError("U must match F in shape and strides")
         st = (SolveStats * k)()

@@ -82,21 +82,49 @@ def test_apply_matches_oracle(built, name, kw):
 
 @pytest.mark.parametrize("tile", [(32, 16), (16, 16), (64, 8), (32, 32)])
 def test_pipelined_variant_and_tiles(tile):
-    """Persistent pipelined SF kernel and every compiled tile shape, on a
-    ragged mesh whose element counts are not multiples of the tile."""
+    """Persistent pipelined SF kernel, the four-wave FP32 kernel and every
+    compiled tile shape, on a ragged mesh whose element counts are not
+    multiples of the tile."""
     m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 77, 45)
     om = oracle_mesh(m)
     x = igamf.rng_normal(2, om.nfunc)
     y_ref = O.apply_mf(om, x)
     xd = torch.from_numpy(x).cuda()
-    for variant in (0, 1):
-        op = igamf.Operator(m, tile_f64=tile, tile_f32=tile, variant_f64=variant,
-                            variant_f32=variant)
+    for variant in (0, 1, 2, 3):
+        if variant == 2 and tile != (16, 16):
+            continue  # the four-wave FP32 kernel exists for 16x16 only
+        if variant == 3 and tile != (16, 16):
+            continue  # the split-element kernel has its own 16x8 tile
+        v64 = 1 if variant == 2 else variant
+        t = None if variant == 3 else tile
+        op = igamf.Operator(m, tile_f64=t, tile_f32=t, variant_f64=v64, variant_f32=variant)
+        assert op.info.variant_f32 == variant
         for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
             y64 = op.apply(xd, assembly=asm).cpu().numpy()
             assert rel(y64, y_ref) < TOL64, (tile, variant, asm, rel(y64, y_ref))
             y32 = op.apply(xd.float(), assembly=asm).double().cpu().numpy()
             assert rel(y32, y_ref) < TOL32, (tile, variant, asm, rel(y32, y_ref))
+
+
+@pytest.mark.parametrize("name,kw", [c for c in CASES if c[1]["p"] >= 3],
+                         ids=[c[0] for c in CASES if c[1]["p"] >= 3])
+def test_split_element_variant(built, name, kw):
+    """Two threads per element (variant 3, 16x8 tiles) on every mesh of
+    degree >= 3, general rational ones included; its EBE companion too."""
+    m, om, _ = built(name, kw)
+    op = igamf.Operator(m, variant_f64=3, variant_f32=3)
+    assert (op.info.tile_x_f32, op.info.tile_y_f32) == (16, 8)
+    x = igamf.rng_normal(0, op.nfunc)
+    y_ref = O.apply_mf(om, x)
+    xd = torch.from_numpy(x).cuda()
+    for kern in (igamf.SF, igamf.EBE):
+        for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
+            y64 = op.apply(xd, kernel=kern, assembly=asm).cpu().numpy()
+            assert rel(y64, y_ref) < TOL64, (name, kern, asm, rel(y64, y_ref))
+            y32 = op.apply(xd.float(), kernel=kern, assembly=asm).double().cpu().numpy()
+            assert rel(y32, y_ref) < TOL32, (name, kern, asm, rel(y32, y_ref))
+    with pytest.raises(igamf.IgaError):
+        igamf.Operator(m, variant_f32=3, tile_f32=(16, 16))
 
 
 def test_general_rational_mode_and_tiles():

@@ -145,3 +145,19 @@ def test_adjoint_gradient_of_solve(case):
         fd = (Jp - Jm) / (2 * h)
         an = (grad * d).sum().item()
         assert abs(fd - an) < 1e-5 * abs(an) + 1e-12, (s, fd, an)
+
+
+@pytest.mark.parametrize("geom", ["annulus", "rational"])
+def test_qp_coords_match_oracle(geom):
+    if geom == "annulus":
+        m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 9, 7)
+    else:
+        m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, 2, 8, perturb=0.1, w_lo=0.8,
+                             w_hi=1.2, seed=4)
+    op = igamf.Operator(m)
+    xq = op.qp_coords().cpu().numpy()
+    om = _om(m)
+    for ey in range(om.ney):
+        for ex in range(om.nex):
+            ref = O.element_qp(om, ex, ey)[4]
+            assert np.abs(xq[ey * om.nex + ex] - ref).max() < 1e-14

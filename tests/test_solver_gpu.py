@@ -190,9 +190,11 @@ def test_block_solve_matches_single_solves():
         assert (U[j] - u).abs().max().item() < 1e-7 * u.abs().max().item(), j
     assert stats[k - 1].status == 0 and stats[k - 1].iterations == 0
     assert U[k - 1].abs().max().item() == 0.0
-    # repeated block solve (graph replay) gives the same result
-    U2, _ = s.solve_block(F)
-    assert torch.equal(U, U2)
+    # a repeated block solve (graph replay) agrees to the tolerance; not bitwise,
+    # since the multi-RHS apply adds seam sums with atomics
+    U2, st2 = s.solve_block(F)
+    assert all(x.status == 0 for x in st2)
+    assert (U - U2).abs().max().item() < 1e-7 * U.abs().max().item()
     with pytest.raises(igamf.IgaError) as e:
         s64.solve_block(F)
     assert e.value.status == 6
