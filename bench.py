@@ -121,12 +121,26 @@ def headline(a, out):
         torch.cuda.synchronize()
         del src, dst
     y32 = torch.empty_like(x32)
+    # the contract's timing: W untimed steps, then K steps between two
+    # synchronisations, wall clock
+    for _ in range(a.warmup):
+        op.apply(x32, y32, kernel=kern, assembly=asm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        op.apply(x32, y32, kernel=kern, assembly=asm)
+    torch.cuda.synchronize()
+    wall32 = (time.perf_counter() - t0) * 1e3 / a.steps
     med32, mean32 = ev_time(lambda: op.apply(x32, y32, kernel=kern, assembly=asm),
                             a.steps, a.warmup)
     k_avg, k_min = kernel_avg_ms(op, x32, y32, a.steps, kern, asm)
+    # the element kernel alone, back to back between two events (no event
+    # per launch): the figure the roofline uses
+    k_b2b = op.time_kernel(x32, torch.empty_like(x32), a.steps, kern, asm)
+    op.apply(x32, y32, kernel=kern, assembly=asm)
     alg_bytes32 = 4 * (2 * nfunc) + 4 * 3 * E * nqp
-    achieved = alg_bytes32 / (k_avg * 1e-3) / 1e9
-    gdofs = n_int / (med32 * 1e-3) / 1e9
+    achieved = alg_bytes32 / (k_b2b * 1e-3) / 1e9
+    gdofs = n_int / (wall32 * 1e-3) / 1e9
     variant = info.variant_f32
     tx, ty = info.tile_x_f32, info.tile_y_f32
     if variant == 1 and a.kernel == "sf":
@@ -149,7 +163,7 @@ def headline(a, out):
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(med32, 5),
+        "ms_per_step": round(wall32, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -167,13 +181,17 @@ def headline(a, out):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": kname,
-                     "kernel_avg_ms": round(k_avg, 5), "kernel_min_ms": round(k_min, 5),
+                     "kernel_avg_ms": round(k_b2b, 5),
+                     "kernel_avg_ms_event_per_launch": round(k_avg, 5),
+                     "kernel_min_ms_event_per_launch": round(k_min, 5),
                      "algorithmic_bytes": alg_bytes32},
-        "apply": {"f32_median_ms": round(med32, 5), "f32_mean_ms": round(mean32, 5),
+        "apply": {"f32_wall_ms_per_step": round(wall32, 5),
+                  "f32_event_median_ms": round(med32, 5), "f32_event_mean_ms": round(mean32, 5),
                   "setup_s": round(setup_s, 3), "op_setup_ms": round(info.setup_ms, 2),
                   "device_bytes": info.device_bytes},
     })
-    log(f"headline f32 apply {med32 * 1e3:.1f} us, kernel {k_avg * 1e3:.1f} us, "
+    log(f"headline f32 apply {wall32 * 1e3:.1f} us/step, kernel {k_b2b * 1e3:.1f} us "
+        f"({k_avg * 1e3:.1f} us with an event pair per launch), "
         f"{achieved / HBM_PEAK_GBS:.3f} of HBM")
     return mesh, op, x64, x32, kern, asm, n_int
 
@@ -230,7 +248,7 @@ def solves(out, op):
         res[f"speedup_fp32_refine_vs_fp64_{an}"] = round(
             res[f"fp64_{an}"]["solve_ms"] / res[f"fp32_refine_{an}"]["solve_ms"], 3)
     res["preconditioner"] = "jacobi"
-    res["inner_rtol"] = 0.1
+    res["inner_rtol"] = "library default (0.5)"
     out["solve"] = res
 
 
@@ -302,7 +320,7 @@ def c5(out):
     y32 = torch.empty_like(x32)
     y64 = torch.empty_like(x)
     t32, _ = ev_time(lambda: op.apply(x32, y32), 10, 2)
-    k32, _ = kernel_avg_ms(op, x32, y32, 10, igamf.SF, igamf.INVLM_GATHER)
+    k32 = op.time_kernel(x32, torch.empty_like(x32), 10)
     t64e, _ = ev_time(lambda: op.apply(x, y64, kernel=igamf.EBE), 5, 1)
     nc = 5 if info.rational == 2 else 3
     b32 = 4 * (2 * op.nfunc) + 4 * nc * op.nelem * op.nqp + 4 * op.nfunc

@@ -178,6 +178,13 @@ int iga_op_get_info(const iga_op *op, iga_op_info *info);
 int iga_op_apply(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb,
                  const void *x, void *y, void *stream);
 
+/* Average duration of the element kernel alone (k_apply or its variant,
+ * without the seam kernel), from HIP events around nlaunch back-to-back
+ * launches on the stream after one warm-up launch.  y (and, in gather
+ * form, the halo slots) receives partial results: a timing call, not an
+ * apply.  No counterpart in the reference. */
+int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, void *y,
+                       int nlaunch, void *stream, double *avg_ms);
 /* Kernel timing of the element-action kernel (k_apply) with HIP events on
  * the stream it runs on.  No counterpart in the reference, which times whole
  * solves only (PetscTime, matrix_free_gpu/matrixfreegpu.cu:299-306).
@@ -269,7 +276,10 @@ typedef enum {
   IGA_PC_JACOBI = 1,     /* diagonal of A */
   IGA_PC_FEM_CG = 2,     /* inner CG + Jacobi on the Q1 FEM matrix
                             (matrix_free_gpu/matrixfreegpu.cu:18-24,273-289) */
-  IGA_PC_FEM_CHEB = 3    /* fixed-degree Chebyshev-Jacobi on the Q1 matrix */
+  IGA_PC_FEM_CHEB = 3    /* fixed-degree Chebyshev-Jacobi polynomial of the Q1
+                            matrix (a linear preconditioner; interval
+                            [lmax/30, lmax], lmax the Gershgorin bound of
+                            D^-1 K); no counterpart in the reference */
 } iga_pc_kind;
 
 typedef enum {
@@ -286,7 +296,7 @@ typedef struct {
   double rtol, abstol, divtol;  /* on ||f - A u||_2 (FP64) */
   int maxit;                    /* total inner iterations cap */
   double inner_rtol;            /* F32_REFINE: residual reduction between
-                                   reliable (FP64) updates */
+                                   reliable (FP64) updates (default 0.5) */
   int inner_maxit;
   int pc_inner_maxit;           /* FEM_CG: inner CG iteration cap */
   double pc_inner_rtol;         /* FEM_CG: inner CG tolerance */

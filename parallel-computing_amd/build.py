@@ -33,7 +33,7 @@ def sources():
     def cost(path):
         fn = os.path.basename(path)
         if fn.startswith("kern_p"):
-            return -(int(fn[6]) + (2 if "_block" in fn else 0))
+            return -(int(fn[6]) * 4 + (0 if "_block" in fn else 2))
         return 0
     return sorted(out, key=cost)
 

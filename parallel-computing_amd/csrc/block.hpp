@@ -95,7 +95,7 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
   oa.nfy = A.nfy;
   oa.mode = 0;
   oa.mask_out = 1;
-  auto gl = [&](int c, int q) { return g[c][q]; };
+  auto gl = [&](int c, int q, int, int) { return g[c][q]; };
   const int nr = (A.k - j0) < RC ? (A.k - j0) : RC;
   for (int r = 0; r < nr; ++r) {
     float ye[NB][NB];

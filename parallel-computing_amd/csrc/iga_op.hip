@@ -155,8 +155,8 @@ int dev_alloc(iga_op *op, void **p, size_t bytes) {
   return IGA_OK;
 }
 
-int apply_raw_impl(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
-                   int mask_in, int mask_out, hipStream_t s, const int *skip, bool seam) {
+int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
+              int mask_in, int mask_out, hipStream_t s, const int *skip, bool y_zeroed) {
   const igak::Layout &L = f64 ? op->L64 : op->L32;
   igak::ApplyArgs a;
   a.x = x;
@@ -180,7 +180,7 @@ int apply_raw_impl(iga_op *op, bool f64, bool ebe, bool gather, const void *x, v
   if (!a.G) return IGA_ERR_UNSUPPORTED;
   const int ntiles = L.ntx * L.nty;
   const int tile = f64 ? op->tile64 : op->tile32;
-  if (!gather) {
+  if (!gather && !y_zeroed) {
     if (f64)
       IGA_HIP_TRY(op->ops->seam64((double *)y, nullptr, op->dirmask, op->nex, op->ney, op->nfx,
                                   op->nfy, L, 0, mask_out, skip, s));
@@ -191,14 +191,15 @@ int apply_raw_impl(iga_op *op, bool f64, bool ebe, bool gather, const void *x, v
   const bool prof = op->prof_on && op->prof_used < op->prof_cap;
   if (prof) IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used], s));
   if (a.wf)
-    IGA_HIP_TRY(op->ops->apply_rat(a, f64, ebe, tile, ntiles, s));
+    IGA_HIP_TRY((f64 ? op->ops->rat64 : op->ops->rat32)(a, ebe, tile, ntiles, s));
   else
-    IGA_HIP_TRY(op->ops->apply(a, f64, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s));
+    IGA_HIP_TRY((f64 ? op->ops->apply64 : op->ops->apply32)(
+        a, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s));
   if (prof) {
     IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used + 1], s));
     ++op->prof_used;
   }
-  if (gather && seam) {
+  if (gather) {
     if (f64)
       IGA_HIP_TRY(op->ops->seam64((double *)y, op->halo64, op->dirmask, op->nex, op->ney,
                                   op->nfx, op->nfy, L, 1, mask_out, skip, s));
@@ -206,27 +207,6 @@ int apply_raw_impl(iga_op *op, bool f64, bool ebe, bool gather, const void *x, v
       IGA_HIP_TRY(op->ops->seam32((float *)y, op->halo32, op->dirmask, op->nex, op->ney,
                                   op->nfx, op->nfy, L, 1, mask_out, skip, s));
   }
-  return IGA_OK;
-}
-
-int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
-              int mask_in, int mask_out, hipStream_t s, const int *skip) {
-  return apply_raw_impl(op, f64, ebe, gather, x, y, mask_in, mask_out, s, skip, true);
-}
-
-int apply_gather_dot(iga_op *op, bool f64, const void *x, void *y, const void *pv,
-                     double *part, int grid, hipStream_t s, const int *skip) {
-  int rc = apply_raw_impl(op, f64, false, true, x, y, 0, 1, s, skip, false);
-  if (rc) return rc;
-  const igak::Layout &L = f64 ? op->L64 : op->L32;
-  if (f64)
-    IGA_HIP_TRY(op->ops->seam_dot64((double *)y, op->halo64, (const double *)pv, op->dirmask,
-                                    op->nex, op->ney, op->nfx, op->nfy, L, 1, part, grid, skip,
-                                    s));
-  else
-    IGA_HIP_TRY(op->ops->seam_dot32((float *)y, op->halo32, (const float *)pv, op->dirmask,
-                                    op->nex, op->ney, op->nfx, op->nfy, L, 1, part, grid, skip,
-                                    s));
   return IGA_OK;
 }
 
@@ -677,6 +657,60 @@ int iga_op_get_block_config(const iga_op *op, int k, int *cfg) {
   *cfg = block_default(op, k);
   for (auto &kv : op->block_tuned)
     if (kv.first == k) *cfg = kv.second;
+  return IGA_OK;
+}
+
+int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, void *y,
+                       int nlaunch, void *stream, double *avg_ms) {
+  if (!op || !x || !y || x == y || nlaunch < 1 || !avg_ms) return IGA_ERR_INVALID_ARGUMENT;
+  if (dt != IGA_F64 && dt != IGA_F32) return IGA_ERR_INVALID_ARGUMENT;
+  const bool f64 = dt == IGA_F64;
+  if ((f64 && !op->G64) || (!f64 && !op->G32)) return IGA_ERR_UNSUPPORTED;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const igak::Layout &L = f64 ? op->L64 : op->L32;
+  igak::ApplyArgs a;
+  a.x = x;
+  a.y = y;
+  a.G = f64 ? (const void *)op->G64 : (const void *)op->G32;
+  a.tabx = f64 ? (const void *)op->tabx64 : (const void *)op->tabx32;
+  a.taby = f64 ? (const void *)op->taby64 : (const void *)op->taby32;
+  a.dirmask = op->dirmask;
+  a.halo = f64 ? (void *)op->halo64 : (void *)op->halo32;
+  a.wf = f64 ? (const void *)op->wts64 : (const void *)op->wts32;
+  a.nex = op->nex;
+  a.ney = op->ney;
+  a.nfx = op->nfx;
+  a.nfy = op->nfy;
+  a.ntx = L.ntx;
+  a.nty = L.nty;
+  a.mode = asmb == IGA_ASSEMBLY_INVLM_GATHER ? 1 : 0;
+  a.mask_in = 0;
+  a.mask_out = 1;
+  a.skip = nullptr;
+  const bool ebe = kern == IGA_KERNEL_EBE;
+  const int ntiles = L.ntx * L.nty;
+  const int tile = f64 ? op->tile64 : op->tile32;
+  hipEvent_t e0, e1;
+  IGA_HIP_TRY(hipEventCreate(&e0));
+  IGA_HIP_TRY(hipEventCreate(&e1));
+  auto launch = [&]() {
+    return a.wf ? (f64 ? op->ops->rat64 : op->ops->rat32)(a, ebe, tile, ntiles, s)
+                : (f64 ? op->ops->apply64 : op->ops->apply32)(
+                      a, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s);
+  };
+  hipError_t e = launch();  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int i = 0; i < nlaunch && e == hipSuccess; ++i) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return IGA_ERR_HIP;
+  *avg_ms = (double)ms / nlaunch;
   return IGA_OK;
 }
 

@@ -1,4 +1,4 @@
-// General rational (per-function weight) apply kernels of degree 1.
+// General rational (per-function weight) FP32 apply kernels of degree 1.
 #include "launch.hpp"
 
 IGAK_DEFINE_RAT(1)

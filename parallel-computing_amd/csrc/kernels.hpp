@@ -209,16 +209,18 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
         uy += d2[b] * t0[b][il];
         if (RAT) uv += n2[b] * t0[b][il];
       }
+      // G accessor: point q = j*NQ + i, also given as (j, il) so that a
+      // thread's own columns can come from a statically indexed array
       const int q = j * NQ + i;
       T ax = T(0), ay = T(0);
       if (RAT) {
         // grad(U/W) W = grad U - U a with a = grad W / W (1/W^2 is in G)
-        ax = gl(3, q);
-        ay = gl(4, q);
+        ax = gl(3, q, j, il);
+        ay = gl(4, q, j, il);
         ux -= uv * ax;
         uy -= uv * ay;
       }
-      const T g11 = gl(0, q), g12 = gl(1, q), g22 = gl(2, q);
+      const T g11 = gl(0, q, j, il), g12 = gl(1, q, j, il), g22 = gl(2, q, j, il);
       const T fx = -(g11 * ux + g12 * uy);
       const T fy = -(g12 * ux + g22 * uy);
       const T fv = RAT ? -(ax * fx + ay * fy) : T(0);
@@ -287,12 +289,12 @@ __device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
       const int q = j * NQ + i;
       T ax = T(0), ay = T(0);
       if (RAT) {
-        ax = gl(3, q);
-        ay = gl(4, q);
+        ax = gl(3, q, j, i);
+        ay = gl(4, q, j, i);
         ux -= uv * ax;
         uy -= uv * ay;
       }
-      const T g11 = gl(0, q), g12 = gl(1, q), g22 = gl(2, q);
+      const T g11 = gl(0, q, j, i), g12 = gl(1, q, j, i), g22 = gl(2, q, j, i);
       const T fx = -(g11 * ux + g12 * uy);
       const T fy = -(g12 * ux + g22 * uy);
       const T fv = RAT ? -(ax * fx + ay * fy) : T(0);
@@ -535,14 +537,32 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   TileStage<T, P, NQ, TX, TY, RAT, NT> stg;
   stg.load(A, ex0, ey0, t);
   constexpr int NC = RAT ? 5 : 3;
+  constexpr int NI = (NQ + SPLIT - 1) / SPLIT;
+  // whole element (one thread) or this thread's quadrature columns (split)
   constexpr bool PREF =
       IGAK_PREFETCH_G && MINW < 4 && SPLIT == 1 && NC * NQ2 * sizeof(T) <= 320;
+  constexpr bool PREFH = IGAK_PREFETCH_G && SPLIT > 1 && NC * NQ * NI * sizeof(T) <= 400;
   T g[PREF ? NC : 1][PREF ? NQ2 : 1];
+  T gh[PREFH ? NC : 1][PREFH ? NQ : 1][PREFH ? NI : 1];
   const T *Gt = static_cast<const T *>(A.G);
   if constexpr (PREF) load_gq<T, NQ2, NC>(Gt, tile, el, NE, g);
-  auto gl = [&](int c, int q) {
+  if constexpr (PREFH) {
+    const int i0 = h * NI;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int j = 0; j < NQ; ++j)
+#pragma unroll
+        for (int il = 0; il < NI; ++il)
+          gh[c][j][il] = (i0 + il < NQ)
+                             ? Gt[gq_index<T, NQ2, NC>(tile, c, j * NQ + i0 + il, el, NE)]
+                             : T(0);
+  }
+  auto gl = [&](int c, int q, int j, int il) {
     if constexpr (PREF)
       return g[c][q];
+    else if constexpr (PREFH)
+      return gh[c][j][il];
     else
       return Gt[gq_index<T, NQ2, NC>(tile, c, q, el, NE)];
   };
@@ -557,7 +577,6 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
-    constexpr int NI = (NQ + SPLIT - 1) / SPLIT;
     if (EBE)
       element_ebe<T, P, NQ, TX, TY, RAT>(X, tbx, tby, lx, ly, gl, ye);
     else
@@ -675,7 +694,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
-      auto gl = [&](int c, int q) { return g[c][q]; };
+      auto gl = [&](int c, int q, int, int) { return g[c][q]; };
       element_sf<T, P, NQ, TX, TY>(X, tbx, tby, lx, ly, gl, ye);
       if (!valid) {
 #pragma unroll
@@ -725,31 +744,37 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
 
 // Seam functions (touched by more than one tile).  mode 0: zero them before
 // a scatter-add apply; mode 1: sum the halo slots of the touching tiles in
-// (tile row, tile column) order.  Grid: x along a seam line, y over the
-// lines -- the P columns right of each vertical tile boundary (all rows),
-// then the P rows below each horizontal one (functions not already on a
-// vertical seam).  Tile sizes are powers of two (shifts sx, sy).
+// (tile row, tile column) order.  One thread per seam function, numbered
+// row by row within each band of P columns right of a vertical tile
+// boundary (so a wavefront covers short runs of a row), then the P rows
+// below each horizontal boundary (functions not on a vertical seam).  32-bit
+// indices; tile sizes are powers of two (shifts sx, sy).
 template <typename T, int P>
 __global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex, int ney, int nfx,
                        int nfy, int sx, int sy, int ntx, int nty, int mode, int mask_out,
                        const int *skip) {
   if (skip && *skip) return;
   const int TX = 1 << sx, TY = 1 << sy;
-  const int line = blockIdx.y, pos = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nvl = (ntx - 1) * P;
+  const int nv = (ntx - 1) * P * nfy;
+  const int nh = (nty - 1) * P * nfx;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv + nh) return;
   int gx, gy;
-  if (line < nvl) {
-    gx = (line / P + 1) * TX + line % P;
-    gy = pos;
-    if (gy >= nfy) return;
+  if (i < nv) {
+    const int pn = P * nfy;
+    const int band = i / pn, r = i - band * pn;
+    gy = r / P;
+    gx = (band + 1) * TX + (r - gy * P);
   } else {
-    const int l = line - nvl;
-    gy = (l / P + 1) * TY + l % P;
-    gx = pos;
-    if (gx >= nfx) return;
+    const int j = i - nv;
+    const int pn = P * nfx;
+    const int band = j / pn, r = j - band * pn;
+    const int rr = r / nfx;
+    gy = (band + 1) * TY + rr;
+    gx = r - rr * nfx;
     int xlo, xhi;
     touching_tiles_sh<P>(gx, nex, sx, xlo, xhi);
-    if (xlo != xhi) return;  // owned by the vertical-seam lines
+    if (xlo != xhi) return;  // owned by the vertical-band pass
   }
   const int Ag = gy * nfx + gx;
   if (mode == 0 || (mask_out && is_dirichlet(dirmask, Ag, gx, gy, nfx, nfy))) {
@@ -767,57 +792,6 @@ __global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex, int
       acc += halo[(size_t)(ty * ntx + tx) * FX * FY + k];
     }
   y[Ag] = acc;
-}
-
-// Inverse-LM gather of the seam functions fused with the partial sums of
-// pv . y over all functions (the CG's p.Ap).  Workgroups take function rows
-// (grid-stride over rows, threads along the row: no integer division); tile
-// sizes are powers of two (shifts sx, sy).  Seam functions are summed from
-// the touching tiles' halo slots in (tile row, tile column) order and
-// stored, the others read back.  One partial per workgroup (256 threads,
-// fixed reduction tree) in part[].
-template <typename T, int P>
-__global__ __launch_bounds__(256) void k_seam_dot(T *y, const T *halo, const T *pv,
-                                                  const uint8_t *dirmask, int nex, int ney,
-                                                  int nfx, int nfy, int sx, int sy, int ntx,
-                                                  int mask_out, double *part, const int *skip) {
-  if (skip && *skip) return;
-  const int TX = 1 << sx, TY = 1 << sy, FX = TX + P, FY = TY + P;
-  double s = 0.0;
-  for (int gy = blockIdx.x; gy < nfy; gy += gridDim.x) {
-    int ylo, yhi;
-    touching_tiles_sh<P>(gy, ney, sy, ylo, yhi);
-    const size_t row = (size_t)gy * nfx;
-    for (int gx = threadIdx.x; gx < nfx; gx += 256) {
-      int xlo, xhi;
-      touching_tiles_sh<P>(gx, nex, sx, xlo, xhi);
-      const size_t i = row + gx;
-      T v;
-      if (xlo != xhi || ylo != yhi) {
-        if (mask_out && is_dirichlet(dirmask, (int)i, gx, gy, nfx, nfy)) {
-          v = T(0);
-        } else {
-          T acc = T(0);
-          for (int ty = ylo; ty <= yhi; ++ty)
-            for (int tx = xlo; tx <= xhi; ++tx)
-              acc += halo[(size_t)(ty * ntx + tx) * FX * FY + (gy - ty * TY) * FX +
-                          (gx - tx * TX)];
-          v = acc;
-        }
-        y[i] = v;
-      } else {
-        v = y[i];
-      }
-      s += (double)pv[i] * (double)v;
-    }
-  }
-  // workgroup sum: wavefront shuffles, then four wavefront sums in order
-  __shared__ double ws[4];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 // --------------------------------------------------------------------------

@@ -95,49 +95,32 @@ struct Launch {
   // variant: 0 one workgroup per tile, 1 persistent pipelined (SF only),
   // 2 one workgroup per tile at four waves per SIMD (FP32 SF, 16x16, p <= 4)
   // 3 two threads per element (SF; 16x8 tiles, which the EBE apply of such
-  // an operator also uses)
-  static hipError_t apply(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles,
-                          int variant, hipStream_t s) {
-    if (tile == TILE_16x8) {
-      if (f64) return ebe ? apply_t<double, 16, 8, true>(a, ntiles, s)
-                          : apply_t<double, 16, 8, false, false, 1, 2>(a, ntiles, s);
-      return ebe ? apply_t<float, 16, 8, true>(a, ntiles, s)
-                 : apply_t<float, 16, 8, false, false, 1, 2>(a, ntiles, s);
-    }
-    if constexpr (P <= 4)
-      if (variant == 2 && !f64 && !ebe && tile == TILE_16x16)
+  // an operator also uses).  One precision per call, so that FP32 and FP64
+  // kernels compile in separate translation units.
+  template <typename T>
+  static hipError_t apply_prec(const ApplyArgs &a, bool ebe, int tile, int ntiles, int variant,
+                               hipStream_t s) {
+    if (tile == TILE_16x8)
+      return ebe ? apply_t<T, 16, 8, true>(a, ntiles, s)
+                 : apply_t<T, 16, 8, false, false, 1, 2>(a, ntiles, s);
+    if constexpr (P <= 4 && sizeof(T) == 4)
+      if (variant == 2 && !ebe && tile == TILE_16x16)
         return apply_t<float, 16, 16, false, false, 4>(a, ntiles, s);
     if (variant == 2) variant = 0;  // EBE applies of a variant-2 operator
-    if (variant == 1 && !ebe)
-      return f64 ? apply_pf_tile<double>(a, tile, ntiles, s)
-                 : apply_pf_tile<float>(a, tile, ntiles, s);
-    if (f64) return ebe ? apply_tile<double, true>(a, tile, ntiles, s)
-                        : apply_tile<double, false>(a, tile, ntiles, s);
-    return ebe ? apply_tile<float, true>(a, tile, ntiles, s)
-               : apply_tile<float, false>(a, tile, ntiles, s);
+    if (variant == 1 && !ebe) return apply_pf_tile<T>(a, tile, ntiles, s);
+    return ebe ? apply_tile<T, true>(a, tile, ntiles, s) : apply_tile<T, false>(a, tile, ntiles, s);
   }
 
   template <typename T>
   static hipError_t seam(T *y, const T *halo, const uint8_t *mask, int nex, int ney, int nfx,
                          int nfy, const Layout &L, int mode, int mask_out, const int *skip,
                          hipStream_t s) {
-    const int lines = (L.ntx - 1) * P + (L.nty - 1) * P;
-    if (lines == 0) return hipSuccess;
-    const int bs = 256, len = nfx > nfy ? nfx : nfy;
+    const int n = (L.ntx - 1) * P * nfy + (L.nty - 1) * P * nfx;
+    if (n == 0) return hipSuccess;
+    const int bs = 256;
     const int sx = __builtin_ctz((unsigned)L.tx), sy = __builtin_ctz((unsigned)L.ty);
-    hipLaunchKernelGGL((k_seam<T, P>), dim3((len + bs - 1) / bs, lines), dim3(bs), 0, s, y, halo,
-                       mask, nex, ney, nfx, nfy, sx, sy, L.ntx, L.nty, mode, mask_out, skip);
-    return hipGetLastError();
-  }
-
-  template <typename T>
-  static hipError_t seam_dot(T *y, const T *halo, const T *pv, const uint8_t *mask, int nex,
-                             int ney, int nfx, int nfy, const Layout &L, int mask_out,
-                             double *part, int grid, const int *skip, hipStream_t s) {
-    // tile sizes are powers of two
-    const int sx = __builtin_ctz((unsigned)L.tx), sy = __builtin_ctz((unsigned)L.ty);
-    hipLaunchKernelGGL((k_seam_dot<T, P>), dim3(grid), dim3(256), 0, s, y, halo, pv, mask, nex,
-                       ney, nfx, nfy, sx, sy, L.ntx, mask_out, part, skip);
+    hipLaunchKernelGGL((k_seam<T, P>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, y, halo, mask,
+                       nex, ney, nfx, nfy, sx, sy, L.ntx, L.nty, mode, mask_out, skip);
     return hipGetLastError();
   }
 
@@ -216,12 +199,10 @@ struct LaunchRat {
                                       : run_t<T, 16, 8, false, 2>(a, ntiles, s);
     return hipErrorInvalidValue;
   }
-  static hipError_t apply(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles,
-                          hipStream_t s) {
-    if (f64) return ebe ? run_tile<double, true>(a, tile, ntiles, s)
-                        : run_tile<double, false>(a, tile, ntiles, s);
-    return ebe ? run_tile<float, true>(a, tile, ntiles, s)
-               : run_tile<float, false>(a, tile, ntiles, s);
+  template <typename T>
+  static hipError_t apply_prec(const ApplyArgs &a, bool ebe, int tile, int ntiles,
+                               hipStream_t s) {
+    return ebe ? run_tile<T, true>(a, tile, ntiles, s) : run_tile<T, false>(a, tile, ntiles, s);
   }
 };
 
@@ -229,8 +210,10 @@ struct BlockArgs;
 
 // per-degree entry points (kern_p<P>.hip, kern_p<P>_rat.hip, kern_p<P>_block.hip)
 struct Ops {
-  hipError_t (*apply)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles, int variant,
-                      hipStream_t);
+  hipError_t (*apply64)(const ApplyArgs &, bool ebe, int tile, int ntiles, int variant,
+                        hipStream_t);
+  hipError_t (*apply32)(const ApplyArgs &, bool ebe, int tile, int ntiles, int variant,
+                        hipStream_t);
   hipError_t (*seam64)(double *, const double *, const uint8_t *, int, int, int, int,
                        const Layout &, int, int, const int *, hipStream_t);
   hipError_t (*seam32)(float *, const float *, const uint8_t *, int, int, int, int,
@@ -249,22 +232,18 @@ struct Ops {
   hipError_t (*kappa_grad)(double *, const double *, const double *, const double *,
                            const double *, const double *, const double *, const uint8_t *, int,
                            int, int, int, const Layout &, hipStream_t);
-  hipError_t (*seam_dot64)(double *, const double *, const double *, const uint8_t *, int, int,
-                           int, int, const Layout &, int, double *, int, const int *,
-                           hipStream_t);
-  hipError_t (*seam_dot32)(float *, const float *, const float *, const uint8_t *, int, int, int,
-                           int, const Layout &, int, double *, int, const int *, hipStream_t);
   hipError_t (*qp_coords)(double *, const double *, const double *, const double *,
                           const double *, int, int, int, hipStream_t);
-  hipError_t (*apply_rat)(const ApplyArgs &, bool f64, bool ebe, int tile, int ntiles,
-                          hipStream_t);
+  hipError_t (*rat64)(const ApplyArgs &, bool ebe, int tile, int ntiles, hipStream_t);
+  hipError_t (*rat32)(const ApplyArgs &, bool ebe, int tile, int ntiles, hipStream_t);
   hipError_t (*block)(const BlockArgs &, int cfg, hipStream_t);
 };
 
 template <int P>
 Ops make_ops() {
   Ops o;
-  o.apply = &Launch<P>::apply;
+  o.apply32 = &Launch<P>::template apply_prec<float>;
+  o.apply64 = nullptr;
   o.seam64 = &Launch<P>::template seam<double>;
   o.seam32 = &Launch<P>::template seam<float>;
   o.geom64 = &Launch<P>::template geometry<double>;
@@ -273,9 +252,7 @@ Ops make_ops() {
   o.diag = &Launch<P>::diag;
   o.kappa_grad = &Launch<P>::kappa_grad;
   o.qp_coords = &Launch<P>::qp_coords;
-  o.seam_dot64 = &Launch<P>::template seam_dot<double>;
-  o.seam_dot32 = &Launch<P>::template seam_dot<float>;
-  o.apply_rat = nullptr;
+  o.rat64 = o.rat32 = nullptr;
   o.block = nullptr;
   return o;
 }
@@ -287,12 +264,16 @@ const Ops *ops_for_degree(int p);
 
 #define IGAK_DEFINE_DEGREE(P)                                                             \
   namespace igak {                                                                        \
-  hipError_t apply_rat_p##P(const ApplyArgs &, bool, bool, int, int, hipStream_t);        \
+  hipError_t apply64_p##P(const ApplyArgs &, bool, int, int, int, hipStream_t);           \
+  hipError_t rat64_p##P(const ApplyArgs &, bool, int, int, hipStream_t);                  \
+  hipError_t rat32_p##P(const ApplyArgs &, bool, int, int, hipStream_t);                  \
   hipError_t block_p##P(const BlockArgs &, int, hipStream_t);                             \
   const Ops *ops_p##P() {                                                                 \
     static const Ops o = [] {                                                             \
       Ops r = make_ops<P>();                                                              \
-      r.apply_rat = &apply_rat_p##P;                                                      \
+      r.apply64 = &apply64_p##P;                                                          \
+      r.rat64 = &rat64_p##P;                                                              \
+      r.rat32 = &rat32_p##P;                                                              \
       r.block = &block_p##P;                                                              \
       return r;                                                                           \
     }();                                                                                  \
@@ -300,10 +281,25 @@ const Ops *ops_for_degree(int p);
   }                                                                                       \
   }
 
+// FP64 apply kernels of degree P (kern_p<P>_f64.hip)
+#define IGAK_DEFINE_F64(P)                                                                \
+  namespace igak {                                                                        \
+  hipError_t apply64_p##P(const ApplyArgs &a, bool ebe, int tile, int ntiles, int variant, \
+                          hipStream_t s) {                                                \
+    return Launch<P>::template apply_prec<double>(a, ebe, tile, ntiles, variant, s);      \
+  }                                                                                       \
+  }
+
+// general rational apply kernels of degree P, one precision per file
 #define IGAK_DEFINE_RAT(P)                                                                \
   namespace igak {                                                                        \
-  hipError_t apply_rat_p##P(const ApplyArgs &a, bool f64, bool ebe, int tile, int ntiles, \
-                            hipStream_t s) {                                              \
-    return LaunchRat<P>::apply(a, f64, ebe, tile, ntiles, s);                             \
+  hipError_t rat32_p##P(const ApplyArgs &a, bool ebe, int tile, int ntiles, hipStream_t s) { \
+    return LaunchRat<P>::template apply_prec<float>(a, ebe, tile, ntiles, s);             \
+  }                                                                                       \
+  }
+#define IGAK_DEFINE_RAT64(P)                                                              \
+  namespace igak {                                                                        \
+  hipError_t rat64_p##P(const ApplyArgs &a, bool ebe, int tile, int ntiles, hipStream_t s) { \
+    return LaunchRat<P>::template apply_prec<double>(a, ebe, tile, ntiles, s);            \
   }                                                                                       \
   }

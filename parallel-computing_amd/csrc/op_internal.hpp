@@ -63,12 +63,11 @@ namespace igai {
 int dev_alloc(iga_op *op, void **p, size_t bytes);
 int set_device(int dev);
 // y = A x with an explicit layout / precision (used by the solver)
+// y_zeroed: y is already zero (scatter-add form then skips its seam-zeroing
+// kernel)
 int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
-              int mask_in, int mask_out, hipStream_t s, const int *skip);
-// y = A x in inverse-LM gather form with the seam gather fused with the
-// per-workgroup partials of pv . y (grid workgroups, partials in part[])
-int apply_gather_dot(iga_op *op, bool f64, const void *x, void *y, const void *pv,
-                     double *part, int grid, hipStream_t s, const int *skip);
+              int mask_in, int mask_out, hipStream_t s, const int *skip,
+              bool y_zeroed = false);
 // Y = A X for k FP32 columns (leading dimension ld), the configuration
 // tuned or defaulted for k; no device switch (callable under graph capture)
 int apply_block_raw(iga_op *op, int k, int ld, const float *X, float *Y, hipStream_t s);

@@ -284,13 +284,17 @@ __global__ void k_cg_rz(const V *r, const V *z, const V *rold, int64_t n, double
 }
 
 // K3: convergence test, beta, p = z + beta p
+// With Apz, also zero Ap for the next scatter-add apply (IEN atomic form),
+// which then needs no seam-zeroing kernel; Ap is dead once this kernel runs.
+// When the iteration stops, Ap is left as it is: cg_run zeroes it on entry.
 template <typename V>
 __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t n,
                              const double *part, double *sc, int *ic, int nb, int flexible,
-                             const int *outer_done, int64_t ld) {
+                             const int *outer_done, int64_t ld, V *__restrict__ Apz) {
   CG_COLUMN(ld)
   p += voff_;
   z += voff_;
+  if (Apz) Apz += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
   const double rr = fold(part + P_RR * RG, nb);
   const double rz = fold(part + P_RZ * RG, nb);
@@ -331,7 +335,10 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t i = b0 + (int64_t)k * RB;
-      if (i < n) p[i] = (V)((double)zv[k] + b * (double)pv[k]);
+      if (i < n) {
+        p[i] = (V)((double)zv[k] + b * (double)pv[k]);
+        if (Apz) Apz[i] = V(0);
+      }
     }
   }
 }
@@ -469,6 +476,57 @@ __global__ void k_csr_diag_inv(const int *rowptr, const int *col, const double *
   d[i] = (v != 0.0) ? 1.0 / v : 1.0;
 }
 
+// Chebyshev-Jacobi preconditioner z = p_m(D^-1 K) D^-1 r on the Q1 matrix
+// (Chebyshev semi-iteration on [lmin, lmax] from z = 0; the coefficients
+// are host constants, so the preconditioner is linear and capturable).
+template <typename V>
+__global__ void k_cheb_init(const V *__restrict__ r, V *__restrict__ z, V *__restrict__ d,
+                            const double *__restrict__ dinv, double c, int64_t n,
+                            const int *skip) {
+  if (skip && *skip) return;
+  GRID_LOOP(i, n) {
+    const V v = (V)(c * (double)r[i] * dinv[i]);
+    d[i] = v;
+    z[i] = v;
+  }
+}
+
+template <typename V>
+__global__ void k_cheb_step(const V *__restrict__ r, const V *__restrict__ q, V *__restrict__ z,
+                            V *__restrict__ d, const double *__restrict__ dinv, double c1,
+                            double c2, int64_t n, const int *skip) {
+  if (skip && *skip) return;
+  GRID_LOOP(i, n) {
+    const double res = dinv[i] * ((double)r[i] - (double)q[i]);
+    const V dn = (V)(c1 * (double)d[i] + c2 * res);
+    d[i] = dn;
+    z[i] = (V)((double)z[i] + (double)dn);
+  }
+}
+
+// per-workgroup maximum over rows of sum_j |K_ij| / K_ii (Gershgorin bound
+// of D^-1 K)
+__global__ void k_gersh(const int *rowptr, const int *col, const double *val, int nrows,
+                        double *part) {
+  double m = 0.0;
+  for (int i = blockIdx.x * RB + threadIdx.x; i < nrows; i += gridDim.x * RB) {
+    double s = 0.0, dg = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      s += fabs(val[k]);
+      if (col[k] == i) dg = fabs(val[k]);
+    }
+    if (dg > 0.0) m = fmax(m, s / dg);
+  }
+  __shared__ double sm[RB];
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = RB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
 template <typename V>
 __global__ void k_cast(const double *a, V *b, int64_t n) {
   GRID_LOOP(i, n) b[i] = (V)a[i];
@@ -547,6 +605,7 @@ struct iga_solver {
   double *fdinv = nullptr;
   double *rin = nullptr, *w = nullptr, *pp = nullptr, *q = nullptr;
   float *rin32 = nullptr, *w32 = nullptr, *pp32 = nullptr, *q32 = nullptr;
+  double cheb_lmax = 0.0;  // Gershgorin bound of D^-1 K (FEM_CHEB)
   std::vector<void *> allocs;
   int gen = -1;  // operator generation the Jacobi diagonal was built for
   // graphs of one chunk of iterations (FP64 PCG, FP32 inner PCG)
@@ -602,13 +661,42 @@ int fem_pc(iga_solver *s, const V *r, V *z, V *rin, V *w, V *pp, V *q, const V *
     LAUNCH((k_cg_pap<V>), nb, pp, q, n, s->part_in, sc, ic, od, (int64_t)0);
     LAUNCH((k_cg_update<V, double, true, false>), nb, z, rin, pp, q, w, (V *)nullptr, s->fdinv,
            1, n, s->part_in, sc, ic, nb, od, (int64_t)0);
-    LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od, (int64_t)0);
+    LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od, (int64_t)0,
+           (V *)nullptr);
+  }
+  return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+}
+
+// z = p_m(D^-1 K) D^-1 r: degree pc_cheb_degree, interval [lmax / 30, lmax]
+// (Saad, Iterative Methods for Sparse Linear Systems, Alg. 12.1)
+template <typename V>
+int cheb_pc(iga_solver *s, const V *r, V *z, V *d, V *q, const V *val, hipStream_t st) {
+  const int *od = s->ic + I_DONE;
+  const int64_t n = s->n;
+  const int nb = s->nb;
+  const double lmax = s->cheb_lmax, lmin = lmax / 30.0;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  LAUNCH((k_cheb_init<V>), nb, r, z, d, s->fdinv, 1.0 / theta, n, od);
+  const int bs = 256;
+  for (int k = 1; k < s->o.pc_cheb_degree; ++k) {
+    hipLaunchKernelGGL((k_spmv<V>), dim3((s->fem_rows + bs - 1) / bs), dim3(bs), 0, st, s->rowptr,
+                       s->col, val, z, q, s->fem_rows, od, (const int *)nullptr);
+    const double rho1 = 1.0 / (2.0 * sigma - rho);
+    LAUNCH((k_cheb_step<V>), nb, r, q, z, d, s->fdinv, rho1 * rho, 2.0 * rho1 / delta, n, od);
+    rho = rho1;
   }
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
 template <typename V>
 int apply_pc(iga_solver *s, bool f64, const V *r, V *z, hipStream_t st) {
+  if (s->o.pc == IGA_PC_FEM_CHEB) {
+    if (f64)
+      return cheb_pc<double>(s, (const double *)r, (double *)z, s->pp, s->q, s->val64, st);
+    return cheb_pc<float>(s, (const float *)r, (float *)z, s->pp32, s->q32, s->val32, st);
+  }
   if (f64)
     return fem_pc<double>(s, (const double *)r, (double *)z, s->rin, s->w, s->pp, s->q, s->val64,
                           st);
@@ -625,16 +713,11 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
   const bool loc = local_pc(s);
-  if (gather && !ebe) {
-    // seam gather fused with the p.Ap partials
-    int rc = igai::apply_gather_dot(s->op, f64, p, Ap, p, s->part + P_PAP * RG, nb, st,
-                                    s->ic + I_DONE);
-    if (rc) return rc;
-  } else {
-    int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE);
-    if (rc) return rc;
-    LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr, (int64_t)0);
-  }
+  // atomic form: Ap was zeroed by the previous k_cg_pupdate (or cg_run)
+  int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE,
+                           /*y_zeroed=*/!gather);
+  if (rc) return rc;
+  LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr, (int64_t)0);
   const V *dinv = s->o.pc != IGA_PC_JACOBI
                       ? nullptr
                       : (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32);
@@ -645,12 +728,12 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
     LAUNCH((k_cg_update<V, V, false, true>), nb, u, r, p, Ap, z, rold, (const V *)nullptr, 0, n,
            s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0);
   if (!loc) {
-    int rc = apply_pc<V>(s, f64, r, z, st);
+    rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
     LAUNCH((k_cg_rz<V>), nb, r, z, rold, n, s->part, s->ic);
   }
   LAUNCH((k_cg_pupdate<V>), nb, p, z, n, s->part, s->sc, s->ic, nb, loc ? 0 : 1,
-         (const int *)nullptr, (int64_t)0);
+         (const int *)nullptr, (int64_t)0, gather ? (V *)nullptr : Ap);
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
@@ -675,6 +758,11 @@ int cg_run(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipS
     LAUNCH((k_cg_rz0<V>), s->nb, r, z, p, n, s->part, s->ic);
     hipLaunchKernelGGL(k_cg_rz0_fin, dim3(1), dim3(RB), 0, st, s->part, s->nb, s->sc, s->ic);
   }
+  // scatter-add form: the first apply adds into a zeroed Ap, later ones into
+  // the Ap that k_cg_pupdate zeroed
+  if (s->o.assembly != IGA_ASSEMBLY_INVLM_GATHER &&
+      hipMemsetAsync(Ap, 0, (size_t)n * sizeof(V), st) != hipSuccess)
+    return IGA_ERR_HIP;
   if (s->o.use_graph && !*gexec) {
     // capture one chunk; the graph reads every scalar from device memory
     hipStream_t cs;
@@ -806,7 +894,8 @@ int bcg_chunk(iga_solver *s, int k, hipStream_t st) {
                        s->br32, s->bp32, s->bAp32, s->bz32, (float *)nullptr, dinv, 0, n, s->bpart,
                        s->bsc, s->bic, nb, (const int *)nullptr, n);
     hipLaunchKernelGGL((k_cg_pupdate<float>), grid, dim3(RB), 0, st, s->bp32, s->bz32, n,
-                       s->bpart, s->bsc, s->bic, nb, 0, (const int *)nullptr, n);
+                       s->bpart, s->bsc, s->bic, nb, 0, (const int *)nullptr, n,
+                       (float *)nullptr);
   }
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
@@ -858,7 +947,7 @@ void iga_solver_opts_default(iga_solver_opts *o) {
   o->abstol = 0.0;
   o->divtol = 1e4;
   o->maxit = 100000;
-  o->inner_rtol = 1e-1;
+  o->inner_rtol = 0.5;
   o->inner_maxit = 100000;
   o->pc_inner_maxit = 20;
   o->pc_inner_rtol = 1e-2;
@@ -885,9 +974,12 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   iga_solver_opts o;
   if (o_in) o = *o_in; else iga_solver_opts_default(&o);
   if (o.kind != IGA_SOLVE_PCG_F64 && o.kind != IGA_SOLVE_PCG_F32_REFINE) return IGA_ERR_INVALID_ARGUMENT;
-  if (o.pc != IGA_PC_NONE && o.pc != IGA_PC_JACOBI && o.pc != IGA_PC_FEM_CG)
-    return IGA_ERR_UNSUPPORTED;
-  if (o.pc == IGA_PC_FEM_CG && !fem) return IGA_ERR_INVALID_ARGUMENT;
+  if (o.pc != IGA_PC_NONE && o.pc != IGA_PC_JACOBI && o.pc != IGA_PC_FEM_CG &&
+      o.pc != IGA_PC_FEM_CHEB)
+    return IGA_ERR_INVALID_ARGUMENT;
+  const bool fem_pc_kind = o.pc == IGA_PC_FEM_CG || o.pc == IGA_PC_FEM_CHEB;
+  if (fem_pc_kind && !fem) return IGA_ERR_INVALID_ARGUMENT;
+  if (o.pc == IGA_PC_FEM_CHEB && o.pc_cheb_degree < 1) return IGA_ERR_INVALID_ARGUMENT;
   if (o.kind == IGA_SOLVE_PCG_F32_REFINE && !op->G32) return IGA_ERR_UNSUPPORTED;
   if (!(o.rtol >= 0) || !(o.abstol >= 0) || o.maxit < 1 || !(o.inner_rtol > 0) ||
       o.inner_maxit < 1 || (o.pc == IGA_PC_FEM_CG && o.pc_inner_maxit < 1))
@@ -934,7 +1026,7 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   }
   if (hipMemset(s->ic, 0, 32 * 4) != hipSuccess || hipMemset(s->sc, 0, 64 * 8) != hipSuccess)
     return fail(IGA_ERR_HIP);
-  if (o.pc == IGA_PC_FEM_CG) {
+  if (fem_pc_kind) {
     iga_csr_view v;
     if (iga_fem_view(fem, &v) != IGA_OK) return fail(IGA_ERR_INVALID_ARGUMENT);
     if (v.nrows != s->n) return fail(IGA_ERR_BAD_MESH);
@@ -958,6 +1050,15 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
       SA(s->q32, n * 4);
       hipLaunchKernelGGL((k_cast<float>), dim3(nblocks(v.nnz)), dim3(RB), 0, nullptr, v.val,
                          s->val32, (int64_t)v.nnz);
+    }
+    if (o.pc == IGA_PC_FEM_CHEB) {
+      std::vector<double> hp(RG);
+      hipLaunchKernelGGL(k_gersh, dim3(RG), dim3(RB), 0, nullptr, v.rowptr, v.col, v.val,
+                         v.nrows, s->part);
+      if (hipMemcpy(hp.data(), s->part, RG * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(IGA_ERR_HIP);
+      for (double x : hp) s->cheb_lmax = std::max(s->cheb_lmax, x);
+      if (!(s->cheb_lmax > 0.0)) return fail(IGA_ERR_BAD_MESH);
     }
   }
 #undef SA

@@ -110,6 +110,8 @@ def _lib():
         L.iga_op_load.argtypes = [VP, C.c_int, C.c_double, VP, VP]
         L.iga_op_profile.argtypes = [VP, C.c_int, C.c_int]
         L.iga_op_profile_read.argtypes = [VP, C.POINTER(C.c_int)] + [C.POINTER(C.c_double)] * 3
+        L.iga_op_time_kernel.argtypes = [VP, C.c_int, C.c_int, C.c_int, VP, VP, C.c_int, VP,
+                                         C.POINTER(C.c_double)]
         L.iga_op_qp_coords.argtypes = [VP, VP, VP]
         L.iga_op_set_kappa.argtypes = [VP, VP, VP]
         L.iga_op_apply_backward.argtypes = [VP, VP, VP, VP, VP, VP]
@@ -309,6 +311,16 @@ class Operator:
         check(_lib().iga_op_load(self._h, kind, value, F.data_ptr(), _stream_ptr(stream)),
               "op_load")
         return F
+
+    def time_kernel(self, x, y, nlaunch=50, kernel=SF, assembly=INVLM_GATHER, stream=None):
+        """Average ms of the element kernel alone over nlaunch back-to-back
+        launches (y receives partial results)."""
+        import torch
+        dt = {torch.float64: F64, torch.float32: F32}[x.dtype]
+        ms = C.c_double()
+        check(_lib().iga_op_time_kernel(self._h, dt, kernel, assembly, x.data_ptr(), y.data_ptr(),
+                                        nlaunch, _stream_ptr(stream), C.byref(ms)), "time_kernel")
+        return ms.value
 
     def qp_coords(self, stream=None):
         """Physical quadrature-point coordinates, float64 CUDA [E, nqp, 2]."""

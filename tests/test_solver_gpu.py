@@ -198,3 +198,29 @@ def test_block_solve_matches_single_solves():
     with pytest.raises(igamf.IgaError) as e:
         s64.solve_block(F)
     assert e.value.status == 6
+
+
+def test_fem_chebyshev_preconditioned_solve():
+    """The Chebyshev-Jacobi polynomial of the Q1 matrix as a fixed
+    preconditioner: converged solutions equal the Jacobi-PCG ones and graph
+    replays are bitwise equal; degree < 1 is rejected.  (How many iterations
+    it saves depends on how well the parameter-grid Q1 matrix matches the
+    geometry; on the quarter annulus Jacobi of A is the better of the two.)"""
+    p, n = 3, 64
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, n)
+    op = igamf.Operator(m)
+    fem = igamf.FemPC(n + p, n + p, igamf.fem_reference_nodes(p, p, n, n))
+    mask = torch.from_numpy(m.arrays()["ID"] >= 0).cuda()
+    F = torch.from_numpy(igamf.rng_normal(17, op.nfunc)).cuda() * mask
+    uj, stj = igamf.Solver(op, None, pc=igamf.PC_JACOBI, rtol=1e-10).solve(F)
+    for kind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        s = igamf.Solver(op, fem, kind=kind, pc=igamf.PC_FEM_CHEB, rtol=1e-10, pc_cheb_degree=8)
+        u, st = s.solve(F)
+        assert st.status == 0 and st.rel_residual <= 1.0001e-10, (kind, st.rel_residual)
+        assert (u - uj).abs().max().item() < 1e-7 * uj.abs().max().item(), kind
+        print("chebyshev-FEM iterations", kind, st.iterations, "jacobi", stj.iterations)
+        u2, _ = s.solve(F)  # graph replay
+        assert torch.equal(u, u2)
+    with pytest.raises(igamf.IgaError) as e:
+        igamf.Solver(op, fem, pc=igamf.PC_FEM_CHEB, pc_cheb_degree=0)
+    assert e.value.status == 1
