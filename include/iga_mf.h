@@ -177,6 +177,15 @@ int iga_op_get_info(const iga_op *op, iga_op_info *info);
  * they use the gather assembly (the op owns its halo buffer). */
 int iga_op_apply(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb,
                  const void *x, void *y, void *stream);
+/* y = A x as iga_op_apply, and the energy product x.y = x^T P_int A x (the
+ * Dirichlet rows of y are zero) accumulated inside the same kernel, written
+ * to *xy (one double, device memory) on `stream`.  Replaces the MatMult +
+ * VecDot pair (p, Ap) of the reference's KSPCG iteration around MyMatMult
+ * (matrix_free_gpu/matrixfreegpu.cu:52-66, 264-301).  The per-tile partials
+ * live in the op: calls on one op must not overlap.  IGA_ERR_UNSUPPORTED for
+ * apply variant 1. */
+int iga_op_apply_dot(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb,
+                     const void *x, void *y, double *xy, void *stream);
 
 /* Average duration of the element kernel alone (k_apply or its variant,
  * without the seam kernel), from HIP events around nlaunch back-to-back
@@ -302,6 +311,16 @@ typedef struct {
   double pc_inner_rtol;         /* FEM_CG: inner CG tolerance */
   int pc_cheb_degree;           /* FEM_CHEB: polynomial degree */
   int use_graph;                /* capture the iteration body in a HIP graph */
+  int fused;                    /* 0: the p.Ap product by its own kernel
+                                   after the apply;
+                                   1 (default): p.Ap accumulated inside the
+                                   apply kernel (same recurrences);
+                                   2: single-reduction CG for Jacobi / no
+                                   preconditioner (Chronopoulos-Gear
+                                   recurrences, p.Ap formed from u.Au and
+                                   u.s, u.Au inside the apply): the apply plus
+                                   one vector kernel per iteration.
+                                   Apply variant 1 cannot fuse: 0 is used. */
 } iga_solver_opts;
 
 typedef struct {

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
   }
   __syncthreads();
 
-  ApplyArgs oa;  // output addressing shared with the single-vector store
+  ApplyArgs oa = {};  // output addressing shared with the single-vector store
   oa.dirmask = A.dirmask;
   oa.nex = A.nex;
   oa.ney = A.ney;
@@ -113,8 +113,10 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
           for (int a = 0; a < NB; ++a) ye[b][a] = 0.f;
       }
     }
+    double dnone = 0.0;
     reduce_store_row16<float, P, TY, false>(oa, ye, red[r & 1], t, 0, ex0, ey0,
-                                            A.Y + (size_t)(j0 + r) * A.ld, nullptr, nullptr);
+                                            A.Y + (size_t)(j0 + r) * A.ld, nullptr, nullptr,
+                                            xs, dnone);
   }
 }
 

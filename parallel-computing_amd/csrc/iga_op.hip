@@ -155,10 +155,29 @@ int dev_alloc(iga_op *op, void **p, size_t bytes) {
   return IGA_OK;
 }
 
-int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
-              int mask_in, int mask_out, hipStream_t s, const int *skip, bool y_zeroed) {
+bool cg_fusable(const iga_op *op, bool f64, bool ebe) {
+  // the persistent variant (1, sum-factorised, no per-function weights) has
+  // no dot output
+  return op->rational == 2 || ebe || (f64 ? op->variant64 : op->variant32) != 1;
+}
+
+int apply_tiles(const iga_op *op, bool f64) {
   const igak::Layout &L = f64 ? op->L64 : op->L32;
-  igak::ApplyArgs a;
+  return L.ntx * L.nty;
+}
+
+int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
+              int mask_in, int mask_out, hipStream_t s, const int *skip, bool y_zeroed,
+              const CgHook *hook) {
+  const igak::Layout &L = f64 ? op->L64 : op->L32;
+  if (hook && !cg_fusable(op, f64, ebe)) return IGA_ERR_UNSUPPORTED;
+  igak::ApplyArgs a = {};
+  if (hook) {
+    a.dot = hook->dot;
+    a.hsrc = hook->hsrc;
+    a.hdst = hook->hdst;
+    a.hn = hook->hn;
+  }
   a.x = x;
   a.y = y;
   a.G = f64 ? (const void *)op->G64 : (const void *)op->G32;
@@ -493,6 +512,7 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
       return fail(IGA_ERR_HIP);
     ALLOC(op->halo32, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 4);
   }
+  ALLOC(op->tdot, (size_t)std::max(op->L64.ntx * op->L64.nty, op->L32.ntx * op->L32.nty) * 8);
   if (hipDeviceSynchronize() != hipSuccess) return fail(IGA_ERR_HIP);
 #undef ALLOC
 #undef H2D
@@ -535,6 +555,45 @@ int iga_op_apply(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb, c
   return igai::apply_raw(op, dt == IGA_F64, kern == IGA_KERNEL_EBE,
                          asmb == IGA_ASSEMBLY_INVLM_GATHER, x, y, 0, 1, (hipStream_t)stream,
                          nullptr);
+}
+
+namespace {
+
+// one workgroup: out = sum of the n per-tile partials (fixed order)
+__global__ void k_fold_tiles(const double *part, int n, double *out) {
+  __shared__ double sm[256];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += part[i];
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sm[threadIdx.x] += sm[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sm[0];
+}
+
+}  // namespace
+
+int iga_op_apply_dot(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb,
+                     const void *x, void *y, double *xy, void *stream) {
+  if (!op || !x || !y || !xy || x == y) return IGA_ERR_INVALID_ARGUMENT;
+  if ((dt != IGA_F64 && dt != IGA_F32) || (kern != IGA_KERNEL_SF && kern != IGA_KERNEL_EBE) ||
+      (asmb != IGA_ASSEMBLY_IEN_ATOMIC && asmb != IGA_ASSEMBLY_INVLM_GATHER))
+    return IGA_ERR_INVALID_ARGUMENT;
+  const bool f64 = dt == IGA_F64, ebe = kern == IGA_KERNEL_EBE;
+  if (!f64 && !op->G32) return IGA_ERR_UNSUPPORTED;
+  if (!igai::cg_fusable(op, f64, ebe)) return IGA_ERR_UNSUPPORTED;
+  int rc = igai::set_device(op->device);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const igai::CgHook hook{op->tdot, nullptr, nullptr, 0};
+  rc = igai::apply_raw(op, f64, ebe, asmb == IGA_ASSEMBLY_INVLM_GATHER, x, y, 0, 1, s, nullptr,
+                       false, &hook);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fold_tiles, dim3(1), dim3(256), 0, s, op->tdot,
+                     igai::apply_tiles(op, f64), xy);
+  return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
 namespace {
@@ -670,7 +729,7 @@ int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, vo
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const igak::Layout &L = f64 ? op->L64 : op->L32;
-  igak::ApplyArgs a;
+  igak::ApplyArgs a = {};
   a.x = x;
   a.y = y;
   a.G = f64 ? (const void *)op->G64 : (const void *)op->G32;

@@ -49,6 +49,12 @@ struct ApplyArgs {
   int mask_in;            // treat x as P_int x
   int mask_out;           // zero Dirichlet rows of y
   const int *skip;        // device flag: nonzero = do nothing (converged solve)
+  // CG fusion (all null / 0 for a plain apply):
+  double *dot;            // [tile]: sum over the tile's non-Dirichlet footprint of
+                          // x_f * (its contribution to y_f), so sum(dot) = x . y
+  const double *hsrc;     // workgroup 0 copies hn doubles hsrc -> hdst (hand-off of
+  double *hdst;           // the CG state between the update kernels around this one)
+  int hn;
 };
 
 __device__ __forceinline__ bool is_dirichlet(const uint8_t *mask, int A, int gx,
@@ -404,9 +410,12 @@ struct TileStage {
 // atomically added (mode 0, after k_seam zeroed them) or written to the
 // tile's halo slot fy*FX+fx (mode 1, summed by k_seam).  General rational
 // meshes scale by the function's weight.
+// xin: the staged input at this footprint point (x, or w x for general
+// weights), accumulated into dacc as xin * v when A.dot is set.
 template <typename T, int P, int TX, int TY, bool RAT>
 __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo, const T *wf,
-                                              int tile, int ex0, int ey0, int fx, int fy, T v) {
+                                              int tile, int ex0, int ey0, int fx, int fy, T v,
+                                              T xin, double &dacc) {
   constexpr int FX = TX + P;
   const int gx = ex0 + fx, gy = ey0 + fy;
   if (gx >= A.nfx || gy >= A.nfy) return;
@@ -419,6 +428,7 @@ __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo,
     if (!shared || A.mode == 0) y[Ag] = T(0);
     return;
   }
+  if (A.dot) dacc += (double)xin * (double)v;
   if (RAT) v *= wf[Ag];
   if (!shared)
     y[Ag] = v;
@@ -465,7 +475,9 @@ __device__ __forceinline__ void row16_x(const T (&ye)[P + 1][P + 1], T (&rin)[P 
 template <typename T, int P, int TY, bool RAT>
 __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (&ye)[P + 1][P + 1],
                                                    T *lds, int t, int tile, int ex0, int ey0,
-                                                   T *y, T *halo, const T *wf) {
+                                                   T *y, T *halo, const T *wf, const T *xs,
+                                                   double &dacc) {
+  constexpr int FX = 16 + P;
   constexpr int NB = P + 1, NT = 16 * TY;
   const int lx = t & 15, ly = t >> 4;
   T rin[NB], rwr[NB];
@@ -495,13 +507,59 @@ __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (
       if (lx < P) w1 += bwr[b * NT + (TY + ly - b) * 16 + lx];
     }
   }
-  put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, ly, v0);
-  if (lx < P) put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, ly, w0);
+  auto xin = [&](int fx, int fy) { return A.dot ? xs[fy * FX + fx] : T(0); };
+  put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, ly, v0, xin(lx, ly),
+                                   dacc);
+  if (lx < P)
+    put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, ly, w0,
+                                     xin(16 + lx, ly), dacc);
   if (ly < P) {
-    put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, TY + ly, v1);
+    put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, TY + ly, v1,
+                                     xin(lx, TY + ly), dacc);
     if (lx < P)
-      put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, TY + ly, w1);
+      put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, TY + ly, w1,
+                                       xin(16 + lx, TY + ly), dacc);
   }
+}
+
+// Sum over the 64 lanes of a wave, on DPP moves (no LDS round trips): pairs
+// (quad_perm 1,0,3,2), quads (2,3,0,1), 8 lanes (row_half_mirror), 16 lanes
+// (row_mirror), then the four rows in order.  Fixed order: reproducible.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_row<0xB1>(v);
+  v += dpp_row<0x4E>(v);
+  v += dpp_row<0x141>(v);
+  v += dpp_row<0x140>(v);
+  auto lane = [&](int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+  };
+  return ((lane(0) + lane(16)) + lane(32)) + lane(48);
+}
+
+// Workgroup sum of the per-thread x.y contributions into A.dot[tile]
+// (fixed reduction and wave order: bitwise reproducible).
+template <int NT>
+__device__ __forceinline__ void store_tile_dot(const ApplyArgs &A, int tile, double v) {
+  static_assert(NT % 64 == 0, "whole waves");
+  __shared__ double wsum[NT / 64];
+  v = wave_sum_dpp(v);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) s += wsum[i];
+    A.dot[tile] = s;
+  }
+}
+
+// CG state hand-off: workgroup 0 copies A.hn doubles (see ApplyArgs)
+__device__ __forceinline__ void cg_handoff(const ApplyArgs &A, int nt) {
+  if (A.hn && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < A.hn; i += nt) A.hdst[i] = A.hsrc[i];
 }
 
 // --------------------------------------------------------------------------
@@ -526,6 +584,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   T(*buf)[NE] = reinterpret_cast<T(*)[NE]>(bufm);
 
   if (A.skip && *A.skip) return;
+  cg_handoff(A, NT);
   const int t = threadIdx.x, h = t % SPLIT, el = t / SPLIT, lx = el % TX, ly = el / TX;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % A.ntx, ty = tile / A.ntx;
@@ -598,8 +657,10 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
 
   T *y = static_cast<T *>(A.y);
   T *halo = static_cast<T *>(A.halo);
+  double dacc = 0.0;
   if constexpr (ROW16) {
-    reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, tile, ex0, ey0, y, halo, wf);
+    reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, tile, ex0, ey0, y, halo, wf, xs, dacc);
+    if (A.dot) store_tile_dot<NT>(A, tile, dacc);
     return;
   }
   // fixed-order reduction over the footprint: pass b adds element row offset b
@@ -635,8 +696,10 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   for (int o = 0; o < NOWN; ++o) {
     const int k = t + o * NT;
     if (k < NFP)
-      put_footprint<T, P, TX, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, k % FX, k / FX, acc[o]);
+      put_footprint<T, P, TX, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, k % FX, k / FX, acc[o],
+                                       A.dot ? xs[k] : T(0), dacc);
   }
+  if (A.dot) store_tile_dot<NT>(A, tile, dacc);
 }
 
 // Persistent, software-pipelined form of k_apply (sum-factorised): a
@@ -703,8 +766,10 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
           for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
       }
     }
+    double dnone = 0.0;  // A.dot is null here (no CG fusion in this variant)
     if constexpr (TX == 16) {
-      reduce_store_row16<T, P, TY, false>(A, ye, bufm, t, tile, ex0, ey0, y, halo, nullptr);
+      reduce_store_row16<T, P, TY, false>(A, ye, bufm, t, tile, ex0, ey0, y, halo, nullptr, xs,
+                                          dnone);
       continue;
     }
     T acc[NOWN];
@@ -737,7 +802,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
       const int k = t + o * NT;
       if (k < NFP)
         put_footprint<T, P, TX, TY, false>(A, y, halo, nullptr, tile, ex0, ey0, k % FX, k / FX,
-                                           acc[o]);
+                                           acc[o], T(0), dnone);
     }
   }
 }

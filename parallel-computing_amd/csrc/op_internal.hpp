@@ -44,6 +44,7 @@ struct iga_op {
   double *halo64 = nullptr;
   float *halo32 = nullptr;
   double *fJ = nullptr;             // load-vector scratch [E][nqp]
+  double *tdot = nullptr;           // per-tile x.y partials (iga_op_apply_dot)
   bool want64 = true, want32 = true;
   int gen = 0;                      // bumped whenever G_q changes (set_kappa)
   int64_t bytes = 0;
@@ -62,12 +63,24 @@ struct iga_op {
 namespace igai {
 int dev_alloc(iga_op *op, void **p, size_t bytes);
 int set_device(int dev);
-// y = A x with an explicit layout / precision (used by the solver)
-// y_zeroed: y is already zero (scatter-add form then skips its seam-zeroing
-// kernel)
+// CG fusion of an apply (see igak::ApplyArgs): per-tile partials of x.y into
+// dot[ntiles], and the hand-off copy of hn doubles hsrc -> hdst
+struct CgHook {
+  double *dot;
+  const double *hsrc;
+  double *hdst;
+  int hn;
+};
+// whether the apply kernel configured for this precision supports CgHook
+bool cg_fusable(const iga_op *op, bool f64, bool ebe);
+// tiles of the apply layout of this precision (entries of CgHook::dot)
+int apply_tiles(const iga_op *op, bool f64);
+
+// y = A x with an explicit precision (used by the solver).  y_zeroed: y is
+// already zero (the scatter-add form then skips its seam-zeroing kernel).
 int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
               int mask_in, int mask_out, hipStream_t s, const int *skip,
-              bool y_zeroed = false);
+              bool y_zeroed = false, const CgHook *hook = nullptr);
 // Y = A X for k FP32 columns (leading dimension ld), the configuration
 // tuned or defaulted for k; no device switch (callable under graph capture)
 int apply_block_raw(iga_op *op, int k, int ld, const float *X, float *Y, hipStream_t s);

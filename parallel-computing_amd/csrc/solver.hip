@@ -99,6 +99,20 @@ __device__ __forceinline__ double fold(const double *slot, int nb) {
   return block_sum_all(s);
 }
 
+// This thread's share of the sum of n partials, eight loads in flight at a
+// time (a serial chain of dependent loads would expose the L2 latency once
+// per partial); fixed order.
+__device__ __forceinline__ double fold_share(const double *__restrict__ a, int n) {
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 7 * RB < n; i += 8 * RB) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] += a[i + k * RB];
+  }
+  for (; i < n; i += RB) acc[0] += a[i];
+  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
 __device__ __forceinline__ bool finite(double x) { return x == x && !isinf(x); }
 
 #define GRID_LOOP(i, n) \
@@ -196,7 +210,8 @@ template <typename V, typename D, bool LOCAL, bool ROLD>
 __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__restrict__ p,
                             const V *__restrict__ Ap, V *__restrict__ z, V *__restrict__ rold,
                             const D *__restrict__ dinv, int prec_norm, int64_t n, double *part,
-                            double *sc, int *ic, int nb, const int *outer_done, int64_t ld) {
+                            double *sc, int *ic, int nb, const int *outer_done, int64_t ld,
+                            const double *__restrict__ tdot, int ntiles) {
   CG_COLUMN(ld)
   u += voff_;
   r += voff_;
@@ -205,7 +220,8 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
   z += voff_;
   if (ROLD) rold += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
-  const double pap = fold(part + P_PAP * RG, nb);
+  // p.Ap: partials of k_cg_pap, or the apply kernel's per-tile partials
+  const double pap = tdot ? block_sum_all(fold_share(tdot, ntiles)) : fold(part + P_PAP * RG, nb);
   const double a = sc[S_RZ] / pap;
   if (!finite(a) || pap == 0.0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ic[I_DONE] = 3;
@@ -409,6 +425,245 @@ __global__ void k_ru_fin(const double *part, int nb, double *sc, int *ic, double
   }
 }
 
+// ---------------------------------------------------------------- fused iteration
+// Single-reduction PCG for local preconditioners (Chronopoulos & Gear, 1989),
+// with p.Ap formed explicitly from products that need no second reduction:
+//   w = A u                      (apply; u.w = sum of its per-tile partials)
+//   beta = g / g_old,  g = r.u
+//   p.Ap = u.w + 2 beta u.s_old + beta^2 (p.Ap)_old     (A symmetric, s = A p)
+//   alpha = g / p.Ap
+//   p = u + beta p;  s = w + beta s;  x += alpha p;  r -= alpha s;  u = M r
+// and the partials of the next r.u, r.r and u.s in the same pass.  An
+// iteration is the apply plus one vector kernel.  The update kernel reads the
+// state it also rewrites, so the state is double-buffered: the kernel reads
+// CUR and writes NEW; the apply kernel between two updates copies NEW to CUR
+// (its workgroup 0, a kernel no update kernel overlaps).
+// State layout (doubles), CUR at 0 and NEW at CG_L:
+enum { CG_RZO = 0, CG_PAP, CG_IT, CG_RR = 4, CG_RZ = CG_RR + RG, CG_US = CG_RZ + RG,
+       CG_L = CG_US + RG };
+
+// Sum over the workgroup of N values at once (fixed order), to every thread.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N]) {
+  __shared__ double sm[N][RB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) sm[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < RB / 64; ++i) s += sm[k][i];
+    v[k] = s;
+  }
+}
+
+// init: x = 0, r = f*fscale, u = M r, p = s = 0; NEW partials of r.r, r.u
+// and u.s (= 0)
+template <typename V, typename F, typename D>
+__global__ void k_cgf_init(const F *__restrict__ f, double fscale, V *__restrict__ x,
+                           V *__restrict__ r, V *__restrict__ u, V *__restrict__ p,
+                           V *__restrict__ sv, const D *__restrict__ dinv, int64_t n,
+                           double *cgs) {
+  double v[3] = {0.0, 0.0, 0.0};
+  GRID_LOOP(i, n) {
+    const V ri = (V)((double)f[i] * fscale);
+    const V ui = dinv ? (V)((double)ri * (double)dinv[i]) : ri;
+    x[i] = V(0);
+    r[i] = ri;
+    u[i] = ui;
+    p[i] = V(0);
+    sv[i] = V(0);
+    v[0] += (double)ri * (double)ri;
+    v[1] += (double)ri * (double)ui;
+  }
+  block_sum_n<3>(v);
+  if (threadIdx.x == 0) {
+    double *nw = cgs + CG_L;
+    nw[CG_RR + blockIdx.x] = v[0];
+    nw[CG_RZ + blockIdx.x] = v[1];
+    nw[CG_US + blockIdx.x] = v[2];
+  }
+}
+
+// one workgroup: tolerances (as k_cg_init_fin) and the NEW scalars of a
+// fresh recursion (g_old = inf makes the first beta 0)
+__global__ void k_cgf_init_fin(double *cgs, int nb, double *sc, int *ic, double rtol2,
+                               double abstol2, double divtol2, int maxit, double upd2) {
+  double *nw = cgs + CG_L;
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += RB) {
+    v[0] += nw[CG_RR + i];
+    v[1] += nw[CG_RZ + i];
+  }
+  block_sum_n<2>(v);
+  if (threadIdx.x == 0) {
+    const double rr = v[0];
+    sc[S_RR0] = rr;
+    sc[S_RR] = rr;
+    sc[S_RZ] = v[1];
+    sc[S_TOL2] = fmax(rtol2 * rr, abstol2);
+    sc[S_DIV2] = divtol2 * rr;
+    sc[S_UPD2] = upd2 * rr;
+    nw[CG_RZO] = HUGE_VAL;
+    nw[CG_PAP] = 0.0;
+    nw[CG_IT] = 0.0;
+    ic[I_ITER] = 0;
+    ic[I_NEXT] = 0;
+    ic[I_MAXIT] = maxit;
+    ic[I_DONE] = !finite(rr) ? 3 : (rr <= sc[S_TOL2] ? 1 : 0);
+  }
+}
+
+// The vector kernel of a fused iteration (see above).  w = A u comes from the
+// apply, the partials of u.w from its tiles (tdot).  Stops (done flag) on
+// convergence, the iteration cap, divergence or breakdown, and -- FP32 inner
+// solve -- when r.r fell below the reliable-update threshold; the state is
+// then that of the last completed iteration.  zero_w: clear w for the next
+// scatter-add apply.
+template <typename V, typename D, bool JAC>
+__global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
+                             V *__restrict__ sv, V *__restrict__ u, V *__restrict__ w,
+                             const D *__restrict__ dinv, int zero_w, int64_t n, double *cgs,
+                             const double *__restrict__ tdot, int ntiles, double *sc, int *ic,
+                             int nb) {
+  if (ic[I_DONE]) return;
+  const double *cur = cgs;
+  double *nw = cgs + CG_L;
+  double f[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += RB) {
+    f[0] += cur[CG_RR + i];
+    f[1] += cur[CG_RZ + i];
+    f[2] += cur[CG_US + i];
+  }
+  f[3] = fold_share(tdot, ntiles);
+  block_sum_n<4>(f);
+  const double rr = f[0], g = f[1], us = f[2], uw = f[3];
+  const int it = (int)cur[CG_IT];
+  int st = 0;
+  if (rr <= sc[S_TOL2]) st = 1;
+  else if (!finite(rr) || rr > sc[S_DIV2]) st = 3;
+  else if (it >= ic[I_MAXIT]) st = 2;
+  else if (rr <= sc[S_UPD2]) st = 4;
+  const double b = g / cur[CG_RZO];
+  const double pap = uw + 2.0 * b * us + b * b * cur[CG_PAP];
+  const double a = g / pap;
+  if (!st && (!finite(a) || pap == 0.0)) st = 3;
+  if (st) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[S_RR] = rr;
+      ic[I_ITER] = it;
+      ic[I_DONE] = st;
+    }
+    return;
+  }
+  constexpr int U = 4;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x; b0 < n;
+       b0 += (int64_t)gridDim.x * RB * U) {
+    V uv[U], wv[U], pv[U], sw[U], xv[U], rv[U];
+    D dv[U];
+    if (b0 + (int64_t)(U - 1) * RB < n) {  // whole step in range: every load issued at once
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b0 + (int64_t)k * RB;
+        uv[k] = u[i];
+        wv[k] = w[i];
+        pv[k] = p[i];
+        sw[k] = sv[i];
+        xv[k] = x[i];
+        rv[k] = r[i];
+        dv[k] = JAC ? dinv[i] : D(1);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b0 + (int64_t)k * RB;
+        const bool in = i < n;
+        uv[k] = in ? u[i] : V(0);
+        wv[k] = in ? w[i] : V(0);
+        pv[k] = in ? p[i] : V(0);
+        sw[k] = in ? sv[i] : V(0);
+        xv[k] = in ? x[i] : V(0);
+        rv[k] = in ? r[i] : V(0);
+        dv[k] = (JAC && in) ? dinv[i] : D(1);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t i = b0 + (int64_t)k * RB;
+      if (i >= n) continue;
+      const V pn = (V)((double)uv[k] + b * (double)pv[k]);
+      const V sn = (V)((double)wv[k] + b * (double)sw[k]);
+      const V rn = (V)((double)rv[k] - a * (double)sn);
+      const V un = JAC ? (V)((double)rn * (double)dv[k]) : rn;
+      p[i] = pn;
+      sv[i] = sn;
+      x[i] = (V)((double)xv[k] + a * (double)pn);
+      r[i] = rn;
+      u[i] = un;
+      if (zero_w) w[i] = V(0);
+      v[0] += (double)rn * (double)rn;
+      v[1] += (double)rn * (double)un;
+      v[2] += (double)un * (double)sn;
+    }
+  }
+  block_sum_n<3>(v);
+  if (threadIdx.x == 0) {
+    nw[CG_RR + blockIdx.x] = v[0];
+    nw[CG_RZ + blockIdx.x] = v[1];
+    nw[CG_US + blockIdx.x] = v[2];
+    if (blockIdx.x == 0) {
+      nw[CG_RZO] = g;
+      nw[CG_PAP] = pap;
+      nw[CG_IT] = (double)(it + 1);
+      sc[S_RR] = rr;
+      sc[S_RZ] = g;
+      sc[S_ALPHA] = a;
+      sc[S_BETA] = b;
+      ic[I_ITER] = it + 1;
+    }
+  }
+}
+
+// reliable update of the fused FP32 recursion: r32 = r64 (true residual),
+// u32 = M r32, NEW partials of r.r, r.u and u.s (p, s and the NEW scalars of
+// the last completed iteration are kept)
+__global__ void k_cgf_ru_reset(const double *__restrict__ r64, float *__restrict__ r32,
+                               float *__restrict__ u32, const float *__restrict__ s32,
+                               const float *__restrict__ dinv, int64_t n, double *cgs) {
+  double v[3] = {0.0, 0.0, 0.0};
+  GRID_LOOP(i, n) {
+    const float ri = (float)r64[i];
+    const float ui = dinv ? (float)((double)ri * (double)dinv[i]) : ri;
+    r32[i] = ri;
+    u32[i] = ui;
+    v[0] += (double)ri * (double)ri;
+    v[1] += (double)ri * (double)ui;
+    v[2] += (double)ui * (double)s32[i];
+  }
+  block_sum_n<3>(v);
+  if (threadIdx.x == 0) {
+    double *nw = cgs + CG_L;
+    nw[CG_RR + blockIdx.x] = v[0];
+    nw[CG_RZ + blockIdx.x] = v[1];
+    nw[CG_US + blockIdx.x] = v[2];
+  }
+}
+
+__global__ void k_cgf_ru_fin(double *sc, int *ic, double upd2) {
+  if (threadIdx.x == 0) {
+    sc[S_UPD2] = upd2;
+    ic[I_DONE] = 0;
+  }
+}
+
 // ---------------------------------------------------------------- outer (FP64)
 // r = f - Au; partials |r|^2 and |f|^2
 __global__ void k_residual(const double *f, const double *Au, double *r, int64_t n,
@@ -431,11 +686,6 @@ __global__ void k_fold2(const double *part, int nb, double *out) {
     out[0] = a;
     out[1] = b;
   }
-}
-
-// u += scale * d (FP32 correction into the FP64 iterate)
-__global__ void k_axpy_f32(double *u, const float *d, double scale, int64_t n) {
-  GRID_LOOP(i, n) u[i] += scale * (double)d[i];
 }
 
 __global__ void k_inv(double *d, int64_t n) {
@@ -610,6 +860,11 @@ struct iga_solver {
   int gen = -1;  // operator generation the Jacobi diagonal was built for
   // graphs of one chunk of iterations (FP64 PCG, FP32 inner PCG)
   hipGraphExec_t g64 = nullptr, g32 = nullptr;
+  // fused iteration: double-buffered recursion state, per-tile u.Au partials,
+  // graphs of one chunk
+  double *cgs = nullptr, *tdot = nullptr;
+  int ntiles64 = 0, ntiles32 = 0;
+  hipGraphExec_t gf64 = nullptr, gf32 = nullptr;
   // block solve (k columns of nfunc): FP64 iterates, FP32 inner vectors,
   // per-column scalars / flags / partial slots, graph of one chunk
   int bk = 0;
@@ -660,7 +915,7 @@ int fem_pc(iga_solver *s, const V *r, V *z, V *rin, V *w, V *pp, V *q, const V *
                        s->col, val, pp, q, s->fem_rows, ic + I_DONE, od);
     LAUNCH((k_cg_pap<V>), nb, pp, q, n, s->part_in, sc, ic, od, (int64_t)0);
     LAUNCH((k_cg_update<V, double, true, false>), nb, z, rin, pp, q, w, (V *)nullptr, s->fdinv,
-           1, n, s->part_in, sc, ic, nb, od, (int64_t)0);
+           1, n, s->part_in, sc, ic, nb, od, (int64_t)0, (const double *)nullptr, 0);
     LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od, (int64_t)0,
            (V *)nullptr);
   }
@@ -713,20 +968,27 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
   const bool loc = local_pc(s);
+  // p.Ap accumulated by the apply kernel itself when it can, else k_cg_pap
+  const bool fdot = s->o.fused == 1 && s->tdot && igai::cg_fusable(s->op, f64, ebe);
+  const igai::CgHook hook{s->tdot, nullptr, nullptr, 0};
+  const double *tdot = fdot ? s->tdot : nullptr;
+  const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
   // atomic form: Ap was zeroed by the previous k_cg_pupdate (or cg_run)
   int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE,
-                           /*y_zeroed=*/!gather);
+                           /*y_zeroed=*/!gather, fdot ? &hook : nullptr);
   if (rc) return rc;
-  LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr, (int64_t)0);
+  if (!fdot)
+    LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr,
+           (int64_t)0);
   const V *dinv = s->o.pc != IGA_PC_JACOBI
                       ? nullptr
                       : (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32);
   if (loc)
     LAUNCH((k_cg_update<V, V, true, false>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n, s->part,
-           s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0);
+           s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles);
   else
     LAUNCH((k_cg_update<V, V, false, true>), nb, u, r, p, Ap, z, rold, (const V *)nullptr, 0, n,
-           s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0);
+           s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles);
   if (!loc) {
     rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
@@ -823,6 +1085,89 @@ int cg_resume(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, h
   return IGA_OK;
 }
 
+// ---- fused iteration (local preconditioners, single solves)
+bool use_fused(const iga_solver *s, bool f64) {
+  return s->o.fused == 2 && local_pc(s) && s->cgs &&
+         igai::cg_fusable(s->op, f64, s->o.kernel == IGA_KERNEL_EBE);
+}
+
+// one fused iteration: apply (+ seam kernel when gathering) and k_cgf_update;
+// u, w, p, sv: the recursion's M r, A u, direction and A p
+template <typename V>
+int cgf_iteration(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w,
+                  hipStream_t st) {
+  const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
+  const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
+  igai::CgHook hook{s->tdot, s->cgs + CG_L, s->cgs, CG_L};
+  // scatter-add form: w was cleared by the previous update (or cgf_run)
+  int rc = igai::apply_raw(s->op, f64, ebe, gather, u, w, 0, 1, st, s->ic + I_DONE,
+                           /*y_zeroed=*/!gather, &hook);
+  if (rc) return rc;
+  const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
+  if (s->o.pc == IGA_PC_JACOBI)
+    LAUNCH((k_cgf_update<V, V, true>), s->nb, x, r, p, sv, u, w,
+           f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32,
+           gather ? 0 : 1, s->n, s->cgs, s->tdot, ntiles, s->sc, s->ic, s->nb);
+  else
+    LAUNCH((k_cgf_update<V, V, false>), s->nb, x, r, p, sv, u, w, (const V *)nullptr,
+           gather ? 0 : 1, s->n, s->cgs, s->tdot, ntiles, s->sc, s->ic, s->nb);
+  return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+}
+
+template <typename V>
+int cgf_chunk(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st) {
+  for (int k = 0; k < CHUNK; ++k) {
+    int rc = cgf_iteration<V>(s, f64, x, r, p, sv, u, w, st);
+    if (rc) return rc;
+  }
+  return IGA_OK;
+}
+
+// Run (or, after k_cgf_ru_reset + k_cgf_ru_fin, continue) a fused recursion
+// until its done flag is set; the init kernels must already be enqueued.
+template <typename V>
+int cgf_run(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st,
+            hipGraphExec_t *gexec, int *iters, int *done) {
+  if (s->o.assembly != IGA_ASSEMBLY_INVLM_GATHER &&
+      hipMemsetAsync(w, 0, (size_t)s->n * sizeof(V), st) != hipSuccess)
+    return IGA_ERR_HIP;
+  if (s->o.use_graph && !*gexec) {
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return IGA_ERR_HIP;
+    hipGraph_t g;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return IGA_ERR_HIP;
+    }
+    int rc = cgf_chunk<V>(s, f64, x, r, p, sv, u, w, cs);
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
+    e = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      *gexec = nullptr;
+      return IGA_ERR_HIP;
+    }
+  }
+  int hic[2];
+  for (;;) {
+    if (s->o.use_graph) {
+      if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
+    } else {
+      int rc = cgf_chunk<V>(s, f64, x, r, p, sv, u, w, st);
+      if (rc) return rc;
+    }
+    if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return IGA_ERR_HIP;
+    if (hic[I_DONE]) break;
+  }
+  *iters = hic[I_ITER];
+  *done = hic[I_DONE];
+  return IGA_OK;
+}
+
 // |f - A u|^2 and |f|^2 into h (host), r = f - A u
 int outer_residual(iga_solver *s, const double *f, const double *u, double *r, double *h,
                    hipStream_t st) {
@@ -892,7 +1237,7 @@ int bcg_chunk(iga_solver *s, int k, hipStream_t st) {
                        s->bsc, s->bic, (const int *)nullptr, n);
     hipLaunchKernelGGL((k_cg_update<float, float, true, false>), grid, dim3(RB), 0, st, s->bd32,
                        s->br32, s->bp32, s->bAp32, s->bz32, (float *)nullptr, dinv, 0, n, s->bpart,
-                       s->bsc, s->bic, nb, (const int *)nullptr, n);
+                       s->bsc, s->bic, nb, (const int *)nullptr, n, (const double *)nullptr, 0);
     hipLaunchKernelGGL((k_cg_pupdate<float>), grid, dim3(RB), 0, st, s->bp32, s->bz32, n,
                        s->bpart, s->bsc, s->bic, nb, 0, (const int *)nullptr, n,
                        (float *)nullptr);
@@ -953,6 +1298,7 @@ void iga_solver_opts_default(iga_solver_opts *o) {
   o->pc_inner_rtol = 1e-2;
   o->pc_cheb_degree = 8;
   o->use_graph = 1;
+  o->fused = 1;
 }
 
 int iga_solver_destroy(iga_solver *s) {
@@ -960,6 +1306,8 @@ int iga_solver_destroy(iga_solver *s) {
   (void)hipSetDevice(s->device);
   if (s->g64) (void)hipGraphExecDestroy(s->g64);
   if (s->g32) (void)hipGraphExecDestroy(s->g32);
+  if (s->gf64) (void)hipGraphExecDestroy(s->gf64);
+  if (s->gf32) (void)hipGraphExecDestroy(s->gf32);
   if (s->gblk) (void)hipGraphExecDestroy(s->gblk);
   for (void *p : s->allocs) (void)hipFree(p);
   for (void *p : s->ballocs) (void)hipFree(p);
@@ -1023,6 +1371,13 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     SA(s->Ap32, n * 4);
     SA(s->z32, n * 4);
     SA(s->rold32, n * 4);
+  }
+  if (o.fused < 0 || o.fused > 2) return fail(IGA_ERR_INVALID_ARGUMENT);
+  if (o.fused) {
+    s->ntiles64 = igai::apply_tiles(op, true);
+    s->ntiles32 = igai::apply_tiles(op, false);
+    SA(s->tdot, (size_t)std::max(s->ntiles64, s->ntiles32) * 8);
+    if (o.fused == 2 && !fem_pc_kind) SA(s->cgs, (size_t)2 * CG_L * 8);
   }
   if (hipMemset(s->ic, 0, 32 * 4) != hipSuccess || hipMemset(s->sc, 0, 64 * 8) != hipSuccess)
     return fail(IGA_ERR_HIP);
@@ -1095,14 +1450,24 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
   const auto t0 = std::chrono::steady_clock::now();
   int status = IGA_OK;
   if (o.kind == IGA_SOLVE_PCG_F64) {
-    LAUNCH((k_cg_init<double, double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, dinv,
-           loc ? 1 : 0,
-           n, s->part);
-    hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
-                       o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit, 0.0);
     int iters = 0, done = 0;
-    rc = cg_run<double>(s, true, s->x64, s->r, s->p, s->Ap, s->z, s->rold, st, &s->g64, &iters,
-                        &done);
+    if (use_fused(s, true)) {
+      // rold serves as s = A p of the fused recursion
+      LAUNCH((k_cgf_init<double, double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, s->rold,
+             dinv, n, s->cgs);
+      hipLaunchKernelGGL(k_cgf_init_fin, dim3(1), dim3(RB), 0, st, s->cgs, nb, s->sc, s->ic,
+                         o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit, 0.0);
+      rc = cgf_run<double>(s, true, s->x64, s->r, s->p, s->rold, s->z, s->Ap, st, &s->gf64,
+                           &iters, &done);
+    } else {
+      LAUNCH((k_cg_init<double, double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, dinv,
+             loc ? 1 : 0, n, s->part);
+      hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
+                         o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit,
+                         0.0);
+      rc = cg_run<double>(s, true, s->x64, s->r, s->p, s->Ap, s->z, s->rold, st, &s->g64, &iters,
+                          &done);
+    }
     if (rc) return rc;
     if (hipMemcpyAsync(u, s->x64, n * 8, hipMemcpyDeviceToDevice, st) != hipSuccess)
       return IGA_ERR_HIP;
@@ -1130,6 +1495,29 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
       status = IGA_ERR_DIVERGED;
     } else if (rr <= tol2) {
       status = IGA_OK;
+    } else if (use_fused(s, false)) {
+      // rold32 serves as s = A p of the fused recursion
+      LAUNCH((k_cgf_init<float, double, float>), nb, s->r, 1.0, s->d32, s->r32, s->z32, s->p32,
+             s->rold32, dinv32, n, s->cgs);
+      hipLaunchKernelGGL(k_cgf_init_fin, dim3(1), dim3(RB), 0, st, s->cgs, nb, s->sc, s->ic,
+                         tol2 / rr, 0.0, o.divtol * o.divtol * ff / rr, o.maxit, RU2);
+      for (;;) {
+        rc = cgf_run<float>(s, false, s->d32, s->r32, s->p32, s->rold32, s->z32, s->Ap32, st,
+                            &s->gf32, &iters, &done);
+        if (rc) return rc;
+        if (done == 3) { status = IGA_ERR_DIVERGED; break; }
+        // reliable update
+        ++outer;
+        LAUNCH(k_ru_accum, nb, s->x64, s->d32, n);
+        rc = outer_residual(s, f, s->x64, s->r, h, st);
+        if (rc) return rc;
+        rr = h[0];
+        if (!(rr == rr) || rr > o.divtol * o.divtol * ff) { status = IGA_ERR_DIVERGED; break; }
+        if (rr <= tol2) { status = IGA_OK; break; }
+        if (done == 2) break;
+        LAUNCH(k_cgf_ru_reset, nb, s->r, s->r32, s->z32, s->rold32, dinv32, n, s->cgs);
+        hipLaunchKernelGGL(k_cgf_ru_fin, dim3(1), dim3(64), 0, st, s->sc, s->ic, RU2 * rr);
+      }
     } else {
       LAUNCH((k_cg_init<float, double, float>), nb, s->r, 1.0, s->d32, s->r32, s->z32, s->p32,
              dinv32,

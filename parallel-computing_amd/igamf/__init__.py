@@ -69,7 +69,7 @@ class SolverOpts(C.Structure):
                 ("rtol", C.c_double), ("abstol", C.c_double), ("divtol", C.c_double),
                 ("maxit", C.c_int), ("inner_rtol", C.c_double), ("inner_maxit", C.c_int),
                 ("pc_inner_maxit", C.c_int), ("pc_inner_rtol", C.c_double),
-                ("pc_cheb_degree", C.c_int), ("use_graph", C.c_int)]
+                ("pc_cheb_degree", C.c_int), ("use_graph", C.c_int), ("fused", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -107,6 +107,7 @@ def _lib():
         L.iga_op_destroy.argtypes = [VP]
         L.iga_op_get_info.argtypes = [VP, C.POINTER(OpInfo)]
         L.iga_op_apply.argtypes = [VP, C.c_int, C.c_int, C.c_int, VP, VP, VP]
+        L.iga_op_apply_dot.argtypes = [VP, C.c_int, C.c_int, C.c_int, VP, VP, VP, VP]
         L.iga_op_load.argtypes = [VP, C.c_int, C.c_double, VP, VP]
         L.iga_op_profile.argtypes = [VP, C.c_int, C.c_int]
         L.iga_op_profile_read.argtypes = [VP, C.POINTER(C.c_int)] + [C.POINTER(C.c_double)] * 3
@@ -294,6 +295,19 @@ class Operator:
         return y
 
     mult = apply  # MATOP_MULT
+
+    def apply_dot(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None):
+        """y = A x and x.y (float64 device scalar) from one kernel
+        (iga_op_apply_dot)."""
+        import torch
+        dt = {torch.float64: F64, torch.float32: F32}[x.dtype]
+        assert x.is_cuda and x.is_contiguous() and x.numel() == self.nfunc
+        if y is None:
+            y = torch.empty_like(x)
+        xy = torch.empty((), dtype=torch.float64, device=x.device)
+        check(_lib().iga_op_apply_dot(self._h, dt, kernel, assembly, x.data_ptr(), y.data_ptr(),
+                                      xy.data_ptr(), _stream_ptr(stream)), "op_apply_dot")
+        return y, xy
 
     def profile(self, enable, max_calls=1024):
         check(_lib().iga_op_profile(self._h, int(enable), max_calls), "op_profile")

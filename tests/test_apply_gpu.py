@@ -80,6 +80,51 @@ def test_apply_matches_oracle(built, name, kw):
     assert np.all(y64[om.Dir] == 0.0)
 
 
+DOT_CASES = [c for c in CASES if c[0] in ("square_p3_ragged", "annulus_p3_n24", "square_p1_n8",
+                                           "square_p6_n9", "rational_p3_ragged",
+                                           "rational_p5_n12")]
+
+
+@pytest.mark.parametrize("name,kw", DOT_CASES, ids=[c[0] for c in DOT_CASES])
+def test_apply_dot_matches_oracle(built, name, kw):
+    """iga_op_apply_dot: the energy product x.Ax accumulated inside the apply
+    kernel equals x . y_oracle (Dirichlet rows of y are zero), for the default
+    and split-element kernels, both assemblies, EBE and SF; y equals the plain
+    apply's (bitwise for the gather assembly)."""
+    m, om, op0 = built(name, kw)
+    x = igamf.rng_normal(5, op0.nfunc)  # Dirichlet entries nonzero too
+    y_ref = O.apply_mf(om, x)
+    xy_ref = float(x @ y_ref)
+    xd = torch.from_numpy(x).cuda()
+    ops = [op0]
+    if kw["p"] >= 3:
+        ops.append(igamf.Operator(m, variant_f64=3, variant_f32=3))
+    for op in ops:
+        for kern in (igamf.SF, igamf.EBE):
+            for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
+                for xin, tol in ((xd, 1e-12), (xd.float(), 2e-6)):
+                    y, xy = op.apply_dot(xin, kernel=kern, assembly=asm)
+                    y0 = op.apply(xin, kernel=kern, assembly=asm)
+                    if asm == igamf.INVLM_GATHER:
+                        assert torch.equal(y, y0)
+                    e = abs(xy.item() - xy_ref) / abs(xy_ref)
+                    assert e < tol, (name, op.info.variant_f32, kern, asm, xin.dtype, e)
+                    # against the product of the returned vectors themselves
+                    e2 = abs(xy.item() - float(xin.double() @ y.double())) / abs(xy_ref)
+                    assert e2 < tol, (name, kern, asm, xin.dtype, e2)
+
+
+def test_apply_dot_unsupported_variant():
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 40)
+    op = igamf.Operator(m, variant_f64=1, variant_f32=1)
+    x = torch.ones(op.nfunc, dtype=torch.float64, device="cuda")
+    with pytest.raises(igamf.IgaError) as e:
+        op.apply_dot(x)
+    assert e.value.status == 6  # UNSUPPORTED
+    y, xy = op.apply_dot(x, kernel=igamf.EBE)  # the EBE kernel has no variant 1
+    assert abs(xy.item() - float(x @ y)) <= 1e-12 * float(x.abs() @ y.abs())
+
+
 @pytest.mark.parametrize("tile", [(32, 16), (16, 16), (64, 8), (32, 32)])
 def test_pipelined_variant_and_tiles(tile):
     """Persistent pipelined SF kernel, the four-wave FP32 kernel and every

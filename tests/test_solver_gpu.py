@@ -114,15 +114,59 @@ def test_graph_replay_equals_eager():
     m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 40)
     op = igamf.Operator(m)
     F = op.load(igamf.LOAD_CONST, 1.0)
-    for skind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+    for skind, fused in [(k, f) for k in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE)
+                         for f in (0, 1, 2)]:
         outs = []
         for g in (0, 1, 1):
-            s = igamf.Solver(op, None, kind=skind, pc=igamf.PC_JACOBI, rtol=1e-9, use_graph=g)
+            s = igamf.Solver(op, None, kind=skind, pc=igamf.PC_JACOBI, rtol=1e-9, use_graph=g,
+                             fused=fused)
             u, st = s.solve(F)
             outs.append((u.clone(), st.iterations))
             u2, st2 = s.solve(F)  # replay the same solver
             assert torch.equal(u, u2)
         assert all(torch.equal(outs[0][0], o[0]) and outs[0][1] == o[1] for o in outs[1:])
+
+
+@pytest.mark.parametrize("geom", ["annulus", "rational"])
+def test_fused_iteration_matches_classic(geom):
+    """p.Ap accumulated inside the apply (fused=1, the default) and the
+    single-reduction iteration with u.Au inside the apply (fused=2) against
+    the iteration with a separate p.Ap kernel (fused=0):
+    same converged solution, iteration counts within 5 % (FP64) / 12 % (FP32
+    inner recursion), for both solve kinds, both assemblies and both local
+    preconditioners; an operator whose apply variant cannot fuse falls back
+    to the separate kernel."""
+    if geom == "rational":
+        m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, 3, 33, 27, perturb=0.1, w_lo=0.8,
+                             w_hi=1.2, seed=11)
+    else:
+        m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 45, 38)
+    op = igamf.Operator(m)
+    F = op.load(igamf.LOAD_CONST, 1.0)
+    for skind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        for pc in (igamf.PC_JACOBI, igamf.PC_NONE):
+            for asm in (igamf.INVLM_GATHER, igamf.IEN_ATOMIC):
+                res = {}
+                for fused in (0, 1, 2):
+                    s = igamf.Solver(op, None, kind=skind, pc=pc, rtol=1e-10, assembly=asm,
+                                     fused=fused)
+                    u, st = s.solve(F)
+                    assert st.status == 0 and st.rel_residual <= 1.0001e-10, (fused, st)
+                    res[fused] = (u, st.iterations)
+                u0, i0 = res[0]
+                for fused in (1, 2):
+                    u1, i1 = res[fused]
+                    d = ((u1 - u0).abs().max() / u0.abs().max()).item()
+                    assert d < 1e-8, (skind, pc, asm, fused, d)
+                    # FP32 recurrences (and the atomic assembly's summation
+                    # order) move the count by a few percent either way
+                    slack = 0.05 if skind == igamf.SOLVE_PCG_F64 else 0.12
+                    assert abs(i1 - i0) <= max(3, slack * i0), (skind, pc, asm, fused, i0, i1)
+    if geom == "annulus":
+        op1 = igamf.Operator(m, variant_f64=1, variant_f32=1)
+        s = igamf.Solver(op1, None, kind=igamf.SOLVE_PCG_F32_REFINE, rtol=1e-10)
+        u, st = s.solve(F)
+        assert st.status == 0 and st.rel_residual <= 1.0001e-10
 
 
 def test_fem_preconditioned_solve():
