@@ -7,8 +7,10 @@ operator and vectors resident in HBM; a step is one apply (inverse-LM gather
 assembly).  Alongside, in the same JSON line:
 
   solve     CG to ||f - A u|| / ||f|| <= 1e-8, f = 1: plain FP64 PCG against
-            FP32 PCG with FP64 reliable updates (defect correction), Jacobi,
-            both assemblies;
+            FP32 PCG with FP64 reliable updates (defect correction), and the
+            same with the inner FP32 operator on FP16-stored G_q, Jacobi,
+            both assemblies, the three iteration forms (separate p.Ap kernel,
+            p.Ap inside the apply, single-reduction iteration);
   block     multi-RHS apply, quarter annulus p = 4, 512 x 512, k = 32
             (autotuned over k in {8, 16, 32});
   c5        unit square p = 5, 2048 x 2048, perturbed interior points and
@@ -103,7 +105,7 @@ def headline(a, out):
     p, n = a.p, a.nelem
     t0 = time.perf_counter()
     mesh = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, p, n, r0=1.0, r1=2.0)
-    op = igamf.Operator(mesh)
+    op = igamf.Operator(mesh, f16_geometry=True)
     setup_s = time.perf_counter() - t0
     info = op.info
     nfunc = op.nfunc
@@ -205,6 +207,28 @@ def fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, nfunc, E, nqp):
     out["apply"]["f64_median_ms"] = round(med64, 5)
     out["apply"]["f64_gdofs"] = round(n_int / (med64 * 1e-3) / 1e9, 4)
     out["apply"]["f64_hbm_frac"] = round(alg_bytes64 / (med64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    k64 = op.time_kernel(x64, torch.empty_like(x64), a.steps, kern, asm)
+    out["apply"]["f64_kernel_ms"] = round(k64, 5)
+    out["apply"]["f64_kernel_hbm_frac"] = round(alg_bytes64 / (k64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    # FP32 arithmetic on the FP16 copy of G_q (the inner operator of the
+    # FP16-geometry refinement solve)
+    if kern == igamf.SF and op.info.f16_geometry:
+        y16 = torch.empty_like(x32)
+        med16, _ = ev_time(lambda: op.apply(x32, y16, kernel=kern, assembly=asm, g16=True),
+                           a.steps, a.warmup)
+        k16 = op.time_kernel(x32, torch.empty_like(x32), a.steps, kern, asm, g16=True)
+        alg_bytes16 = 4 * (2 * nfunc) + 2 * 3 * E * nqp
+        y64r = op.apply(x64, kernel=kern, assembly=asm)
+        op.apply(x32, y16, kernel=kern, assembly=asm, g16=True)
+        out["apply"]["g16"] = {
+            "wall_event_median_ms": round(med16, 5), "kernel_ms": round(k16, 5),
+            "gdofs": round(n_int / (med16 * 1e-3) / 1e9, 4),
+            "algorithmic_bytes": alg_bytes16,
+            "kernel_hbm_frac": round(alg_bytes16 / (k16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "max_rel_diff_vs_f64": ((y16.double() - y64r).abs().max()
+                                    / y64r.abs().max()).item(),
+            "f16_scale": op.info.f16_scale}
+        log(f"g16 apply kernel {k16 * 1e3:.1f} us")
     if a.detail:
         var = {}
         for kn, kv in (("sf", igamf.SF), ("ebe", igamf.EBE)):
@@ -218,35 +242,52 @@ def fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, nfunc, E, nqp):
 
 
 def solves(out, op):
-    """Plain FP64 PCG against FP32 PCG + FP64 reliable updates, Jacobi, both
-    assemblies, f = 1, to 1e-8 relative residual (FP64 residual check)."""
+    """Plain FP64 PCG against FP32 PCG + FP64 reliable updates, Jacobi, f = 1,
+    to 1e-8 relative residual (FP64 residual check), for both assemblies and
+    the three iteration forms (fused = 0: separate p.Ap kernel; 1: p.Ap inside
+    the apply kernel; 2: single-reduction iteration)."""
     import torch
     import igamf
     F = op.load(igamf.LOAD_CONST, 1.0)
     res = {}
     sols = {}
+    cases = [("fp64", igamf.SOLVE_PCG_F64, 0), ("fp32_refine", igamf.SOLVE_PCG_F32_REFINE, 0)]
+    if op.info.f16_geometry:
+        cases.append(("fp32g16_refine", igamf.SOLVE_PCG_F32_REFINE, 1))
     for an, av in (("gather", igamf.INVLM_GATHER), ("atomic", igamf.IEN_ATOMIC)):
-        for name, kind in (("fp64", igamf.SOLVE_PCG_F64),
-                           ("fp32_refine", igamf.SOLVE_PCG_F32_REFINE)):
-            s = igamf.Solver(op, None, kind=kind, pc=igamf.PC_JACOBI, rtol=1e-8,
-                             assembly=av, maxit=200000)
-            s.solve(F)  # first call captures the CG chunk graph
-            ts = []
-            for _ in range(3):
-                u, st = s.solve(F)
-                ts.append(st.solve_ms)
-            torch.cuda.synchronize()
-            res[f"{name}_{an}"] = {"solve_ms": round(median(ts), 2), "iterations": st.iterations,
-                                   "reliable_updates": st.outer_iterations,
-                                   "rel_residual": st.rel_residual, "status": st.status}
-            sols[f"{name}_{an}"] = u
-            log(f"solve {name} {an}: {median(ts):.1f} ms, {st.iterations} its")
-    ref = sols["fp64_gather"]
-    for k in ("fp32_refine_gather", "fp64_atomic", "fp32_refine_atomic"):
-        res[f"{k}_solution_rel_diff"] = (sols[k] - ref).norm().item() / ref.norm().item()
-    for an in ("gather", "atomic"):
-        res[f"speedup_fp32_refine_vs_fp64_{an}"] = round(
-            res[f"fp64_{an}"]["solve_ms"] / res[f"fp32_refine_{an}"]["solve_ms"], 3)
+        for fused in (0, 1, 2):
+            for name, kind, g16 in cases:
+                s = igamf.Solver(op, None, kind=kind, pc=igamf.PC_JACOBI, rtol=1e-8,
+                                 assembly=av, maxit=200000, fused=fused, inner_f16_geometry=g16)
+                s.solve(F)  # first call captures the CG chunk graph
+                ts = []
+                for _ in range(3):
+                    u, st = s.solve(F)
+                    ts.append(st.solve_ms)
+                torch.cuda.synchronize()
+                key = f"{name}_{an}_fused{fused}"
+                res[key] = {"solve_ms": round(median(ts), 2), "iterations": st.iterations,
+                            "reliable_updates": st.outer_iterations,
+                            "us_per_iteration": round(median(ts) * 1e3 / st.iterations, 1),
+                            "rel_residual": st.rel_residual, "status": st.status}
+                sols[key] = u
+                log(f"solve {key}: {median(ts):.1f} ms, {st.iterations} its")
+    ref = sols["fp64_gather_fused0"]
+    for k in sols:
+        if k != "fp64_gather_fused0":
+            res[k]["solution_rel_diff"] = (sols[k] - ref).norm().item() / ref.norm().item()
+    best = {}
+    for name, _, _ in cases:
+        k = min((k for k in res if k.startswith(name + "_")), key=lambda k: res[k]["solve_ms"])
+        best[name] = k
+        res[f"best_{name}"] = k
+    classic64 = min(res["fp64_gather_fused0"]["solve_ms"], res["fp64_atomic_fused0"]["solve_ms"])
+    for name, _, _ in cases[1:]:
+        res[f"speedup_best_{name}_vs_best_fp64"] = round(
+            res[best["fp64"]]["solve_ms"] / res[best[name]]["solve_ms"], 3)
+        # against the iteration with a separate p.Ap kernel (the classic form)
+        res[f"speedup_best_{name}_vs_classic_fp64"] = round(
+            classic64 / res[best[name]]["solve_ms"], 3)
     res["preconditioner"] = "jacobi"
     res["inner_rtol"] = "library default (0.5)"
     out["solve"] = res

@@ -116,7 +116,15 @@ int iga_rng_uniform(uint64_t seed, int64_t n, double lo, double hi, double *out)
  * factors G_q = kappa_q w_q det(J_q) J_q^-1 J_q^-T are precomputed in FP64. */
 typedef struct iga_op iga_op;
 
-typedef enum { IGA_F64 = 0, IGA_F32 = 1 } iga_dtype;
+typedef enum {
+  IGA_F64 = 0,
+  IGA_F32 = 1,
+  IGA_F32_G16 = 2  /* FP32 vectors and arithmetic on the FP16 copy of the
+                      geometry factors (iga_op_opts.f16_geometry; SF kernel,
+                      16x16 tiles or variant 3, not general rational meshes):
+                      half the operator bytes, G_q rounded to 11 bits (a
+                      symmetric perturbation of A of relative size <= 2^-11) */
+} iga_dtype;
 
 typedef enum {
   IGA_KERNEL_SF = 0,  /* sum-factorised element action */
@@ -153,6 +161,9 @@ typedef struct {
                                    -1 (default): 3 for FP32 at p >= 5 and for
                                    FP64 at p = 3 (not general rational),
                                    else 0 */
+  int f16_geometry;   /* also keep an FP16 copy of the FP32 G_q, divided by a
+                         power of two that maps their largest magnitude to
+                         2^14 (IGA_F32_G16 applies; default 0) */
 } iga_op_opts;
 
 typedef struct {
@@ -162,6 +173,8 @@ typedef struct {
   int64_t device_bytes;       /* device memory held by the operator */
   double setup_ms;            /* wall time of iga_op_create */
   int variant_f64, variant_f32; /* apply variant in use (see iga_op_opts) */
+  int f16_geometry;           /* the FP16 copy of G_q exists */
+  double f16_scale;           /* FP16 G_q = G_q / f16_scale (a power of two) */
 } iga_op_info;
 
 void iga_op_opts_default(iga_op_opts *o);
@@ -311,6 +324,11 @@ typedef struct {
   double pc_inner_rtol;         /* FEM_CG: inner CG tolerance */
   int pc_cheb_degree;           /* FEM_CHEB: polynomial degree */
   int use_graph;                /* capture the iteration body in a HIP graph */
+  int inner_f16_geometry;       /* F32_REFINE: the inner FP32 PCG applies
+                                   IGA_F32_G16 (the op must keep the FP16
+                                   copy); the FP64 residual of the reliable
+                                   updates removes its perturbation (default
+                                   0) */
   int fused;                    /* 0: the p.Ap product by its own kernel
                                    after the apply;
                                    1 (default): p.Ap accumulated inside the

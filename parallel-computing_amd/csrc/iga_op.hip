@@ -161,6 +161,10 @@ bool cg_fusable(const iga_op *op, bool f64, bool ebe) {
   return op->rational == 2 || ebe || (f64 ? op->variant64 : op->variant32) != 1;
 }
 
+bool cg_fusable_dt(const iga_op *op, int dt, bool ebe) {
+  return dt == IGA_F32_G16 || cg_fusable(op, dt == IGA_F64, ebe);
+}
+
 int apply_tiles(const iga_op *op, bool f64) {
   const igak::Layout &L = f64 ? op->L64 : op->L32;
   return L.ntx * L.nty;
@@ -169,9 +173,19 @@ int apply_tiles(const iga_op *op, bool f64) {
 int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
               int mask_in, int mask_out, hipStream_t s, const int *skip, bool y_zeroed,
               const CgHook *hook) {
+  return apply_raw_dt(op, f64 ? IGA_F64 : IGA_F32, ebe, gather, x, y, mask_in, mask_out, s, skip,
+                      y_zeroed, hook);
+}
+
+int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void *y,
+                 int mask_in, int mask_out, hipStream_t s, const int *skip, bool y_zeroed,
+                 const CgHook *hook) {
+  const bool f64 = dt == IGA_F64, g16 = dt == IGA_F32_G16;
   const igak::Layout &L = f64 ? op->L64 : op->L32;
-  if (hook && !cg_fusable(op, f64, ebe)) return IGA_ERR_UNSUPPORTED;
+  if (g16 && (!op->G16 || ebe)) return IGA_ERR_UNSUPPORTED;
+  if (hook && !cg_fusable_dt(op, dt, ebe)) return IGA_ERR_UNSUPPORTED;
   igak::ApplyArgs a = {};
+  a.gscale = op->g16_scale;
   if (hook) {
     a.dot = hook->dot;
     a.hsrc = hook->hsrc;
@@ -180,7 +194,7 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
   }
   a.x = x;
   a.y = y;
-  a.G = f64 ? (const void *)op->G64 : (const void *)op->G32;
+  a.G = f64 ? (const void *)op->G64 : (g16 ? (const void *)op->G16 : (const void *)op->G32);
   a.tabx = f64 ? (const void *)op->tabx64 : (const void *)op->tabx32;
   a.taby = f64 ? (const void *)op->taby64 : (const void *)op->taby32;
   a.dirmask = op->dirmask;
@@ -209,7 +223,9 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
   }
   const bool prof = op->prof_on && op->prof_used < op->prof_cap;
   if (prof) IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used], s));
-  if (a.wf)
+  if (g16)
+    IGA_HIP_TRY(op->ops->apply16(a, tile, ntiles, s));
+  else if (a.wf)
     IGA_HIP_TRY((f64 ? op->ops->rat64 : op->ops->rat32)(a, ebe, tile, ntiles, s));
   else
     IGA_HIP_TRY((f64 ? op->ops->apply64 : op->ops->apply32)(
@@ -230,6 +246,19 @@ int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *
 }
 
 }  // namespace igai
+
+namespace {
+// largest |v| over n floats, as the bit pattern of a non-negative float
+// (ordered like unsigned integers): atomicMax per thread
+__global__ void k_absmax_bits(const float *v, int64_t n, unsigned *out) {
+  float m = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(v[i]));
+  if (m != m) m = __builtin_inff();
+  atomicMax(out, __float_as_uint(m));
+}
+}  // namespace
 
 extern "C" {
 
@@ -511,6 +540,32 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
                                   op->kappa, nex, ney, nfx, L, s))
       return fail(IGA_ERR_HIP);
     ALLOC(op->halo32, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 4);
+    if (o.f16_geometry) {
+      // FP16 copy for IGA_F32_G16: G32 / 2^e with the largest magnitude at
+      // most 2^14 (range margin below 65504; values 2^28 times smaller stay
+      // normal)
+      if (rational == 2 || !igak::g16_compiled(op->tile32) || !ops->apply16)
+        return fail(IGA_ERR_UNSUPPORTED);
+      unsigned *dmax = nullptr;
+      ALLOC(dmax, sizeof(unsigned));
+      if (hipMemset(dmax, 0, sizeof(unsigned)) != hipSuccess) return fail(IGA_ERR_HIP);
+      hipLaunchKernelGGL(k_absmax_bits, dim3(1024), dim3(256), 0, s, op->G32, (int64_t)n, dmax);
+      unsigned hb = 0;
+      if (hipMemcpy(&hb, dmax, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(IGA_ERR_HIP);
+      float gmax;
+      std::memcpy(&gmax, &hb, sizeof(float));
+      if (!std::isfinite(gmax)) return fail(IGA_ERR_NOT_FINITE);
+      int e = 0;
+      if (gmax > 0.f) {
+        (void)std::frexp(gmax, &e);  // gmax < 2^e
+        e -= 14;
+      }
+      op->g16_scale = std::ldexp(1.0f, e);
+      ALLOC(op->G16, n * 2);
+      if (ops->g16conv(op->G32, op->G16, L, std::ldexp(1.0f, -e), s) != hipSuccess)
+        return fail(IGA_ERR_HIP);
+    }
   }
   ALLOC(op->tdot, (size_t)std::max(op->L64.ntx * op->L64.nty, op->L32.ntx * op->L32.nty) * 8);
   if (hipDeviceSynchronize() != hipSuccess) return fail(IGA_ERR_HIP);
@@ -540,21 +595,23 @@ int iga_op_get_info(const iga_op *op, iga_op_info *info) {
   info->setup_ms = op->setup_ms;
   info->variant_f64 = op->variant64;
   info->variant_f32 = op->variant32;
+  info->f16_geometry = op->G16 != nullptr;
+  info->f16_scale = op->g16_scale;
   return IGA_OK;
 }
 
 int iga_op_apply(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb, const void *x,
                  void *y, void *stream) {
   if (!op || !x || !y || x == y) return IGA_ERR_INVALID_ARGUMENT;
-  if ((dt != IGA_F64 && dt != IGA_F32) || (kern != IGA_KERNEL_SF && kern != IGA_KERNEL_EBE) ||
+  if ((dt != IGA_F64 && dt != IGA_F32 && dt != IGA_F32_G16) ||
+      (kern != IGA_KERNEL_SF && kern != IGA_KERNEL_EBE) ||
       (asmb != IGA_ASSEMBLY_IEN_ATOMIC && asmb != IGA_ASSEMBLY_INVLM_GATHER))
     return IGA_ERR_INVALID_ARGUMENT;
   if (dt == IGA_F32 && !op->G32) return IGA_ERR_UNSUPPORTED;
   int rc = igai::set_device(op->device);
   if (rc) return rc;
-  return igai::apply_raw(op, dt == IGA_F64, kern == IGA_KERNEL_EBE,
-                         asmb == IGA_ASSEMBLY_INVLM_GATHER, x, y, 0, 1, (hipStream_t)stream,
-                         nullptr);
+  return igai::apply_raw_dt(op, dt, kern == IGA_KERNEL_EBE, asmb == IGA_ASSEMBLY_INVLM_GATHER, x,
+                            y, 0, 1, (hipStream_t)stream, nullptr);
 }
 
 namespace {
@@ -578,18 +635,19 @@ __global__ void k_fold_tiles(const double *part, int n, double *out) {
 int iga_op_apply_dot(iga_op *op, iga_dtype dt, iga_kernel kern, iga_assembly asmb,
                      const void *x, void *y, double *xy, void *stream) {
   if (!op || !x || !y || !xy || x == y) return IGA_ERR_INVALID_ARGUMENT;
-  if ((dt != IGA_F64 && dt != IGA_F32) || (kern != IGA_KERNEL_SF && kern != IGA_KERNEL_EBE) ||
+  if ((dt != IGA_F64 && dt != IGA_F32 && dt != IGA_F32_G16) ||
+      (kern != IGA_KERNEL_SF && kern != IGA_KERNEL_EBE) ||
       (asmb != IGA_ASSEMBLY_IEN_ATOMIC && asmb != IGA_ASSEMBLY_INVLM_GATHER))
     return IGA_ERR_INVALID_ARGUMENT;
   const bool f64 = dt == IGA_F64, ebe = kern == IGA_KERNEL_EBE;
-  if (!f64 && !op->G32) return IGA_ERR_UNSUPPORTED;
-  if (!igai::cg_fusable(op, f64, ebe)) return IGA_ERR_UNSUPPORTED;
+  if (dt == IGA_F32 && !op->G32) return IGA_ERR_UNSUPPORTED;
+  if (!igai::cg_fusable_dt(op, dt, ebe)) return IGA_ERR_UNSUPPORTED;
   int rc = igai::set_device(op->device);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const igai::CgHook hook{op->tdot, nullptr, nullptr, 0};
-  rc = igai::apply_raw(op, f64, ebe, asmb == IGA_ASSEMBLY_INVLM_GATHER, x, y, 0, 1, s, nullptr,
-                       false, &hook);
+  rc = igai::apply_raw_dt(op, dt, ebe, asmb == IGA_ASSEMBLY_INVLM_GATHER, x, y, 0, 1, s, nullptr,
+                          false, &hook);
   if (rc) return rc;
   hipLaunchKernelGGL(k_fold_tiles, dim3(1), dim3(256), 0, s, op->tdot,
                      igai::apply_tiles(op, f64), xy);
@@ -722,9 +780,10 @@ int iga_op_get_block_config(const iga_op *op, int k, int *cfg) {
 int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, void *y,
                        int nlaunch, void *stream, double *avg_ms) {
   if (!op || !x || !y || x == y || nlaunch < 1 || !avg_ms) return IGA_ERR_INVALID_ARGUMENT;
-  if (dt != IGA_F64 && dt != IGA_F32) return IGA_ERR_INVALID_ARGUMENT;
-  const bool f64 = dt == IGA_F64;
-  if ((f64 && !op->G64) || (!f64 && !op->G32)) return IGA_ERR_UNSUPPORTED;
+  if (dt != IGA_F64 && dt != IGA_F32 && dt != IGA_F32_G16) return IGA_ERR_INVALID_ARGUMENT;
+  const bool f64 = dt == IGA_F64, g16 = dt == IGA_F32_G16;
+  if ((f64 && !op->G64) || (dt == IGA_F32 && !op->G32)) return IGA_ERR_UNSUPPORTED;
+  if (g16 && (!op->G16 || kern == IGA_KERNEL_EBE)) return IGA_ERR_UNSUPPORTED;
   int rc = igai::set_device(op->device);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -732,7 +791,8 @@ int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, vo
   igak::ApplyArgs a = {};
   a.x = x;
   a.y = y;
-  a.G = f64 ? (const void *)op->G64 : (const void *)op->G32;
+  a.gscale = op->g16_scale;
+  a.G = f64 ? (const void *)op->G64 : (g16 ? (const void *)op->G16 : (const void *)op->G32);
   a.tabx = f64 ? (const void *)op->tabx64 : (const void *)op->tabx32;
   a.taby = f64 ? (const void *)op->taby64 : (const void *)op->taby32;
   a.dirmask = op->dirmask;
@@ -755,6 +815,7 @@ int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, vo
   IGA_HIP_TRY(hipEventCreate(&e0));
   IGA_HIP_TRY(hipEventCreate(&e1));
   auto launch = [&]() {
+    if (g16) return op->ops->apply16(a, tile, ntiles, s);
     return a.wf ? (f64 ? op->ops->rat64 : op->ops->rat32)(a, ebe, tile, ntiles, s)
                 : (f64 ? op->ops->apply64 : op->ops->apply32)(
                       a, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s);

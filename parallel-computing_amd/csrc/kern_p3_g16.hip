@@ -1,0 +1,4 @@
+// FP32 apply kernels of degree 3 on FP16-stored geometry factors.
+#include "launch.hpp"
+
+IGAK_DEFINE_G16(3)

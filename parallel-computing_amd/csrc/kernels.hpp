@@ -19,6 +19,8 @@
 // (matrix_free/src/GlobalAssemblyMF.cpp:166-205).
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 // 1: load all of an element's G_q into registers at kernel entry;
@@ -55,6 +57,7 @@ struct ApplyArgs {
   const double *hsrc;     // workgroup 0 copies hn doubles hsrc -> hdst (hand-off of
   double *hdst;           // the CG state between the update kernels around this one)
   int hn;
+  float gscale;           // FP16-stored G_q: the power of two they were divided by
 };
 
 __device__ __forceinline__ bool is_dirichlet(const uint8_t *mask, int A, int gx,
@@ -125,23 +128,29 @@ template <>
 struct Vec16<double> {
   typedef double type __attribute__((ext_vector_type(2)));
 };
+template <>
+struct Vec16<_Float16> {
+  typedef _Float16 type __attribute__((ext_vector_type(8)));
+};
 
-template <typename T, int NQ2, int NC = 3>
-__device__ __forceinline__ void load_gq(const T *G, int tile, int t, int NT, T (&g)[NC][NQ2]) {
-  constexpr int VW = 16 / sizeof(T);
+// G_q of element t of a tile into registers of type T (TG: storage type)
+template <typename TG, int NQ2, int NC = 3, typename T>
+__device__ __forceinline__ void load_gq(const TG *G, int tile, int t, int NT, T (&g)[NC][NQ2]) {
+  constexpr int VW = 16 / sizeof(TG);
   constexpr int NV = NQ2 / VW;
-  typedef typename Vec16<T>::type V;
+  typedef typename Vec16<TG>::type V;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const T *Gc = G + ((size_t)tile * NC + c) * NQ2 * NT;
+    const TG *Gc = G + ((size_t)tile * NC + c) * NQ2 * NT;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       const V v = *reinterpret_cast<const V *>(Gc + ((size_t)k * NT + t) * VW);
 #pragma unroll
-      for (int l = 0; l < VW; ++l) g[c][k * VW + l] = v[l];
+      for (int l = 0; l < VW; ++l) g[c][k * VW + l] = (T)v[l];
     }
 #pragma unroll
-    for (int r = NV * VW; r < NQ2; ++r) g[c][r] = Gc[(size_t)NV * VW * NT + (size_t)(r - NV * VW) * NT + t];
+    for (int r = NV * VW; r < NQ2; ++r)
+      g[c][r] = (T)Gc[(size_t)NV * VW * NT + (size_t)(r - NV * VW) * NT + t];
   }
 }
 
@@ -570,8 +579,10 @@ __device__ __forceinline__ void cg_handoff(const ApplyArgs &A, int nt) {
 // SPLIT = 2: two threads per element (sum-factorised only), each
 // contracting half of the quadrature columns; halves the per-thread state
 // of high degrees and FP64.
+// TG: storage type of G_q (T, or _Float16 for the FP16-geometry FP32 apply:
+// values divided by A.gscale, the element result multiplied back by it).
 template <typename T, int P, int NQ, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1,
-          int SPLIT = 1>
+          int SPLIT = 1, typename TG = T>
 __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   static_assert(SPLIT == 1 || !EBE, "split elements are sum-factorised");
   constexpr int NB = P + 1, NE = TX * TY, NT = NE * SPLIT, FX = TX + P, FY = TY + P;
@@ -603,8 +614,8 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   constexpr bool PREFH = IGAK_PREFETCH_G && SPLIT > 1 && NC * NQ * NI * sizeof(T) <= 400;
   T g[PREF ? NC : 1][PREF ? NQ2 : 1];
   T gh[PREFH ? NC : 1][PREFH ? NQ : 1][PREFH ? NI : 1];
-  const T *Gt = static_cast<const T *>(A.G);
-  if constexpr (PREF) load_gq<T, NQ2, NC>(Gt, tile, el, NE, g);
+  const TG *Gt = static_cast<const TG *>(A.G);
+  if constexpr (PREF) load_gq<TG, NQ2, NC>(Gt, tile, el, NE, g);
   if constexpr (PREFH) {
     const int i0 = h * NI;
 #pragma unroll
@@ -614,7 +625,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
 #pragma unroll
         for (int il = 0; il < NI; ++il)
           gh[c][j][il] = (i0 + il < NQ)
-                             ? Gt[gq_index<T, NQ2, NC>(tile, c, j * NQ + i0 + il, el, NE)]
+                             ? (T)Gt[gq_index<TG, NQ2, NC>(tile, c, j * NQ + i0 + il, el, NE)]
                              : T(0);
   }
   auto gl = [&](int c, int q, int j, int il) {
@@ -623,7 +634,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
     else if constexpr (PREFH)
       return gh[c][j][il];
     else
-      return Gt[gq_index<T, NQ2, NC>(tile, c, q, el, NE)];
+      return (T)Gt[gq_index<TG, NQ2, NC>(tile, c, q, el, NE)];
   };
   const T *wf = static_cast<const T *>(A.wf);
   stg.store(A, ex0, ey0, t, xs, tbx, tby);
@@ -646,6 +657,13 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int a = 0; a < NB; ++a) ye[b][a] += dpp_row<0xB1>(ye[b][a]);
+    }
+    if constexpr (!std::is_same_v<TG, T>) {
+      const T gs = (T)A.gscale;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NB; ++a) ye[b][a] *= gs;
     }
     if (!valid) {
 #pragma unroll
@@ -857,6 +875,24 @@ __global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex, int
       acc += halo[(size_t)(ty * ntx + tx) * FX * FY + k];
     }
   y[Ag] = acc;
+}
+
+// FP16 copy of FP32 G_q (same tile layout, 8 values per 16-byte vector):
+// G16 = G32 / scale, rounded to nearest
+template <int NQ2, int NC>
+__global__ void k_g16_from_g32(const float *__restrict__ G32, _Float16 *__restrict__ G16,
+                               int64_t n, int NT, float inv_scale) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(i % NT);
+    const int64_t r = i / NT;
+    const int q = (int)(r % NQ2);
+    const int64_t r2 = r / NQ2;
+    const int c = (int)(r2 % NC);
+    const int tile = (int)(r2 / NC);
+    G16[gq_index<_Float16, NQ2, NC>(tile, c, q, t, NT)] =
+        (_Float16)(G32[gq_index<float, NQ2, NC>(tile, c, q, t, NT)] * inv_scale);
+  }
 }
 
 // --------------------------------------------------------------------------

@@ -36,6 +36,7 @@ inline bool tile_compiled(int p, int id) {
 }
 inline bool pf_compiled(int p, int id) { return p <= 4 || id == TILE_16x16; }
 inline bool w4_compiled(int p, int id, bool f64) { return p <= 4 && id == TILE_16x16 && !f64; }
+inline bool g16_compiled(int id) { return id == TILE_16x16 || id == TILE_16x8; }
 inline bool rat_compiled(int p, int id) {
   return id == TILE_16x16 || id == TILE_16x8 || (p <= 4 && id == TILE_32x16);
 }
@@ -44,10 +45,29 @@ template <int P>
 struct Launch {
   static constexpr int NQ = P + 1;
 
-  template <typename T, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1, int SPLIT = 1>
+  template <typename T, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1, int SPLIT = 1,
+            typename TG = T>
   static hipError_t apply_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE, RAT, MINW, SPLIT>), dim3(ntiles),
+    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE, RAT, MINW, SPLIT, TG>), dim3(ntiles),
                        dim3(TX * TY * SPLIT), 0, s, a);
+    return hipGetLastError();
+  }
+  // FP32 SF apply on FP16-stored G_q (polynomial / tensor-weight meshes):
+  // the 16x16 one-workgroup-per-tile kernel, or the split-element one on
+  // 16x8 tiles
+  static hipError_t apply_g16(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {
+    if (tile == TILE_16x16) return apply_t<float, 16, 16, false, false, 1, 1, _Float16>(a, ntiles, s);
+    if (tile == TILE_16x8) return apply_t<float, 16, 8, false, false, 1, 2, _Float16>(a, ntiles, s);
+    return hipErrorInvalidValue;
+  }
+  static hipError_t g16_from_g32(const float *G32, _Float16 *G16, const Layout &L,
+                                 float inv_scale, hipStream_t s) {
+    const int NT = L.tx * L.ty;
+    const int64_t n = (int64_t)L.ntx * L.nty * 3 * NQ * NQ * NT;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL((k_g16_from_g32<NQ * NQ, 3>), dim3((unsigned)nb), dim3(256), 0, s, G32,
+                       G16, n, NT, inv_scale);
     return hipGetLastError();
   }
   template <typename T, bool EBE>
@@ -237,6 +257,8 @@ struct Ops {
   hipError_t (*rat64)(const ApplyArgs &, bool ebe, int tile, int ntiles, hipStream_t);
   hipError_t (*rat32)(const ApplyArgs &, bool ebe, int tile, int ntiles, hipStream_t);
   hipError_t (*block)(const BlockArgs &, int cfg, hipStream_t);
+  hipError_t (*apply16)(const ApplyArgs &, int tile, int ntiles, hipStream_t);
+  hipError_t (*g16conv)(const float *, _Float16 *, const Layout &, float, hipStream_t);
 };
 
 template <int P>
@@ -254,6 +276,8 @@ Ops make_ops() {
   o.qp_coords = &Launch<P>::qp_coords;
   o.rat64 = o.rat32 = nullptr;
   o.block = nullptr;
+  o.apply16 = nullptr;
+  o.g16conv = nullptr;
   return o;
 }
 
@@ -268,6 +292,8 @@ const Ops *ops_for_degree(int p);
   hipError_t rat64_p##P(const ApplyArgs &, bool, int, int, hipStream_t);                  \
   hipError_t rat32_p##P(const ApplyArgs &, bool, int, int, hipStream_t);                  \
   hipError_t block_p##P(const BlockArgs &, int, hipStream_t);                             \
+  hipError_t apply16_p##P(const ApplyArgs &, int, int, hipStream_t);                      \
+  hipError_t g16conv_p##P(const float *, _Float16 *, const Layout &, float, hipStream_t); \
   const Ops *ops_p##P() {                                                                 \
     static const Ops o = [] {                                                             \
       Ops r = make_ops<P>();                                                              \
@@ -275,6 +301,8 @@ const Ops *ops_for_degree(int p);
       r.rat64 = &rat64_p##P;                                                              \
       r.rat32 = &rat32_p##P;                                                              \
       r.block = &block_p##P;                                                              \
+      r.apply16 = &apply16_p##P;                                                          \
+      r.g16conv = &g16conv_p##P;                                                          \
       return r;                                                                           \
     }();                                                                                  \
     return &o;                                                                            \
@@ -287,6 +315,18 @@ const Ops *ops_for_degree(int p);
   hipError_t apply64_p##P(const ApplyArgs &a, bool ebe, int tile, int ntiles, int variant, \
                           hipStream_t s) {                                                \
     return Launch<P>::template apply_prec<double>(a, ebe, tile, ntiles, variant, s);      \
+  }                                                                                       \
+  }
+
+// FP32 apply kernels on FP16-stored G_q of degree P (kern_p<P>_g16.hip)
+#define IGAK_DEFINE_G16(P)                                                                \
+  namespace igak {                                                                        \
+  hipError_t apply16_p##P(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {      \
+    return Launch<P>::apply_g16(a, tile, ntiles, s);                                      \
+  }                                                                                       \
+  hipError_t g16conv_p##P(const float *G32, _Float16 *G16, const Layout &L, float inv,    \
+                          hipStream_t s) {                                                \
+    return Launch<P>::g16_from_g32(G32, G16, L, inv, s);                                  \
   }                                                                                       \
   }
 

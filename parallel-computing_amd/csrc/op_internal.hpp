@@ -40,6 +40,8 @@ struct iga_op {
   double *G64 = nullptr;            // kappa-weighted
   double *G0 = nullptr;             // geometry only (== G64 when kappa unset)
   float *G32 = nullptr;
+  _Float16 *G16 = nullptr;          // FP16 copy of G32 / g16_scale (f16_geometry)
+  float g16_scale = 1.0f;
   double *kappa = nullptr;          // device [E][nqp] or nullptr
   double *halo64 = nullptr;
   float *halo32 = nullptr;
@@ -73,6 +75,7 @@ struct CgHook {
 };
 // whether the apply kernel configured for this precision supports CgHook
 bool cg_fusable(const iga_op *op, bool f64, bool ebe);
+bool cg_fusable_dt(const iga_op *op, int dt, bool ebe);
 // tiles of the apply layout of this precision (entries of CgHook::dot)
 int apply_tiles(const iga_op *op, bool f64);
 
@@ -81,6 +84,10 @@ int apply_tiles(const iga_op *op, bool f64);
 int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
               int mask_in, int mask_out, hipStream_t s, const int *skip,
               bool y_zeroed = false, const CgHook *hook = nullptr);
+// the same with an iga_dtype (IGA_F32_G16: FP32 arithmetic on the FP16 G_q)
+int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void *y,
+                 int mask_in, int mask_out, hipStream_t s, const int *skip,
+                 bool y_zeroed = false, const CgHook *hook = nullptr);
 // Y = A X for k FP32 columns (leading dimension ld), the configuration
 // tuned or defaulted for k; no device switch (callable under graph capture)
 int apply_block_raw(iga_op *op, int k, int ld, const float *X, float *Y, hipStream_t s);

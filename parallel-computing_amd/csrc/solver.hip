@@ -85,6 +85,28 @@ __device__ __forceinline__ double block_sum_all(double v) {
   return res;
 }
 
+// Sum over the workgroup of N values at once (fixed order), to every thread.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N]) {
+  __shared__ double sm[N][RB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) sm[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < RB / 64; ++i) s += sm[k][i];
+    v[k] = s;
+  }
+}
+
 // Write this workgroup's partial sum of v.
 __device__ __forceinline__ void put_partial(double *slot, double v) {
   v = block_sum_all(v);
@@ -220,6 +242,40 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
   z += voff_;
   if (ROLD) rold += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
+  // Four elements per thread and step, every load of a step issued before
+  // any store; the first step's loads are issued before the p.Ap fold and
+  // each next step's before the current one is processed, so their latency
+  // overlaps the fold and the stores.
+  constexpr int U = 4;
+  const int64_t step = (int64_t)gridDim.x * RB * U;
+  auto load = [&](int64_t b, V (&uu)[U], V (&pv)[U], V (&rv)[U], V (&av)[U], D (&dv)[U]) {
+    if (b + (int64_t)(U - 1) * RB < n) {  // whole step in range: unconditional loads
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b + (int64_t)k * RB;
+        uu[k] = u[i];
+        pv[k] = p[i];
+        rv[k] = r[i];
+        av[k] = Ap[i];
+        if (LOCAL) dv[k] = dinv ? dinv[i] : D(1);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b + (int64_t)k * RB;
+        const bool in = i < n;
+        uu[k] = in ? u[i] : V(0);
+        pv[k] = in ? p[i] : V(0);
+        rv[k] = in ? r[i] : V(0);
+        av[k] = in ? Ap[i] : V(0);
+        if (LOCAL) dv[k] = (dinv && in) ? dinv[i] : D(1);
+      }
+    }
+  };
+  int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x;
+  V uu[U], pv[U], rv[U], av[U];
+  D dv[U];
+  if (b0 < n) load(b0, uu, pv, rv, av, dv);
   // p.Ap: partials of k_cg_pap, or the apply kernel's per-tile partials
   const double pap = tdot ? block_sum_all(fold_share(tdot, ntiles)) : fold(part + P_PAP * RG, nb);
   const double a = sc[S_RZ] / pap;
@@ -227,35 +283,12 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
     if (blockIdx.x == 0 && threadIdx.x == 0) ic[I_DONE] = 3;
     return;
   }
-  // four elements per thread and step, every load issued before any store
-  constexpr int U = 4;
   double rr = 0.0, rz = 0.0;
-  for (int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x; b0 < n;
-       b0 += (int64_t)gridDim.x * RB * U) {
-    V uu[U], pv[U], rv[U], av[U];
-    double dv[U];
-    if (b0 + (int64_t)(U - 1) * RB < n) {  // whole step in range: unconditional loads
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const int64_t i = b0 + (int64_t)k * RB;
-        uu[k] = u[i];
-        pv[k] = p[i];
-        rv[k] = r[i];
-        av[k] = Ap[i];
-        dv[k] = (LOCAL && dinv) ? (double)dinv[i] : 1.0;
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const int64_t i = b0 + (int64_t)k * RB;
-        const bool in = i < n;
-        uu[k] = in ? u[i] : V(0);
-        pv[k] = in ? p[i] : V(0);
-        rv[k] = in ? r[i] : V(0);
-        av[k] = in ? Ap[i] : V(0);
-        dv[k] = (LOCAL && dinv && in) ? (double)dinv[i] : 1.0;
-      }
-    }
+  while (b0 < n) {
+    const int64_t bn = b0 + step;
+    V uu2[U], pv2[U], rv2[U], av2[U];
+    D dv2[U];
+    if (bn < n) load(bn, uu2, pv2, rv2, av2, dv2);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t i = b0 + (int64_t)k * RB;
@@ -265,7 +298,7 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
       const V ri = (V)((double)rv[k] - a * (double)av[k]);
       r[i] = ri;
       if (LOCAL) {
-        const V zi = (V)((double)ri * dv[k]);
+        const V zi = (V)((double)ri * (double)dv[k]);
         z[i] = zi;
         rz += (double)ri * (double)zi;
         rr += prec_norm ? (double)zi * (double)zi : (double)ri * (double)ri;
@@ -273,6 +306,15 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
         rr += (double)ri * (double)ri;
       }
     }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      uu[k] = uu2[k];
+      pv[k] = pv2[k];
+      rv[k] = rv2[k];
+      av[k] = av2[k];
+      dv[k] = dv2[k];
+    }
+    b0 = bn;
   }
   put_partial(part + P_RR * RG, rr);
   if (LOCAL) put_partial(part + P_RZ * RG, rz);
@@ -312,9 +354,43 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
   z += voff_;
   if (Apz) Apz += voff_;
   if (ic[I_DONE] || (outer_done && *outer_done)) return;
-  const double rr = fold(part + P_RR * RG, nb);
-  const double rz = fold(part + P_RZ * RG, nb);
-  const double ro = flexible ? fold(part + P_RZO * RG, nb) : 0.0;
+  constexpr int U = 4;
+  const int64_t step = (int64_t)gridDim.x * RB * U;
+  auto load = [&](int64_t bq, V (&zv)[U], V (&pv)[U]) {
+    if (bq + (int64_t)(U - 1) * RB < n) {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        zv[k] = z[bq + (int64_t)k * RB];
+        pv[k] = p[bq + (int64_t)k * RB];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = bq + (int64_t)k * RB;
+        zv[k] = i < n ? z[i] : V(0);
+        pv[k] = i < n ? p[i] : V(0);
+      }
+    }
+  };
+  // first step's loads before the folds, each next step's before the current
+  // one is processed (latency overlap, as in k_cg_update)
+  int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x;
+  V zv[U], pv[U];
+  if (b0 < n) load(b0, zv, pv);
+  double f[3];
+  {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) {
+      a0 += part[P_RR * RG + i];
+      a1 += part[P_RZ * RG + i];
+      if (flexible) a2 += part[P_RZO * RG + i];
+    }
+    f[0] = a0;
+    f[1] = a1;
+    f[2] = a2;
+  }
+  block_sum_n<3>(f);
+  const double rr = f[0], rz = f[1], ro = f[2];
   const int it = ic[I_NEXT];
   int st = 0;
   if (rr <= sc[S_TOL2]) st = 1;
@@ -330,24 +406,10 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
     if (st) ic[I_DONE] = st;
   }
   if (st && st != 4) return;  // a reliable update keeps the search direction
-  constexpr int U = 4;
-  for (int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x; b0 < n;
-       b0 += (int64_t)gridDim.x * RB * U) {
-    V zv[U], pv[U];
-    if (b0 + (int64_t)(U - 1) * RB < n) {
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        zv[k] = z[b0 + (int64_t)k * RB];
-        pv[k] = p[b0 + (int64_t)k * RB];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const int64_t i = b0 + (int64_t)k * RB;
-        zv[k] = i < n ? z[i] : V(0);
-        pv[k] = i < n ? p[i] : V(0);
-      }
-    }
+  while (b0 < n) {
+    const int64_t bn = b0 + step;
+    V zv2[U], pv2[U];
+    if (bn < n) load(bn, zv2, pv2);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t i = b0 + (int64_t)k * RB;
@@ -356,6 +418,12 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
         if (Apz) Apz[i] = V(0);
       }
     }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      zv[k] = zv2[k];
+      pv[k] = pv2[k];
+    }
+    b0 = bn;
   }
 }
 
@@ -442,28 +510,6 @@ __global__ void k_ru_fin(const double *part, int nb, double *sc, int *ic, double
 enum { CG_RZO = 0, CG_PAP, CG_IT, CG_RR = 4, CG_RZ = CG_RR + RG, CG_US = CG_RZ + RG,
        CG_L = CG_US + RG };
 
-// Sum over the workgroup of N values at once (fixed order), to every thread.
-template <int N>
-__device__ __forceinline__ void block_sum_n(double (&v)[N]) {
-  __shared__ double sm[N][RB / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) sm[k][w] = v[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < RB / 64; ++i) s += sm[k][i];
-    v[k] = s;
-  }
-}
-
 // init: x = 0, r = f*fscale, u = M r, p = s = 0; NEW partials of r.r, r.u
 // and u.s (= 0)
 template <typename V, typename F, typename D>
@@ -536,6 +582,45 @@ __global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict
   if (ic[I_DONE]) return;
   const double *cur = cgs;
   double *nw = cgs + CG_L;
+  constexpr int U = 4;
+  const int64_t step = (int64_t)gridDim.x * RB * U;
+  struct Step {
+    V u[U], w[U], p[U], s[U], x[U], r[U];
+    D d[U];
+  };
+  auto load = [&](int64_t b, Step &q) {
+    if (b + (int64_t)(U - 1) * RB < n) {  // whole step in range: unconditional loads
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b + (int64_t)k * RB;
+        q.u[k] = u[i];
+        q.w[k] = w[i];
+        q.p[k] = p[i];
+        q.s[k] = sv[i];
+        q.x[k] = x[i];
+        q.r[k] = r[i];
+        q.d[k] = JAC ? dinv[i] : D(1);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int64_t i = b + (int64_t)k * RB;
+        const bool in = i < n;
+        q.u[k] = in ? u[i] : V(0);
+        q.w[k] = in ? w[i] : V(0);
+        q.p[k] = in ? p[i] : V(0);
+        q.s[k] = in ? sv[i] : V(0);
+        q.x[k] = in ? x[i] : V(0);
+        q.r[k] = in ? r[i] : V(0);
+        q.d[k] = (JAC && in) ? dinv[i] : D(1);
+      }
+    }
+  };
+  // first step's loads before the folds, each next step's before the current
+  // one is processed
+  int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x;
+  Step q;
+  if (b0 < n) load(b0, q);
   double f[4] = {0.0, 0.0, 0.0, 0.0};
   for (int i = threadIdx.x; i < nb; i += RB) {
     f[0] += cur[CG_RR + i];
@@ -563,49 +648,22 @@ __global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict
     }
     return;
   }
-  constexpr int U = 4;
   double v[3] = {0.0, 0.0, 0.0};
-  for (int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x; b0 < n;
-       b0 += (int64_t)gridDim.x * RB * U) {
-    V uv[U], wv[U], pv[U], sw[U], xv[U], rv[U];
-    D dv[U];
-    if (b0 + (int64_t)(U - 1) * RB < n) {  // whole step in range: every load issued at once
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const int64_t i = b0 + (int64_t)k * RB;
-        uv[k] = u[i];
-        wv[k] = w[i];
-        pv[k] = p[i];
-        sw[k] = sv[i];
-        xv[k] = x[i];
-        rv[k] = r[i];
-        dv[k] = JAC ? dinv[i] : D(1);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const int64_t i = b0 + (int64_t)k * RB;
-        const bool in = i < n;
-        uv[k] = in ? u[i] : V(0);
-        wv[k] = in ? w[i] : V(0);
-        pv[k] = in ? p[i] : V(0);
-        sw[k] = in ? sv[i] : V(0);
-        xv[k] = in ? x[i] : V(0);
-        rv[k] = in ? r[i] : V(0);
-        dv[k] = (JAC && in) ? dinv[i] : D(1);
-      }
-    }
+  while (b0 < n) {
+    const int64_t bn = b0 + step;
+    Step q2;
+    if (bn < n) load(bn, q2);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t i = b0 + (int64_t)k * RB;
       if (i >= n) continue;
-      const V pn = (V)((double)uv[k] + b * (double)pv[k]);
-      const V sn = (V)((double)wv[k] + b * (double)sw[k]);
-      const V rn = (V)((double)rv[k] - a * (double)sn);
-      const V un = JAC ? (V)((double)rn * (double)dv[k]) : rn;
+      const V pn = (V)((double)q.u[k] + b * (double)q.p[k]);
+      const V sn = (V)((double)q.w[k] + b * (double)q.s[k]);
+      const V rn = (V)((double)q.r[k] - a * (double)sn);
+      const V un = JAC ? (V)((double)rn * (double)q.d[k]) : rn;
       p[i] = pn;
       sv[i] = sn;
-      x[i] = (V)((double)xv[k] + a * (double)pn);
+      x[i] = (V)((double)q.x[k] + a * (double)pn);
       r[i] = rn;
       u[i] = un;
       if (zero_w) w[i] = V(0);
@@ -613,6 +671,8 @@ __global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict
       v[1] += (double)rn * (double)un;
       v[2] += (double)un * (double)sn;
     }
+    q = q2;
+    b0 = bn;
   }
   block_sum_n<3>(v);
   if (threadIdx.x == 0) {
@@ -891,6 +951,11 @@ int salloc(iga_solver *s, void **p, size_t bytes) {
 
 bool local_pc(const iga_solver *s) { return s->o.pc == IGA_PC_NONE || s->o.pc == IGA_PC_JACOBI; }
 
+// operator precision of the FP64 iteration or of the FP32 inner iteration
+int apply_dt(const iga_solver *s, bool f64) {
+  return f64 ? IGA_F64 : (s->o.inner_f16_geometry ? IGA_F32_G16 : IGA_F32);
+}
+
 #define LAUNCH(kern, grid, ...)                                              \
   do {                                                                       \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(RB), 0, st, __VA_ARGS__);      \
@@ -968,14 +1033,15 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
   const bool loc = local_pc(s);
+  const int dt = apply_dt(s, f64);
   // p.Ap accumulated by the apply kernel itself when it can, else k_cg_pap
-  const bool fdot = s->o.fused == 1 && s->tdot && igai::cg_fusable(s->op, f64, ebe);
+  const bool fdot = s->o.fused == 1 && s->tdot && igai::cg_fusable_dt(s->op, dt, ebe);
   const igai::CgHook hook{s->tdot, nullptr, nullptr, 0};
   const double *tdot = fdot ? s->tdot : nullptr;
   const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
   // atomic form: Ap was zeroed by the previous k_cg_pupdate (or cg_run)
-  int rc = igai::apply_raw(s->op, f64, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE,
-                           /*y_zeroed=*/!gather, fdot ? &hook : nullptr);
+  int rc = igai::apply_raw_dt(s->op, dt, ebe, gather, p, Ap, 0, 1, st, s->ic + I_DONE,
+                              /*y_zeroed=*/!gather, fdot ? &hook : nullptr);
   if (rc) return rc;
   if (!fdot)
     LAUNCH((k_cg_pap<V>), nb, p, Ap, n, s->part, s->sc, s->ic, (const int *)nullptr,
@@ -1088,7 +1154,7 @@ int cg_resume(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, h
 // ---- fused iteration (local preconditioners, single solves)
 bool use_fused(const iga_solver *s, bool f64) {
   return s->o.fused == 2 && local_pc(s) && s->cgs &&
-         igai::cg_fusable(s->op, f64, s->o.kernel == IGA_KERNEL_EBE);
+         igai::cg_fusable_dt(s->op, apply_dt(s, f64), s->o.kernel == IGA_KERNEL_EBE);
 }
 
 // one fused iteration: apply (+ seam kernel when gathering) and k_cgf_update;
@@ -1100,8 +1166,8 @@ int cgf_iteration(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w,
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
   igai::CgHook hook{s->tdot, s->cgs + CG_L, s->cgs, CG_L};
   // scatter-add form: w was cleared by the previous update (or cgf_run)
-  int rc = igai::apply_raw(s->op, f64, ebe, gather, u, w, 0, 1, st, s->ic + I_DONE,
-                           /*y_zeroed=*/!gather, &hook);
+  int rc = igai::apply_raw_dt(s->op, apply_dt(s, f64), ebe, gather, u, w, 0, 1, st,
+                              s->ic + I_DONE, /*y_zeroed=*/!gather, &hook);
   if (rc) return rc;
   const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
   if (s->o.pc == IGA_PC_JACOBI)
@@ -1373,6 +1439,9 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     SA(s->rold32, n * 4);
   }
   if (o.fused < 0 || o.fused > 2) return fail(IGA_ERR_INVALID_ARGUMENT);
+  if (o.inner_f16_geometry &&
+      (o.kind != IGA_SOLVE_PCG_F32_REFINE || !op->G16 || o.kernel == IGA_KERNEL_EBE))
+    return fail(IGA_ERR_UNSUPPORTED);
   if (o.fused) {
     s->ntiles64 = igai::apply_tiles(op, true);
     s->ntiles32 = igai::apply_tiles(op, false);

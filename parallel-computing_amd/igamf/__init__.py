@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libigamf.so")
 
-F64, F32 = 0, 1
+F64, F32, F32_G16 = 0, 1, 2  # F32_G16: FP32 arithmetic on FP16-stored G_q
 SF, EBE = 0, 1
 IEN_ATOMIC, INVLM_GATHER = 0, 1
 GEOM_UNIT_SQUARE, GEOM_QUARTER_ANNULUS, GEOM_PERTURBED_SQUARE = 0, 1, 2
@@ -52,7 +52,7 @@ class MeshParams(C.Structure):
 class OpOpts(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("device", "want_f64", "want_f32", "tile_x_f64",
                                        "tile_y_f64", "tile_x_f32", "tile_y_f32",
-                                       "variant_f64", "variant_f32")]
+                                       "variant_f64", "variant_f32", "f16_geometry")]
 
 
 class OpInfo(C.Structure):
@@ -61,7 +61,9 @@ class OpInfo(C.Structure):
                                        "tile_y_f32")] + [("device_bytes", C.c_int64),
                                                          ("setup_ms", C.c_double),
                                                          ("variant_f64", C.c_int),
-                                                         ("variant_f32", C.c_int)]
+                                                         ("variant_f32", C.c_int),
+                                                         ("f16_geometry", C.c_int),
+                                                         ("f16_scale", C.c_double)]
 
 
 class SolverOpts(C.Structure):
@@ -69,7 +71,8 @@ class SolverOpts(C.Structure):
                 ("rtol", C.c_double), ("abstol", C.c_double), ("divtol", C.c_double),
                 ("maxit", C.c_int), ("inner_rtol", C.c_double), ("inner_maxit", C.c_int),
                 ("pc_inner_maxit", C.c_int), ("pc_inner_rtol", C.c_double),
-                ("pc_cheb_degree", C.c_int), ("use_graph", C.c_int), ("fused", C.c_int)]
+                ("pc_cheb_degree", C.c_int), ("use_graph", C.c_int),
+                ("inner_f16_geometry", C.c_int), ("fused", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -249,12 +252,13 @@ class Operator:
     """The matrix-free operator y = A x on one GPU (iga_op)."""
 
     def __init__(self, mesh, device=0, want_f64=True, want_f32=True, tile_f64=None,
-                 tile_f32=None, kappa=None, variant_f64=-1, variant_f32=-1):
+                 tile_f32=None, kappa=None, variant_f64=-1, variant_f32=-1, f16_geometry=False):
         import torch
         self.mesh = mesh
         o = OpOpts()
         _lib().iga_op_opts_default(C.byref(o))
         o.device, o.want_f64, o.want_f32 = device, int(want_f64), int(want_f32)
+        o.f16_geometry = int(f16_geometry)
         if tile_f64:
             o.tile_x_f64, o.tile_y_f64 = tile_f64
         if tile_f32:
@@ -284,9 +288,19 @@ class Operator:
         return (torch.zeros(self.nfunc, dtype=dtype, device=self.device),
                 torch.zeros(self.nfunc, dtype=dtype, device=self.device))
 
-    def apply(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None):
+    @staticmethod
+    def _dtype(x, g16):
         import torch
         dt = {torch.float64: F64, torch.float32: F32}[x.dtype]
+        if g16:
+            assert dt == F32, "FP16 geometry applies take float32 vectors"
+            dt = F32_G16
+        return dt
+
+    def apply(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None, g16=False):
+        """y = A x (iga_op_apply); g16: FP32 arithmetic on the FP16 G_q."""
+        import torch
+        dt = self._dtype(x, g16)
         assert x.is_cuda and x.is_contiguous() and x.numel() == self.nfunc
         if y is None:
             y = torch.empty_like(x)
@@ -296,11 +310,11 @@ class Operator:
 
     mult = apply  # MATOP_MULT
 
-    def apply_dot(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None):
+    def apply_dot(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None, g16=False):
         """y = A x and x.y (float64 device scalar) from one kernel
         (iga_op_apply_dot)."""
         import torch
-        dt = {torch.float64: F64, torch.float32: F32}[x.dtype]
+        dt = self._dtype(x, g16)
         assert x.is_cuda and x.is_contiguous() and x.numel() == self.nfunc
         if y is None:
             y = torch.empty_like(x)
@@ -326,11 +340,11 @@ class Operator:
               "op_load")
         return F
 
-    def time_kernel(self, x, y, nlaunch=50, kernel=SF, assembly=INVLM_GATHER, stream=None):
+    def time_kernel(self, x, y, nlaunch=50, kernel=SF, assembly=INVLM_GATHER, stream=None,
+                    g16=False):
         """Average ms of the element kernel alone over nlaunch back-to-back
         launches (y receives partial results)."""
-        import torch
-        dt = {torch.float64: F64, torch.float32: F32}[x.dtype]
+        dt = self._dtype(x, g16)
         ms = C.c_double()
         check(_lib().iga_op_time_kernel(self._h, dt, kernel, assembly, x.data_ptr(), y.data_ptr(),
                                         nlaunch, _stream_ptr(stream), C.byref(ms)), "time_kernel")

@@ -125,6 +125,62 @@ def test_apply_dot_unsupported_variant():
     assert abs(xy.item() - float(x @ y)) <= 1e-12 * float(x.abs() @ y.abs())
 
 
+# FP16-stored G_q (unit roundoff 2^-11 per value): the output error is at
+# most 2^-11 times the sum of the contributions' magnitudes, a few times |y|
+TOL16 = 3e-3
+G16_CASES = [c for c in CASES if not c[0].startswith("rational")]
+
+
+@pytest.mark.parametrize("name,kw", G16_CASES, ids=[c[0] for c in G16_CASES])
+def test_f16_geometry_apply(built, name, kw):
+    """IGA_F32_G16: FP32 arithmetic on the FP16 copy of G_q, against the
+    FP64 oracle (TOL16), both assemblies; its x.Ax product; and the FP16
+    operator is still symmetric to FP32 rounding (its perturbation of A is a
+    fixed symmetric one), checked on interior-supported vectors."""
+    m, om, _ = built(name, kw)
+    op = igamf.Operator(m, f16_geometry=True)
+    assert op.info.f16_geometry == 1 and op.info.f16_scale > 0
+    x = igamf.rng_normal(0, op.nfunc)
+    y_ref = O.apply_mf(om, x)
+    x32 = torch.from_numpy(x).cuda().float()
+    for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
+        y = op.apply(x32, assembly=asm, g16=True).double().cpu().numpy()
+        e = rel(y, y_ref)
+        assert e < TOL16, (name, asm, e)
+        if name.startswith("annulus"):
+            # curved geometry: G_q are not FP16 numbers, the rounded copy is used
+            # (on the unit square with 2-point Gauss rules they can be exact)
+            y32 = op.apply(x32, assembly=asm).double().cpu().numpy()
+            assert rel(y, y32) > 1e-5
+        y2, xy = op.apply_dot(x32, assembly=asm, g16=True)
+        e2 = abs(xy.item() - float(x32.double() @ y2.double())) / abs(float(x @ y_ref))
+        assert e2 < 2e-6, (name, asm, e2)
+    mask = torch.from_numpy((om.ID >= 0).astype(np.float32)).cuda()
+    u = torch.from_numpy(igamf.rng_normal(1, op.nfunc)).cuda().float() * mask
+    v = torch.from_numpy(igamf.rng_normal(2, op.nfunc)).cuda().float() * mask
+    au = op.apply(u, g16=True).double()
+    av = op.apply(v, g16=True).double()
+    a, b = float(v.double() @ au), float(u.double() @ av)
+    scale = float(v.double().abs() @ au.abs())
+    assert abs(a - b) <= 1e-5 * scale, (name, a, b)
+    with pytest.raises(igamf.IgaError):
+        op.apply(x32, kernel=igamf.EBE, g16=True)
+
+
+def test_f16_geometry_unsupported():
+    m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, 2, 20, perturb=0.1, w_lo=0.8, w_hi=1.2,
+                         seed=3)
+    with pytest.raises(igamf.IgaError) as e:
+        igamf.Operator(m, f16_geometry=True)  # per-function weights
+    assert e.value.status == 6
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 40)
+    with pytest.raises(igamf.IgaError):
+        igamf.Operator(m, f16_geometry=True, tile_f32=(32, 16))
+    op = igamf.Operator(m)
+    with pytest.raises(igamf.IgaError):
+        op.apply(torch.zeros(op.nfunc, device="cuda"), g16=True)  # no FP16 copy
+
+
 @pytest.mark.parametrize("tile", [(32, 16), (16, 16), (64, 8), (32, 32)])
 def test_pipelined_variant_and_tiles(tile):
     """Persistent pipelined SF kernel, the four-wave FP32 kernel and every

@@ -169,6 +169,35 @@ def test_fused_iteration_matches_classic(geom):
         assert st.status == 0 and st.rel_residual <= 1.0001e-10
 
 
+@pytest.mark.parametrize("asm", [igamf.INVLM_GATHER, igamf.IEN_ATOMIC])
+def test_f16_geometry_inner_solve(asm):
+    """FP32 inner PCG on the FP16-geometry operator under FP64 reliable
+    updates: the FP64 residual removes the operator's 2^-11 perturbation, so
+    the solve reaches rtol 1e-10 and matches the FP64 solve; the inner
+    iteration count stays near the FP32-geometry one."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 64, 50)
+    op = igamf.Operator(m, f16_geometry=True)
+    F = op.load(igamf.LOAD_CONST, 1.0)
+    s64 = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F64, rtol=1e-10, assembly=asm)
+    u64, st64 = s64.solve(F)
+    its = {}
+    for g16 in (0, 1):
+        for fused in (0, 1, 2):
+            s = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F32_REFINE, rtol=1e-10, assembly=asm,
+                             inner_f16_geometry=g16, fused=fused)
+            u, st = s.solve(F)
+            assert st.status == 0 and st.rel_residual <= 1.0001e-10, (g16, fused, st)
+            d = ((u - u64).abs().max() / u64.abs().max()).item()
+            assert d < 1e-8, (g16, fused, d)
+            its[(g16, fused)] = st.iterations
+    assert its[(1, 1)] <= 1.25 * its[(0, 1)], its
+    with pytest.raises(igamf.IgaError):  # FP16 inner operator needs the FP32 inner solve
+        igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F64, inner_f16_geometry=1)
+    with pytest.raises(igamf.IgaError):  # and an op that keeps the FP16 copy
+        igamf.Solver(igamf.Operator(m), None, kind=igamf.SOLVE_PCG_F32_REFINE,
+                     inner_f16_geometry=1)
+
+
 def test_fem_preconditioned_solve():
     """The reference's FEM preconditioner (inner CG + Jacobi on the Q1 matrix):
     converged solutions agree with Jacobi-PCG, and its iteration count stays
