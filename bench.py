@@ -151,7 +151,7 @@ def headline(a, out):
         minw = 4 if (variant == 2 and a.kernel == "sf") else 1
         split = 2 if (variant == 3 and a.kernel == "sf") else 1
         ebe = "true" if a.kernel == "ebe" else "false"
-        kname = f"k_apply<float, {p}, {p + 1}, {tx}, {ty}, {ebe}, false, {minw}, {split}>"
+        kname = f"k_apply<float, {p}, {p + 1}, {tx}, {ty}, {ebe}, false, {minw}, {split}, float>"
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
   __shared__ float xs[RC * NFP];
   __shared__ float tbx[2 * TAB][TX];
   __shared__ float tby[2 * TAB][TY];
-  __shared__ float red[2][2 * NB * NT];
+  __shared__ float red[2][row16_lds<P, TY>()];
 
   const int t = threadIdx.x, lx = t % TX, ly = t / TX;
   // the rhs chunks of one tile are consecutive logical ids, which xcd_tile
@@ -96,6 +96,8 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
   oa.mode = 0;
   oa.mask_out = 1;
   auto gl = [&](int c, int q, int, int) { return g[c][q]; };
+  // block tile with neighbours on all sides, ring Dirichlet set (see put_footprint)
+  const bool interior = bx > 0 && by > 0 && bx < A.nbx - 1 && by < A.nby - 1 && !A.dirmask;
   const int nr = (A.k - j0) < RC ? (A.k - j0) : RC;
   for (int r = 0; r < nr; ++r) {
     float ye[NB][NB];
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
     double dnone = 0.0;
     reduce_store_row16<float, P, TY, false>(oa, ye, red[r & 1], t, 0, ex0, ey0,
                                             A.Y + (size_t)(j0 + r) * A.ld, nullptr, nullptr,
-                                            xs, dnone);
+                                            xs, dnone, interior);
   }
 }
 

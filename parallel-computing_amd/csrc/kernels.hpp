@@ -421,12 +421,29 @@ struct TileStage {
 // meshes scale by the function's weight.
 // xin: the staged input at this footprint point (x, or w x for general
 // weights), accumulated into dacc as xin * v when A.dot is set.
+// interior: the tile has neighbours on all four sides and the Dirichlet set
+// is the boundary ring (or rows are not masked), so no footprint function
+// is Dirichlet or outside the mesh and a function is shared exactly when it
+// lies in the first P or the last P footprint columns / rows.
 template <typename T, int P, int TX, int TY, bool RAT>
 __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo, const T *wf,
                                               int tile, int ex0, int ey0, int fx, int fy, T v,
-                                              T xin, double &dacc) {
+                                              T xin, double &dacc, bool interior) {
   constexpr int FX = TX + P;
   const int gx = ex0 + fx, gy = ey0 + fy;
+  if (interior) {
+    const int Ag = gy * A.nfx + gx;
+    const bool shared = fx < P || fx >= TX || fy < P || fy >= TY;
+    if (A.dot) dacc += (double)xin * (double)v;
+    if (RAT) v *= wf[Ag];
+    if (!shared)
+      y[Ag] = v;
+    else if (A.mode == 0)
+      atomic_add(&y[Ag], v);
+    else
+      halo[(size_t)tile * (FX * (TY + P)) + fy * FX + fx] = v;
+    return;
+  }
   if (gx >= A.nfx || gy >= A.nfy) return;
   int xlo, xhi, ylo, yhi;
   touching_tiles<P>(gx, A.nex, TX, xlo, xhi);
@@ -481,13 +498,22 @@ __device__ __forceinline__ void row16_x(const T (&ye)[P + 1][P + 1], T (&rin)[P 
   }
 }
 
+// LDS values reduce_store_row16 needs for a 16 x TY tile: element-row sums
+// of the tile's own footprint columns (bin) and of the P columns past its
+// right edge (bwr), each with P zero rows above and below so that the
+// column sums read no out-of-tile rows conditionally.
+template <int P, int TY>
+constexpr int row16_lds() {
+  return 2 * (P + 1) * (TY + 2 * P) * 16;
+}
+
 template <typename T, int P, int TY, bool RAT>
 __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (&ye)[P + 1][P + 1],
                                                    T *lds, int t, int tile, int ex0, int ey0,
                                                    T *y, T *halo, const T *wf, const T *xs,
-                                                   double &dacc) {
+                                                   double &dacc, bool interior) {
   constexpr int FX = 16 + P;
-  constexpr int NB = P + 1, NT = 16 * TY;
+  constexpr int NB = P + 1, NT = 16 * TY, R = (TY + 2 * P) * 16;  // values per b
   const int lx = t & 15, ly = t >> 4;
   T rin[NB], rwr[NB];
 #pragma unroll
@@ -495,39 +521,45 @@ __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (
     rin[b] = ye[b][0];
     rwr[b] = T(0);
   }
-  row16_x<P, 1>(ye, rin, rwr);
-  T *bin = lds, *bwr = lds + NB * NT;
+  row16_x<P, 1>(ye, rin, rwr);  // rwr is zero on lanes lx >= P
+  T *bin = lds, *bwr = lds + NB * R;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    bin[b * NT + t] = rin[b];
-    if (lx < P) bwr[b * NT + t] = rwr[b];
+    bin[b * R + (ly + P) * 16 + lx] = rin[b];
+    bwr[b * R + (ly + P) * 16 + lx] = rwr[b];
+  }
+  // the zero rows: 2 buffers x NB x 2P rows x 16
+  for (int i = t; i < 2 * NB * 2 * P * 16; i += NT) {
+    const int l = i & 15, r = (i >> 4) % (2 * P), u = (i >> 4) / (2 * P);
+    const int row = r < P ? r : TY + r;  // rows -P..-1 and TY..TY+P-1, shifted by P
+    lds[u * R + row * 16 + l] = T(0);
   }
   __syncthreads();
   // rows 0..TY-1 of this thread's column, and rows TY..TY+P-1 on ly < P
-  T v0 = T(0), v1 = T(0), w0 = T(0), w1 = T(0);
+  T v0 = T(0), w0 = T(0);
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    if (b <= ly) {
-      v0 += bin[b * NT + (ly - b) * 16 + lx];
-      if (lx < P) w0 += bwr[b * NT + (ly - b) * 16 + lx];
-    }
-    if (ly < P && b > ly) {
-      v1 += bin[b * NT + (TY + ly - b) * 16 + lx];
-      if (lx < P) w1 += bwr[b * NT + (TY + ly - b) * 16 + lx];
-    }
+    v0 += bin[b * R + (ly - b + P) * 16 + lx];
+    w0 += bwr[b * R + (ly - b + P) * 16 + lx];
   }
   auto xin = [&](int fx, int fy) { return A.dot ? xs[fy * FX + fx] : T(0); };
   put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, ly, v0, xin(lx, ly),
-                                   dacc);
+                                   dacc, interior);
   if (lx < P)
     put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, ly, w0,
-                                     xin(16 + lx, ly), dacc);
+                                     xin(16 + lx, ly), dacc, interior);
   if (ly < P) {
+    T v1 = T(0), w1 = T(0);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      v1 += bin[b * R + (TY + ly - b + P) * 16 + lx];
+      w1 += bwr[b * R + (TY + ly - b + P) * 16 + lx];
+    }
     put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, TY + ly, v1,
-                                     xin(lx, TY + ly), dacc);
+                                     xin(lx, TY + ly), dacc, interior);
     if (lx < P)
       put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, TY + ly, w1,
-                                       xin(16 + lx, TY + ly), dacc);
+                                       xin(16 + lx, TY + ly), dacc, interior);
   }
 }
 
@@ -591,7 +623,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   __shared__ T xs[NFP];
   __shared__ T tbx[2 * TAB][TX];
   __shared__ T tby[2 * TAB][TY];
-  __shared__ T bufm[(ROW16 ? 2 : 1) * NB * NE];
+  __shared__ T bufm[ROW16 ? row16_lds<P, TY>() : NB * NE];
   T(*buf)[NE] = reinterpret_cast<T(*)[NE]>(bufm);
 
   if (A.skip && *A.skip) return;
@@ -601,6 +633,11 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   const int tx = tile % A.ntx, ty = tile / A.ntx;
   const int ex0 = tx * TX, ey0 = ty * TY;
   const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
+  // uniform over the workgroup: every element of the tile exists; the tile is
+  // interior (see put_footprint)
+  const bool full = ex0 + TX <= A.nex && ey0 + TY <= A.ney;
+  const bool interior = tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
+                        (!A.mask_out || !A.dirmask);
 
   // x footprint and tables first, then the geometry factors of this element
   // (held in registers unless they are many: high degree, general rational)
@@ -665,7 +702,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
 #pragma unroll
         for (int a = 0; a < NB; ++a) ye[b][a] *= gs;
     }
-    if (!valid) {
+    if (!full && !valid) {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -677,7 +714,8 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   T *halo = static_cast<T *>(A.halo);
   double dacc = 0.0;
   if constexpr (ROW16) {
-    reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, tile, ex0, ey0, y, halo, wf, xs, dacc);
+    reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, tile, ex0, ey0, y, halo, wf, xs, dacc,
+                                      interior);
     if (A.dot) store_tile_dot<NT>(A, tile, dacc);
     return;
   }
@@ -715,7 +753,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
     const int k = t + o * NT;
     if (k < NFP)
       put_footprint<T, P, TX, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, k % FX, k / FX, acc[o],
-                                       A.dot ? xs[k] : T(0), dacc);
+                                       A.dot ? xs[k] : T(0), dacc, interior);
   }
   if (A.dot) store_tile_dot<NT>(A, tile, dacc);
 }
@@ -732,7 +770,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
   __shared__ T xs[NFP];
   __shared__ T tbx[2 * TAB][TX];
   __shared__ T tby[2 * TAB][TY];
-  __shared__ T bufm[(TX == 16 ? 2 : 1) * NB * NT];
+  __shared__ T bufm[TX == 16 ? row16_lds<P, TY>() : NB * NT];
   T(*buf)[NT] = reinterpret_cast<T(*)[NT]>(bufm);
 
   if (A.skip && *A.skip) return;
@@ -757,6 +795,8 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
     const int tx = tile % A.ntx, ty = tile / A.ntx;
     const int ex0 = tx * TX, ey0 = ty * TY;
     const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
+    const bool interior = tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
+                          (!A.mask_out || !A.dirmask);
     __syncthreads();  // previous tile's LDS reads are done
     stn.store(A, ex0, ey0, t, xs, tbx, tby);
     T g[3][NQ2];
@@ -787,7 +827,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
     double dnone = 0.0;  // A.dot is null here (no CG fusion in this variant)
     if constexpr (TX == 16) {
       reduce_store_row16<T, P, TY, false>(A, ye, bufm, t, tile, ex0, ey0, y, halo, nullptr, xs,
-                                          dnone);
+                                          dnone, interior);
       continue;
     }
     T acc[NOWN];
@@ -820,7 +860,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
       const int k = t + o * NT;
       if (k < NFP)
         put_footprint<T, P, TX, TY, false>(A, y, halo, nullptr, tile, ex0, ey0, k % FX, k / FX,
-                                           acc[o], T(0), dnone);
+                                           acc[o], T(0), dnone, interior);
     }
   }
 }
