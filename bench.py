@@ -57,6 +57,13 @@ def parse():
                     help="run a 256 MiB device copy first (PMC byte-counter calibration)")
     ap.add_argument("--only-apply", action="store_true",
                     help="headline apply only (for profiler runs)")
+    ap.add_argument("--also-time", default="",
+                    help="with --only-apply: also run the element kernel of these forms "
+                         "back to back (comma-separated: g16, f64), for profiler runs")
+    ap.add_argument("--solve-configs", default="",
+                    help="comma-separated solve keys to run (e.g. fp64_atomic_fused1); "
+                         "default all")
+    ap.add_argument("--solve-reps", type=int, default=3, help="timed solves per config")
     return ap.parse_args()
 
 
@@ -241,7 +248,7 @@ def fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, nfunc, E, nqp):
         out["apply"]["variants"] = var
 
 
-def solves(out, op):
+def solves(out, op, only=(), reps=3):
     """Plain FP64 PCG against FP32 PCG + FP64 reliable updates, Jacobi, f = 1,
     to 1e-8 relative residual (FP64 residual check), for both assemblies and
     the three iteration forms (fused = 0: separate p.Ap kernel; 1: p.Ap inside
@@ -257,21 +264,26 @@ def solves(out, op):
     for an, av in (("gather", igamf.INVLM_GATHER), ("atomic", igamf.IEN_ATOMIC)):
         for fused in (0, 1, 2):
             for name, kind, g16 in cases:
+                key = f"{name}_{an}_fused{fused}"
+                if only and key not in only:
+                    continue
                 s = igamf.Solver(op, None, kind=kind, pc=igamf.PC_JACOBI, rtol=1e-8,
                                  assembly=av, maxit=200000, fused=fused, inner_f16_geometry=g16)
                 s.solve(F)  # first call captures the CG chunk graph
                 ts = []
-                for _ in range(3):
+                for _ in range(reps):
                     u, st = s.solve(F)
                     ts.append(st.solve_ms)
                 torch.cuda.synchronize()
-                key = f"{name}_{an}_fused{fused}"
                 res[key] = {"solve_ms": round(median(ts), 2), "iterations": st.iterations,
                             "reliable_updates": st.outer_iterations,
                             "us_per_iteration": round(median(ts) * 1e3 / st.iterations, 1),
                             "rel_residual": st.rel_residual, "status": st.status}
                 sols[key] = u
                 log(f"solve {key}: {median(ts):.1f} ms, {st.iterations} its")
+    if only:
+        out["solve"] = res
+        return
     ref = sols["fp64_gather_fused0"]
     for k in sols:
         if k != "fp64_gather_fused0":
@@ -479,10 +491,17 @@ def main():
         sys.exit(2)
     out = {}
     mesh, op, x64, x32, kern, asm, n_int = headline(a, out)
+    if a.only_apply and a.also_time:
+        import torch
+        for form in a.also_time.split(","):
+            if form == "g16":
+                op.time_kernel(x32, torch.empty_like(x32), a.steps, kern, asm, g16=True)
+            elif form == "f64":
+                op.time_kernel(x64, torch.empty_like(x64), a.steps, kern, asm)
     if not a.only_apply:
         fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, op.nfunc, op.nelem, op.nqp)
         if not a.no_solve:
-            solves(out, op)
+            solves(out, op, tuple(x for x in a.solve_configs.split(",") if x), a.solve_reps)
         if not a.no_cpu:
             cpu_baseline(a, out, mesh, x64, op, kern, asm, n_int)
         del op

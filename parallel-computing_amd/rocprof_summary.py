@@ -2,6 +2,12 @@
 
     python parallel-computing_amd/rocprof_summary.py stats RUN.db OUT.txt --cmd "..."
         per-kernel dispatch count and average / min / max / total duration
+    python parallel-computing_amd/rocprof_summary.py gaps RUN.db OUT.txt --cmd "..."
+        per kernel: average duration and average idle time since the previous
+        dispatch ended (graph-replayed iteration loops)
+    python parallel-computing_amd/rocprof_summary.py counters RUN.db OUT.txt --cmd "..."
+        per kernel: average of every collected counter per dispatch, and per
+        wave for SQ_INSTS_* when SQ_WAVES was collected
     python parallel-computing_amd/rocprof_summary.py traffic FETCH.db WRITE.db OUT.json \\
         --kernel NAME --copy-bytes B --cmd "..." [--p P --nelem N]
         HBM bytes per launch of the kernel whose name contains NAME, from
@@ -52,6 +58,54 @@ def stats(a):
     print("\n".join(lines[:14]))
 
 
+def kernel_spans(db):
+    cur = sqlite3.connect(db).cursor()
+    cols = [r[1] for r in cur.execute("pragma table_info(kernels)")]
+    name, start, end = cols.index("name"), cols.index("start"), cols.index("end")
+    return sorted((r[start], r[end], r[name]) for r in cur.execute("select * from kernels"))
+
+
+def gaps(a):
+    rows = kernel_spans(a.db)
+    dur = collections.defaultdict(list)
+    gap = collections.defaultdict(list)
+    for (s0, e0, _), (s1, e1, n1) in zip(rows, rows[1:]):
+        g = (s1 - e0) / 1e3
+        if g < 50.0:  # within a replayed loop, not a host round trip
+            dur[n1].append((e1 - s1) / 1e3)
+            gap[n1].append(g)
+    lines = ["# rocprofv3 --kernel-trace: per kernel, average duration and idle time "
+             "before it (gaps under 50 us)", f"# command: {a.cmd}", "",
+             f"{'calls':>6} {'avg_us':>10} {'gap_us':>8}  kernel"]
+    for n in sorted(dur, key=lambda k: -sum(dur[k])):
+        v, g = dur[n], gap[n]
+        lines.append(f"{len(v):6d} {sum(v) / len(v):10.2f} {sum(g) / len(g):8.2f}  "
+                     f"{short(n.replace('(anonymous namespace)::', '').split('(')[0].replace('void ', ''), 90)}")
+    open(a.out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines[:20]))
+
+
+def counters(a):
+    names, vals = counter_rows(a.db)
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d, n in names.items():
+        for c, v in vals[d].items():
+            per[n][c].append(v)
+    lines = ["# rocprofv3 --pmc: per kernel, counter averages per dispatch (per wave for "
+             "SQ_INSTS_*)", f"# command: {a.cmd}", ""]
+    for n in sorted(per):
+        cs = per[n]
+        waves = sum(cs["SQ_WAVES"]) / len(cs["SQ_WAVES"]) if "SQ_WAVES" in cs else 0.0
+        lines.append(short(n.replace("(anonymous namespace)::", "").split("(")[0]
+                           .replace("void ", ""), 110))
+        for c in sorted(cs):
+            avg = sum(cs[c]) / len(cs[c])
+            extra = f"  ({avg / waves:.1f} per wave)" if waves and c.startswith("SQ_INSTS") else ""
+            lines.append(f"    {c:24s} {avg:14.6g}  [{len(cs[c])} dispatches]{extra}")
+    open(a.out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def pick(names, vals, counter, pred):
     return [(d, vals[d][counter]) for d, n in names.items() if pred(n) and counter in vals[d]]
 
@@ -89,6 +143,14 @@ def main():
     s.add_argument("db")
     s.add_argument("out")
     s.add_argument("--cmd", default="")
+    c = sub.add_parser("counters")
+    c.add_argument("db")
+    c.add_argument("out")
+    c.add_argument("--cmd", default="")
+    g = sub.add_parser("gaps")
+    g.add_argument("db")
+    g.add_argument("out")
+    g.add_argument("--cmd", default="")
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
@@ -99,7 +161,7 @@ def main():
     t.add_argument("--p", type=int, default=0)
     t.add_argument("--nelem", type=int, default=0)
     a = ap.parse_args()
-    stats(a) if a.what == "stats" else traffic(a)
+    {"stats": stats, "gaps": gaps, "counters": counters, "traffic": traffic}[a.what](a)
 
 
 if __name__ == "__main__":
