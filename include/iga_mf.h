@@ -298,10 +298,16 @@ typedef enum {
   IGA_PC_JACOBI = 1,     /* diagonal of A */
   IGA_PC_FEM_CG = 2,     /* inner CG + Jacobi on the Q1 FEM matrix
                             (matrix_free_gpu/matrixfreegpu.cu:18-24,273-289) */
-  IGA_PC_FEM_CHEB = 3    /* fixed-degree Chebyshev-Jacobi polynomial of the Q1
+  IGA_PC_FEM_CHEB = 3,   /* fixed-degree Chebyshev-Jacobi polynomial of the Q1
                             matrix (a linear preconditioner; interval
                             [lmax/30, lmax], lmax the Gershgorin bound of
                             D^-1 K); no counterpart in the reference */
+  IGA_PC_FEM_ILU = 4     /* ILU(0) of the Q1 matrix in natural ordering (the
+                            -pc_type ilu of the reference's plain-method runs,
+                            PETSc PCILU with 0 levels), factored on the host
+                            at set-up; triangular solves level-scheduled in
+                            one workgroup, so meant for small meshes (tens of
+                            thousands of functions) */
 } iga_pc_kind;
 
 typedef enum {

@@ -323,6 +323,47 @@ def invlm(nloc, nelem, nfunc, IEN):
     return en, off, ei, bi
 
 
+# --------------------------------------------------------------------------- ILU(0)
+def ilu0(rowptr, col, val):
+    """ILU(0) in natural ordering, the factorisation of PETSc's PCILU with 0
+    levels of fill (PETSc 3.18.6, matrix_free/CMakeLists.txt:55; selected by
+    the plain-method runs' -pc_type ilu): for each row i and each k < i in its
+    pattern, l_ik = a_ik / u_kk, then a_ij -= l_ik u_kj for every j > k that
+    both rows hold.  Pure Python loops: small matrices only.  Returns the
+    factor values in the matrix's pattern (unit L below the diagonal, U on and
+    above it) and the diagonal positions."""
+    rowptr = np.asarray(rowptr); col = np.asarray(col)
+    a = np.array(val, dtype=np.float64)
+    n = len(rowptr) - 1
+    diag = np.full(n, -1, np.int64)
+    pos = []
+    for i in range(n):
+        d = {int(col[e]): e for e in range(rowptr[i], rowptr[i + 1])}
+        pos.append(d)
+        diag[i] = d[i]
+    for i in range(n):
+        for e in range(rowptr[i], diag[i]):
+            k = int(col[e])
+            a[e] /= a[diag[k]]
+            for f in range(diag[k] + 1, rowptr[k + 1]):
+                g = pos[i].get(int(col[f]))
+                if g is not None:
+                    a[g] -= a[e] * a[f]
+    return a, diag
+
+
+def ilu0_solve(rowptr, col, lu, diag, r):
+    """z = (L U)^-1 r for the factors of ilu0 (sequential triangular solves)."""
+    n = len(rowptr) - 1
+    z = np.array(r, dtype=np.float64)
+    for i in range(n):
+        z[i] -= sum(lu[e] * z[col[e]] for e in range(rowptr[i], diag[i]))
+    for i in range(n - 1, -1, -1):
+        z[i] = (z[i] - sum(lu[e] * z[col[e]] for e in range(diag[i] + 1, rowptr[i + 1]))) \
+            / lu[diag[i]]
+    return z
+
+
 # --------------------------------------------------------------------------- reference units
 _ref_mf = None
 _ref_gpu = None

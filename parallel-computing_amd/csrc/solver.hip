@@ -776,6 +776,37 @@ __global__ void k_spmv(const int *rowptr, const int *col, const V *val, const V 
   y[i] = (V)s;
 }
 
+// z = (L U)^-1 r for the ILU(0) factors in CSR (L unit lower, U upper with
+// its diagonal; the factors share the matrix's pattern), one workgroup:
+// rows of a level depend only on rows of earlier levels, a barrier between
+// levels.  Forward levels fptr/frows, backward bptr/brows.
+template <typename V>
+__global__ __launch_bounds__(1024) void k_ilu_solve(
+    const int *__restrict__ rp, const int *__restrict__ ci, const V *__restrict__ lu,
+    const int *__restrict__ diag, const int *__restrict__ fptr, const int *__restrict__ frows,
+    int nfl, const int *__restrict__ bptr, const int *__restrict__ brows, int nbl,
+    const V *__restrict__ r, V *z, const int *skip) {
+  if (skip && *skip) return;
+  for (int l = 0; l < nfl; ++l) {
+    for (int k = fptr[l] + (int)threadIdx.x; k < fptr[l + 1]; k += (int)blockDim.x) {
+      const int i = frows[k];
+      double s = (double)r[i];
+      for (int e = rp[i]; e < diag[i]; ++e) s -= (double)lu[e] * (double)z[ci[e]];
+      z[i] = (V)s;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < nbl; ++l) {
+    for (int k = bptr[l] + (int)threadIdx.x; k < bptr[l + 1]; k += (int)blockDim.x) {
+      const int i = brows[k];
+      double s = (double)z[i];
+      for (int e = diag[i] + 1; e < rp[i + 1]; ++e) s -= (double)lu[e] * (double)z[ci[e]];
+      z[i] = (V)(s / (double)lu[diag[i]]);
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_csr_diag_inv(const int *rowptr, const int *col, const double *val, double *d,
                                int nrows) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -887,6 +918,73 @@ inline int nblocks(int64_t n) {
   return (int)b;
 }
 
+// ILU(0) in natural ordering (PETSc PCILU, levels 0): for each row i and each
+// k < i in its pattern, l_ik = a_ik / u_kk, then a_ij -= l_ik u_kj for the
+// j > k of row i that row k also holds.  Columns sorted per row.  Returns the
+// position of each diagonal in diag; false on a missing or zero pivot.
+bool ilu0_factor(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                 std::vector<double> &a, std::vector<int> &diag) {
+  diag.assign(n, -1);
+  for (int i = 0; i < n; ++i)
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+      if (ci[e] == i) diag[i] = e;
+  for (int i = 0; i < n; ++i) {
+    if (diag[i] < 0) return false;
+    for (int e = rp[i]; e < diag[i]; ++e) {
+      const int k = ci[e];
+      const double ukk = a[diag[k]];
+      if (ukk == 0.0) return false;
+      const double l = a[e] / ukk;
+      a[e] = l;
+      // row i entries right of k against row k entries right of its diagonal
+      int f = diag[k] + 1;
+      for (int g = e + 1; g < rp[i + 1] && f < rp[k + 1];) {
+        if (ci[g] == ci[f]) {
+          a[g] -= l * a[f];
+          ++g;
+          ++f;
+        } else if (ci[g] < ci[f]) {
+          ++g;
+        } else {
+          ++f;
+        }
+      }
+    }
+    if (a[diag[i]] == 0.0) return false;
+  }
+  return true;
+}
+
+// level sets of the triangular solves: forward, a row after the rows left of
+// its diagonal; backward, after the rows right of it
+void ilu0_levels(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                 const std::vector<int> &diag, bool forward, std::vector<int> &ptr,
+                 std::vector<int> &rows) {
+  std::vector<int> lev(n, 0);
+  int nl = 0;
+  if (forward) {
+    for (int i = 0; i < n; ++i) {
+      int l = 0;
+      for (int e = rp[i]; e < diag[i]; ++e) l = std::max(l, lev[ci[e]] + 1);
+      lev[i] = l;
+      nl = std::max(nl, l + 1);
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      int l = 0;
+      for (int e = diag[i] + 1; e < rp[i + 1]; ++e) l = std::max(l, lev[ci[e]] + 1);
+      lev[i] = l;
+      nl = std::max(nl, l + 1);
+    }
+  }
+  ptr.assign(nl + 1, 0);
+  for (int i = 0; i < n; ++i) ++ptr[lev[i] + 1];
+  for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+  rows.assign(n, 0);
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
+}
+
 }  // namespace
 
 struct iga_solver {
@@ -916,6 +1014,13 @@ struct iga_solver {
   double *rin = nullptr, *w = nullptr, *pp = nullptr, *q = nullptr;
   float *rin32 = nullptr, *w32 = nullptr, *pp32 = nullptr, *q32 = nullptr;
   double cheb_lmax = 0.0;  // Gershgorin bound of D^-1 K (FEM_CHEB)
+  // ILU(0) factors of K (FEM_ILU): values in K's pattern, diagonal
+  // positions, level sets of the two triangular solves
+  double *ilu64 = nullptr;
+  float *ilu32 = nullptr;
+  int *ilu_diag = nullptr, *ilu_fptr = nullptr, *ilu_frows = nullptr, *ilu_bptr = nullptr,
+      *ilu_brows = nullptr;
+  int ilu_nfl = 0, ilu_nbl = 0;
   std::vector<void *> allocs;
   int gen = -1;  // operator generation the Jacobi diagonal was built for
   // graphs of one chunk of iterations (FP64 PCG, FP32 inner PCG)
@@ -1012,6 +1117,19 @@ int cheb_pc(iga_solver *s, const V *r, V *z, V *d, V *q, const V *val, hipStream
 
 template <typename V>
 int apply_pc(iga_solver *s, bool f64, const V *r, V *z, hipStream_t st) {
+  if (s->o.pc == IGA_PC_FEM_ILU) {
+    if (f64)
+      hipLaunchKernelGGL((k_ilu_solve<double>), dim3(1), dim3(1024), 0, st, s->rowptr, s->col,
+                         s->ilu64, s->ilu_diag, s->ilu_fptr, s->ilu_frows, s->ilu_nfl,
+                         s->ilu_bptr, s->ilu_brows, s->ilu_nbl, (const double *)r, (double *)z,
+                         s->ic + I_DONE);
+    else
+      hipLaunchKernelGGL((k_ilu_solve<float>), dim3(1), dim3(1024), 0, st, s->rowptr, s->col,
+                         s->ilu32, s->ilu_diag, s->ilu_fptr, s->ilu_frows, s->ilu_nfl,
+                         s->ilu_bptr, s->ilu_brows, s->ilu_nbl, (const float *)r, (float *)z,
+                         s->ic + I_DONE);
+    return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+  }
   if (s->o.pc == IGA_PC_FEM_CHEB) {
     if (f64)
       return cheb_pc<double>(s, (const double *)r, (double *)z, s->pp, s->q, s->val64, st);
@@ -1389,9 +1507,10 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   if (o_in) o = *o_in; else iga_solver_opts_default(&o);
   if (o.kind != IGA_SOLVE_PCG_F64 && o.kind != IGA_SOLVE_PCG_F32_REFINE) return IGA_ERR_INVALID_ARGUMENT;
   if (o.pc != IGA_PC_NONE && o.pc != IGA_PC_JACOBI && o.pc != IGA_PC_FEM_CG &&
-      o.pc != IGA_PC_FEM_CHEB)
+      o.pc != IGA_PC_FEM_CHEB && o.pc != IGA_PC_FEM_ILU)
     return IGA_ERR_INVALID_ARGUMENT;
-  const bool fem_pc_kind = o.pc == IGA_PC_FEM_CG || o.pc == IGA_PC_FEM_CHEB;
+  const bool fem_pc_kind =
+      o.pc == IGA_PC_FEM_CG || o.pc == IGA_PC_FEM_CHEB || o.pc == IGA_PC_FEM_ILU;
   if (fem_pc_kind && !fem) return IGA_ERR_INVALID_ARGUMENT;
   if (o.pc == IGA_PC_FEM_CHEB && o.pc_cheb_degree < 1) return IGA_ERR_INVALID_ARGUMENT;
   if (o.kind == IGA_SOLVE_PCG_F32_REFINE && !op->G32) return IGA_ERR_UNSUPPORTED;
@@ -1483,6 +1602,40 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
         return fail(IGA_ERR_HIP);
       for (double x : hp) s->cheb_lmax = std::max(s->cheb_lmax, x);
       if (!(s->cheb_lmax > 0.0)) return fail(IGA_ERR_BAD_MESH);
+    }
+    if (o.pc == IGA_PC_FEM_ILU) {
+      const int nr = v.nrows;
+      std::vector<int> hrp(nr + 1), hci(v.nnz), hdiag, fptr, frows, bptr, brows;
+      std::vector<double> hlu(v.nnz);
+      if (hipMemcpy(hrp.data(), v.rowptr, (nr + 1) * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(hci.data(), v.col, v.nnz * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(hlu.data(), v.val, v.nnz * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(IGA_ERR_HIP);
+      if (!ilu0_factor(nr, hrp, hci, hlu, hdiag)) return fail(IGA_ERR_BAD_MESH);
+      ilu0_levels(nr, hrp, hci, hdiag, true, fptr, frows);
+      ilu0_levels(nr, hrp, hci, hdiag, false, bptr, brows);
+      s->ilu_nfl = (int)fptr.size() - 1;
+      s->ilu_nbl = (int)bptr.size() - 1;
+      SA(s->ilu64, (size_t)v.nnz * 8);
+      SA(s->ilu_diag, (size_t)nr * 4);
+      SA(s->ilu_fptr, fptr.size() * 4);
+      SA(s->ilu_frows, (size_t)nr * 4);
+      SA(s->ilu_bptr, bptr.size() * 4);
+      SA(s->ilu_brows, (size_t)nr * 4);
+      if (hipMemcpy(s->ilu64, hlu.data(), v.nnz * 8, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(s->ilu_diag, hdiag.data(), nr * 4, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(s->ilu_fptr, fptr.data(), fptr.size() * 4, hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          hipMemcpy(s->ilu_frows, frows.data(), nr * 4, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(s->ilu_bptr, bptr.data(), bptr.size() * 4, hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          hipMemcpy(s->ilu_brows, brows.data(), nr * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(IGA_ERR_HIP);
+      if (o.kind == IGA_SOLVE_PCG_F32_REFINE) {
+        SA(s->ilu32, (size_t)v.nnz * 4);
+        hipLaunchKernelGGL((k_cast<float>), dim3(nblocks(v.nnz)), dim3(RB), 0, nullptr, s->ilu64,
+                           s->ilu32, (int64_t)v.nnz);
+      }
     }
   }
 #undef SA

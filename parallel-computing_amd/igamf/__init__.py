@@ -21,7 +21,7 @@ F64, F32, F32_G16 = 0, 1, 2  # F32_G16: FP32 arithmetic on FP16-stored G_q
 SF, EBE = 0, 1
 IEN_ATOMIC, INVLM_GATHER = 0, 1
 GEOM_UNIT_SQUARE, GEOM_QUARTER_ANNULUS, GEOM_PERTURBED_SQUARE = 0, 1, 2
-PC_NONE, PC_JACOBI, PC_FEM_CG, PC_FEM_CHEB = 0, 1, 2, 3
+PC_NONE, PC_JACOBI, PC_FEM_CG, PC_FEM_CHEB, PC_FEM_ILU = 0, 1, 2, 3, 4
 SOLVE_PCG_F64, SOLVE_PCG_F32_REFINE = 0, 1
 LOAD_REF_POLY, LOAD_CONST, LOAD_SINSIN = 0, 1, 2
 IGA_ERR_DIVERGED, IGA_ERR_NOT_CONVERGED = 10, 11

@@ -408,3 +408,36 @@ def test_oracle_matches_live_reference(p, n):
     assert rel(O.apply_mf(m, x), O.ref_apply_mf(m, x)) < 1e-13
     assert rel(O.apply_sf_ref(m, x), O.ref_apply_sf(m, x)) < 1e-13
     assert rel(O.load(m, O.LOAD_REF_POLY), O.ref_load(m)) < 1e-13
+
+
+def test_ilu0_invariants():
+    """ILU(0) of the oracle's Q1 FEM matrix: (L U)_ij = A_ij on the pattern
+    (the defining property of zero-fill ILU), and on a tridiagonal matrix,
+    where no fill arises, L U is the exact LU (scipy.linalg.lu_factor)."""
+    import scipy.linalg as sla
+    import scipy.sparse as sp
+    p, n = 2, 6
+    nf = n + p
+    rows, cols, e2c, d2c, val = O.fem_assemble(nf, nf, O.fem_nodes(p, p, n, n))
+    N = nf * nf
+    A = sp.csr_matrix((val, (rows, cols)), shape=(N, N))
+    A.sort_indices()
+    lu, diag = O.ilu0(A.indptr, A.indices, A.data)
+    F = sp.csr_matrix((lu, A.indices, A.indptr), shape=(N, N)).toarray()
+    L = np.tril(F, -1) + np.eye(N)
+    U = np.triu(F)
+    LU = L @ U
+    pat = A.toarray() != 0
+    assert np.abs(LU[pat] - A.toarray()[pat]).max() < 1e-12 * np.abs(val).max()
+    # the solve inverts L U
+    r = np.random.default_rng(0).standard_normal(N)
+    z = O.ilu0_solve(A.indptr, A.indices, lu, diag, r)
+    assert np.abs(LU @ z - r).max() < 1e-10 * np.abs(r).max()
+    # tridiagonal: exact
+    m = 12
+    T = sp.diags([-np.ones(m - 1), 2.5 * np.ones(m), -1.3 * np.ones(m - 1)], [-1, 0, 1]).tocsr()
+    T.sort_indices()
+    lu2, diag2 = O.ilu0(T.indptr, T.indices, T.data)
+    b = np.arange(1.0, m + 1)
+    x = O.ilu0_solve(T.indptr, T.indices, lu2, diag2, b)
+    assert np.abs(x - sla.lu_solve(sla.lu_factor(T.toarray()), b)).max() < 1e-13 * np.abs(x).max()

@@ -226,6 +226,54 @@ def test_fem_preconditioned_solve():
     assert its[(igamf.SOLVE_PCG_F64, 128)] < its[("jacobi", 128)], its
 
 
+def _pcg_count(A, F, prec, rtol):
+    """Iterations of plain PCG (numpy, FP64, zero start) to ||r|| <= rtol ||F||."""
+    x = np.zeros_like(F); r = F.copy(); z = prec(r); p = z.copy(); rz = r @ z
+    nf = np.linalg.norm(F)
+    for it in range(1, 10000):
+        Ap = A @ p
+        a = rz / (p @ Ap)
+        x += a * p; r -= a * Ap
+        if np.linalg.norm(r) <= rtol * nf:
+            return it, x
+        z = prec(r); rzn = r @ z; p = z + (rzn / rz) * p; rz = rzn
+    return -1, x
+
+
+def test_fem_ilu_preconditioned_solve():
+    """ILU(0) of the Q1 FEM matrix as the preconditioner (the plain-method
+    configuration: p = 2, 32^2, rtol 1e-10): both solve kinds converge to the
+    direct solution, and the FP64 iteration count equals that of a numpy PCG
+    with the oracle's ILU(0) of the oracle's FEM matrix (within 2)."""
+    import scipy.sparse as sp
+    p, n = 2, 32
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, n)
+    om = _oracle_mesh(m)
+    op = igamf.Operator(m)
+    nodes = igamf.fem_reference_nodes(p, p, n, n)
+    fem = igamf.FemPC(n + p, n + p, nodes)
+    F = op.load(igamf.LOAD_SINSIN, 2 * np.pi ** 2)
+    Fh = F.cpu().numpy()
+    u_ref = _direct(m, om, Fh)
+    its = {}
+    for kind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        s = igamf.Solver(op, fem, kind=kind, pc=igamf.PC_FEM_ILU, rtol=1e-10)
+        u, st = s.solve(F)
+        assert st.status == 0 and st.rel_residual <= 1.0001e-10, (kind, st)
+        err = np.abs(u.cpu().numpy() - u_ref).max() / np.abs(u_ref).max()
+        assert err < 1e-7, (kind, err)
+        its[kind] = st.iterations
+    nf = n + p
+    rows, cols, _, _, val = O.fem_assemble(nf, nf, O.fem_nodes(p, p, n, n))
+    K = sp.csr_matrix((val, (rows, cols)), shape=(nf * nf, nf * nf))
+    K.sort_indices()
+    lu, diag = O.ilu0(K.indptr, K.indices, K.data)
+    A = O.assembled_matrix(om)
+    it_ref, _ = _pcg_count(A, Fh, lambda r: O.ilu0_solve(K.indptr, K.indices, lu, diag, r),
+                           1e-10)
+    assert it_ref > 0 and abs(its[igamf.SOLVE_PCG_F64] - it_ref) <= 2, (its, it_ref)
+
+
 def test_iteration_limit_and_bad_options():
     m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 2, 32)
     op = igamf.Operator(m)
