@@ -104,6 +104,30 @@ int iga_mesh_build(const iga_mesh_params *params, iga_mesh **out);
 int iga_mesh_describe(const iga_mesh *m, iga_mesh_desc *desc);
 int iga_mesh_destroy(iga_mesh *m);
 
+/* ---- the reference's on-disk partition format --------------------------
+ * A partition file of FileManager::WritePartition
+ * (matrix_free/src/FileManager.cpp:3-69, read there by ReadPartition
+ * :114-280): lines "nlocalfunc: N", "nlocalelemx: N", "nlocalelemy: N", then
+ * named sections (ID, ghostID, Dir, IEN, ElemSize1, ElemSize2, CP,
+ * NURBSExtraction1, NURBSExtraction2), each followed by one line of
+ * space-separated values; reals with 16 significant digits.  The degrees
+ * are not in it (they are in the preprocess info file).  A file without the
+ * element sizes or extraction operators (the short WritePartition form,
+ * :71-112) gives IGA_ERR_BAD_MESH. */
+int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out);
+/* Write m in the full form (ghostID empty: one partition). */
+int iga_mesh_write_partition(const iga_mesh *m, const char *filename);
+typedef struct {
+  int p, q;
+  double Lx, Ly;
+  int nelem_x, nelem_y, part_num_1d, dim;
+  char base_name[256];
+} iga_preprocess_info;
+/* The preprocess info file (FileManager::WritePreprocessInfo / Read...,
+ * FileManager.cpp:375-448); the partition of rank r is
+ * base_name + "_" + r + ".txt" (:450-452). */
+int iga_read_preprocess_info(const char *filename, iga_preprocess_info *info);
+
 /* splitmix64 -> Box-Muller, the generator of every synthetic input */
 int iga_rng_normal(uint64_t seed, int64_t n, double *out);
 int iga_rng_uniform(uint64_t seed, int64_t n, double lo, double hi, double *out);

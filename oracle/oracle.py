@@ -392,6 +392,11 @@ def ref_mf():
         L.ref_apply_mf.argtypes = mesh_args + [_F64P, _F64P]
         L.ref_apply_sf.argtypes = mesh_args + [_F64P, _F64P]
         L.ref_load.argtypes = mesh_args + [_F64P]
+        L.ref_write_partition.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _F64P, _F64P,
+                                          _F64P, _I32P, _I32P, C.c_int, _I32P, C.c_int, _F64P,
+                                          C.c_int, _F64P, C.c_int]
+        L.ref_read_partition.argtypes = [C.c_char_p, _I32P, _F64P, _F64P, _F64P, _I32P, _I32P,
+                                         _I32P, _F64P, _F64P, C.c_int]
     return _ref_mf
 
 
@@ -430,3 +435,34 @@ def ref_load(m):
     F = np.empty(m.nfunc)
     ref_mf().ref_load(*_mesh_args(m), F)
     return F
+
+
+def ref_write_partition(fn, a):
+    """The reference's FileManager::WritePartition (full form) of mesh arrays a
+    (as igamf.Mesh.arrays())."""
+    L = ref_mf()
+    nf = len(a["ID"])
+    i32 = lambda v: np.ascontiguousarray(v, np.int32)
+    f64 = lambda v: np.ascontiguousarray(v, np.float64)
+    L.ref_write_partition(os.fsencode(fn), nf, a["nex"], a["ney"], f64(a["h1"]), f64(a["h2"]),
+                          f64(a["CP"]), i32(a["ID"]), i32(a["Dir"]), len(a["Dir"]),
+                          i32(a["IEN"]), len(a["IEN"]), f64(a["C1"]), len(a["C1"]),
+                          f64(a["C2"]), len(a["C2"]))
+
+
+def ref_read_partition(fn, cap):
+    """The reference's FileManager::ReadPartition (full form) -> dict of arrays."""
+    L = ref_mf()
+    cnt = np.zeros(11, np.int32)
+    bufs = {k: np.zeros(cap, np.float64) for k in ("h1", "h2", "CP", "C1", "C2")}
+    ibufs = {k: np.zeros(cap, np.int32) for k in ("ID", "Dir", "IEN")}
+    rc = L.ref_read_partition(os.fsencode(fn), cnt, bufs["h1"], bufs["h2"], bufs["CP"],
+                              ibufs["ID"], ibufs["Dir"], ibufs["IEN"], bufs["C1"], bufs["C2"],
+                              cap)
+    if rc != 0:
+        raise ValueError("capacity too small")
+    names = ("h1", "h2", "CP", "ID", "Dir", "IEN", "C1", "C2")
+    out = {"nfunc": int(cnt[0]), "nex": int(cnt[1]), "ney": int(cnt[2])}
+    for k, n in zip(names, cnt[3:]):
+        out[k] = (bufs.get(k) if k in bufs else ibufs[k])[:n].copy()
+    return out

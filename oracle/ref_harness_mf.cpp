@@ -5,7 +5,8 @@
 //
 // What runs reference code: BSplineBasis, BernsteinBasis, QuadraturePoint,
 // NURBSExtractionGenerator, IENGenerator, IDGenerator, AbscissaeGenerator,
-// ElementMF::GenerateElement, ElementMFSF::GenerateElement.
+// ElementMF::GenerateElement, ElementMFSF::GenerateElement,
+// FileManager::WritePartition / ReadPartition.
 // What is restated here because its translation unit needs PETSc:
 //   - the element contraction of matrix_free/src/LocalAssemblyMF.cpp:31-67,
 //   - the element load of LocalAssemblyMF.cpp:3-29,
@@ -13,6 +14,7 @@
 //   - the gather / scatter / Dirichlet steps of
 //     matrix_free/src/GlobalAssemblyMF.cpp:141-210 (PETSc Vec calls),
 //   - the serial renumbering of matrix_free_gpu/src/Partition.cpp:258-269.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -21,6 +23,7 @@
 #include "BernsteinBasis.hpp"
 #include "ElementMF.hpp"
 #include "ElementMFSF.hpp"
+#include "FileManager.hpp"
 #include "IDGenerator.hpp"
 #include "IENGenerator.hpp"
 #include "NURBSExtractionGenerator.hpp"
@@ -302,6 +305,49 @@ int ref_load(int p, int q, int nex, int ney, int nfunc, const double *CP,
       }
     }
   for (int d = 0; d < ndir; ++d) F[Dir[d]] = 0.0;
+  return 0;
+}
+
+// FileManager::WritePartition (full form) of the given arrays
+int ref_write_partition(const char *fn, int nfunc, int nex, int ney, const double *h1,
+                        const double *h2, const double *cp, const int *id, const int *dir,
+                        int ndir, const int *ien, int nien, const double *c1, int nc1,
+                        const double *c2, int nc2) {
+  FileManager fm;
+  fm.WritePartition(fn, nfunc, nex, ney, std::vector<double>(h1, h1 + nex),
+                    std::vector<double>(h2, h2 + ney), std::vector<double>(cp, cp + 2 * nfunc),
+                    std::vector<int>(id, id + nfunc), std::vector<int>(),
+                    std::vector<int>(dir, dir + ndir), std::vector<int>(ien, ien + nien),
+                    std::vector<double>(c1, c1 + nc1), std::vector<double>(c2, c2 + nc2));
+  return 0;
+}
+
+// FileManager::ReadPartition (full form) into caller arrays of capacity cap
+// each; counts[11] = nfunc, nex, ney, then the lengths read of h1, h2, cp,
+// id, dir, ien, c1, c2 (-1 when one exceeds cap).
+int ref_read_partition(const char *fn, int *counts, double *h1, double *h2, double *cp, int *id,
+                       int *dir, int *ien, double *c1, double *c2, int cap) {
+  FileManager fm;
+  int nfunc = 0, nex = 0, ney = 0;
+  std::vector<double> vh1, vh2, vcp, vc1, vc2;
+  std::vector<int> vid, vghost, vdir, vien;
+  fm.ReadPartition(fn, nfunc, nex, ney, vh1, vh2, vcp, vid, vghost, vdir, vien, vc1, vc2);
+  const size_t sz[] = {vh1.size(), vh2.size(), vcp.size(), vid.size(), vdir.size(),
+                       vien.size(), vc1.size(), vc2.size()};
+  for (size_t k : sz)
+    if (k > (size_t)cap) return -1;
+  counts[0] = nfunc;
+  counts[1] = nex;
+  counts[2] = ney;
+  for (int k = 0; k < 8; ++k) counts[3 + k] = (int)sz[k];
+  std::copy(vh1.begin(), vh1.end(), h1);
+  std::copy(vh2.begin(), vh2.end(), h2);
+  std::copy(vcp.begin(), vcp.end(), cp);
+  std::copy(vid.begin(), vid.end(), id);
+  std::copy(vdir.begin(), vdir.end(), dir);
+  std::copy(vien.begin(), vien.end(), ien);
+  std::copy(vc1.begin(), vc1.end(), c1);
+  std::copy(vc2.begin(), vc2.end(), c2);
   return 0;
 }
 

@@ -15,8 +15,12 @@
 // identity: coefficient i is the blossom at (t_{i+1}, ..., t_{i+p}).
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "iga_mf.h"
@@ -265,6 +269,147 @@ int iga_mesh_describe(const iga_mesh *m, iga_mesh_desc *d) {
 int iga_mesh_destroy(iga_mesh *m) {
   delete m;
   return IGA_OK;
+}
+
+int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out) {
+  if (!filename || !out) return IGA_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (p < 1 || p > 8 || q < 1 || q > 8) return IGA_ERR_INVALID_DEGREE;
+  std::FILE *f = std::fopen(filename, "r");
+  if (!f) return IGA_ERR_INVALID_ARGUMENT;
+  std::string text;
+  {
+    char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    std::fclose(f);
+  }
+  // split into lines
+  std::vector<std::string> lines;
+  for (size_t a = 0; a <= text.size();) {
+    size_t b = text.find('\n', a);
+    if (b == std::string::npos) b = text.size();
+    std::string l = text.substr(a, b - a);
+    if (!l.empty() && l.back() == '\r') l.pop_back();
+    lines.push_back(l);
+    a = b + 1;
+  }
+  auto header = [&](size_t i, const char *key, long &v) {
+    const size_t k = std::strlen(key);
+    if (i >= lines.size() || lines[i].compare(0, k, key) != 0) return false;
+    char *end = nullptr;
+    v = std::strtol(lines[i].c_str() + k, &end, 10);
+    return end != lines[i].c_str() + k;
+  };
+  long nfunc = 0, nex = 0, ney = 0;
+  if (!header(0, "nlocalfunc: ", nfunc) || !header(1, "nlocalelemx: ", nex) ||
+      !header(2, "nlocalelemy: ", ney) || nfunc < 1 || nex < 1 || ney < 1)
+    return IGA_ERR_BAD_MESH;
+  std::map<std::string, std::vector<double>> sec;
+  for (size_t i = 3; i + 1 < lines.size(); ++i) {
+    const std::string &name = lines[i];
+    if (name.empty()) continue;
+    std::vector<double> v;
+    const char *c = lines[i + 1].c_str();
+    for (;;) {
+      char *end = nullptr;
+      const double d = std::strtod(c, &end);
+      if (end == c) break;
+      v.push_back(d);
+      c = end;
+    }
+    sec[name] = std::move(v);
+    ++i;
+  }
+  const char *need[] = {"ID", "Dir", "IEN", "ElemSize1", "ElemSize2", "CP", "NURBSExtraction1",
+                        "NURBSExtraction2"};
+  for (const char *k : need)
+    if (!sec.count(k)) return IGA_ERR_BAD_MESH;
+  const size_t nloc = (size_t)(p + 1) * (q + 1);
+  if (sec["ID"].size() != (size_t)nfunc || sec["CP"].size() != 2 * (size_t)nfunc ||
+      sec["IEN"].size() != (size_t)nex * ney * nloc || sec["ElemSize1"].size() != (size_t)nex ||
+      sec["ElemSize2"].size() != (size_t)ney ||
+      sec["NURBSExtraction1"].size() != (size_t)nex * (p + 1) * (p + 1) ||
+      sec["NURBSExtraction2"].size() != (size_t)ney * (q + 1) * (q + 1))
+    return IGA_ERR_BAD_MESH;
+  iga_mesh *m = new (std::nothrow) iga_mesh();
+  if (!m) return IGA_ERR_OUT_OF_MEMORY;
+  m->p = p;
+  m->q = q;
+  m->nex = (int)nex;
+  m->ney = (int)ney;
+  m->cp = sec["CP"];
+  m->h1 = sec["ElemSize1"];
+  m->h2 = sec["ElemSize2"];
+  m->C1 = sec["NURBSExtraction1"];
+  m->C2 = sec["NURBSExtraction2"];
+  auto ints = [](const std::vector<double> &v, std::vector<int> &o) {
+    o.resize(v.size());
+    for (size_t i = 0; i < v.size(); ++i) o[i] = (int)v[i];
+  };
+  ints(sec["ID"], m->id);
+  ints(sec["Dir"], m->dir);
+  ints(sec["IEN"], m->ien);
+  *out = m;
+  return IGA_OK;
+}
+
+int iga_mesh_write_partition(const iga_mesh *m, const char *filename) {
+  if (!m || !filename) return IGA_ERR_INVALID_ARGUMENT;
+  std::FILE *f = std::fopen(filename, "w");
+  if (!f) return IGA_ERR_INVALID_ARGUMENT;
+  std::fprintf(f, "nlocalfunc: %d\nnlocalelemx: %d\nnlocalelemy: %d\n", (int)m->id.size(), m->nex,
+               m->ney);
+  auto ints = [&](const char *name, const std::vector<int> &v) {
+    std::fprintf(f, "%s\n", name);
+    for (int x : v) std::fprintf(f, "%d ", x);
+    std::fprintf(f, "\n");
+  };
+  auto reals = [&](const char *name, const std::vector<double> &v) {
+    std::fprintf(f, "%s\n", name);
+    for (double x : v) std::fprintf(f, "%.16g ", x);
+    std::fprintf(f, "\n");
+  };
+  ints("ID", m->id);
+  ints("ghostID", {});
+  ints("Dir", m->dir);
+  ints("IEN", m->ien);
+  reals("ElemSize1", m->h1);
+  reals("ElemSize2", m->h2);
+  reals("CP", m->cp);
+  reals("NURBSExtraction1", m->C1);
+  reals("NURBSExtraction2", m->C2);
+  return std::fclose(f) == 0 ? IGA_OK : IGA_ERR_INVALID_ARGUMENT;
+}
+
+int iga_read_preprocess_info(const char *filename, iga_preprocess_info *info) {
+  if (!filename || !info) return IGA_ERR_INVALID_ARGUMENT;
+  std::memset(info, 0, sizeof(*info));
+  std::FILE *f = std::fopen(filename, "r");
+  if (!f) return IGA_ERR_INVALID_ARGUMENT;
+  char line[512];
+  int seen = 0;
+  while (std::fgets(line, sizeof(line), f)) {
+    std::string l(line);
+    while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+    auto val = [&](const char *key) -> const char * {
+      const size_t k = std::strlen(key);
+      return l.compare(0, k, key) == 0 ? l.c_str() + k : nullptr;
+    };
+    const char *v;
+    if ((v = val("p: "))) info->p = std::atoi(v), seen |= 1;
+    else if ((v = val("q: "))) info->q = std::atoi(v), seen |= 2;
+    else if ((v = val("Lx: "))) info->Lx = std::atof(v);
+    else if ((v = val("Ly: "))) info->Ly = std::atof(v);
+    else if ((v = val("nElemX: "))) info->nelem_x = std::atoi(v), seen |= 4;
+    else if ((v = val("nElemY: "))) info->nelem_y = std::atoi(v), seen |= 8;
+    else if ((v = val("part_num_1d: "))) info->part_num_1d = std::atoi(v);
+    else if ((v = val("dim: "))) info->dim = std::atoi(v);
+    else if ((v = val("base_name: ")))
+      std::snprintf(info->base_name, sizeof(info->base_name), "%s", v);
+  }
+  std::fclose(f);
+  return seen == 15 ? IGA_OK : IGA_ERR_BAD_MESH;
 }
 
 }  // extern "C"

@@ -75,6 +75,22 @@ class SolverOpts(C.Structure):
                 ("inner_f16_geometry", C.c_int), ("fused", C.c_int)]
 
 
+class PreprocessInfo(C.Structure):
+    _fields_ = [("p", C.c_int), ("q", C.c_int), ("Lx", C.c_double), ("Ly", C.c_double),
+                ("nelem_x", C.c_int), ("nelem_y", C.c_int), ("part_num_1d", C.c_int),
+                ("dim", C.c_int), ("base_name", C.c_char * 256)]
+
+
+def read_preprocess_info(filename):
+    """The reference's preprocess info file as a dict (iga_read_preprocess_info)."""
+    info = PreprocessInfo()
+    check(_lib().iga_read_preprocess_info(os.fsencode(filename), C.byref(info)),
+          "read_preprocess_info")
+    return {"p": info.p, "q": info.q, "Lx": info.Lx, "Ly": info.Ly, "nelem_x": info.nelem_x,
+            "nelem_y": info.nelem_y, "part_num_1d": info.part_num_1d, "dim": info.dim,
+            "base_name": info.base_name.decode()}
+
+
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int),
                 ("rel_residual", C.c_double), ("solve_ms", C.c_double), ("status", C.c_int)]
@@ -102,6 +118,9 @@ def _lib():
         L.iga_mesh_build.argtypes = [C.POINTER(MeshParams), C.POINTER(VP)]
         L.iga_mesh_describe.argtypes = [VP, C.POINTER(MeshDesc)]
         L.iga_mesh_destroy.argtypes = [VP]
+        L.iga_mesh_read_partition.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(VP)]
+        L.iga_mesh_write_partition.argtypes = [VP, C.c_char_p]
+        L.iga_read_preprocess_info.argtypes = [C.c_char_p, C.POINTER(PreprocessInfo)]
         L.iga_rng_normal.argtypes = [C.c_uint64, C.c_int64, VP]
         L.iga_rng_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, VP]
         L.iga_op_opts_default.argtypes = [C.POINTER(OpOpts)]
@@ -206,6 +225,22 @@ class Mesh:
         h = VP()
         check(_lib().iga_mesh_build(C.byref(prm), C.byref(h)), "mesh_build")
         return cls(handle=h)
+
+    @classmethod
+    def read_partition(cls, filename, p, q):
+        """A partition file in the reference's FileManager format
+        (iga_mesh_read_partition)."""
+        h = VP()
+        check(_lib().iga_mesh_read_partition(os.fsencode(filename), p, q, C.byref(h)),
+              "mesh_read_partition")
+        return cls(handle=h)
+
+    def write_partition(self, filename):
+        """iga_mesh_write_partition (meshes built or read by the library)."""
+        if self._h is None:
+            raise ValueError("write_partition needs a library-owned mesh")
+        check(_lib().iga_mesh_write_partition(self._h, os.fsencode(filename)),
+              "mesh_write_partition")
 
     @classmethod
     def from_arrays(cls, p, q, nex, ney, CP, W, IEN, ID, Dir, C1, C2, h1, h2, kappa=None):

@@ -107,3 +107,59 @@ def test_mesh_build_rejects_invalid_input():
     assert igamf.lib().iga_mesh_build(C.byref(prm), C.byref(h)) == 4
     prm = igamf.MeshParams(1, 1, 20, 20, 1.0, 1.0, 2.0, 0.0, 1.0, 1.0, 0)
     assert igamf.lib().iga_mesh_build(C.byref(prm), C.byref(h)) == 2
+
+
+def _same_mesh(a, b):
+    for k in ("ID", "Dir", "IEN"):
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    for k in ("CP", "C1", "C2", "h1", "h2"):
+        x, y = np.asarray(a[k], np.float64), np.asarray(b[k], np.float64)
+        assert x.shape == y.shape, k
+        # 16 significant digits on disk: within 1e-15 relative
+        assert np.all(np.abs(x - y) <= 1e-15 * np.maximum(np.abs(x), 1e-300)), k
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference units not built (oracle/_ref)")
+def test_partition_file_matches_reference_filemanager(tmp_path):
+    """The reference's partition format (FileManager::WritePartition /
+    ReadPartition, compiled from the reference): the library's writer emits
+    the same bytes as the reference's writer for the same mesh, the
+    reference's reader reads the library's file back, and the library's reader
+    reads the reference's file back."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 10, 7)
+    a = m.arrays()
+    ours, refs = str(tmp_path / "ours_0.txt"), str(tmp_path / "ref_0.txt")
+    m.write_partition(ours)
+    O.ref_write_partition(refs, a)
+    assert open(ours, "rb").read() == open(refs, "rb").read()
+    r = O.ref_read_partition(ours, 4096)
+    assert (r["nfunc"], r["nex"], r["ney"]) == (len(a["ID"]), 10, 7)
+    _same_mesh(a, r)
+    m2 = igamf.Mesh.read_partition(refs, 3, 3)
+    _same_mesh(a, m2.arrays())
+
+
+def test_partition_reader_rejects_bad_files(tmp_path):
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 2, 6)
+    fn = str(tmp_path / "part_0.txt")
+    m.write_partition(fn)
+    m2 = igamf.Mesh.read_partition(fn, 2, 2)
+    _same_mesh(m.arrays(), m2.arrays())
+    with pytest.raises(igamf.IgaError) as e:  # IEN holds 9 functions per element, not 16
+        igamf.Mesh.read_partition(fn, 3, 3)
+    assert e.value.status == 5
+    text = open(fn).read()
+    short = str(tmp_path / "short_0.txt")  # the short form has no extraction operators
+    open(short, "w").write(text.split("ElemSize1")[0] + "CP\n" + text.split("CP\n")[1].split("\n")[0]
+                           + "\n")
+    with pytest.raises(igamf.IgaError) as e:
+        igamf.Mesh.read_partition(short, 2, 2)
+    assert e.value.status == 5
+    with pytest.raises(igamf.IgaError):
+        igamf.Mesh.read_partition(str(tmp_path / "missing.txt"), 2, 2)
+    info = str(tmp_path / "info.txt")
+    open(info, "w").write("p: 3\nq: 3\nLx: 1\nLy: 1.5\nnElemX: 64\nnElemY: 32\npart_num_1d: 1\n"
+                          "dim: 2\nbase_name: part\n")
+    d = igamf.read_preprocess_info(info)
+    assert (d["p"], d["q"], d["nelem_x"], d["nelem_y"], d["Ly"], d["base_name"]) == \
+        (3, 3, 64, 32, 1.5, "part")
