@@ -115,7 +115,8 @@ int iga_mesh_destroy(iga_mesh *m);
  * element sizes or extraction operators (the short WritePartition form,
  * :71-112) gives IGA_ERR_BAD_MESH. */
 int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out);
-/* Write m in the full form (ghostID empty: one partition). */
+/* Write m in the full form (ghostID empty: one partition).  The format has
+ * no NURBS weights: IGA_ERR_UNSUPPORTED for a mesh that has them. */
 int iga_mesh_write_partition(const iga_mesh *m, const char *filename);
 typedef struct {
   int p, q;

@@ -356,6 +356,7 @@ int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out) 
 
 int iga_mesh_write_partition(const iga_mesh *m, const char *filename) {
   if (!m || !filename) return IGA_ERR_INVALID_ARGUMENT;
+  if (m->has_w) return IGA_ERR_UNSUPPORTED;  // the format carries no NURBS weights
   std::FILE *f = std::fopen(filename, "w");
   if (!f) return IGA_ERR_INVALID_ARGUMENT;
   std::fprintf(f, "nlocalfunc: %d\nnlocalelemx: %d\nnlocalelemy: %d\n", (int)m->id.size(), m->nex,

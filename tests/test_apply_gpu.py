@@ -332,7 +332,7 @@ def test_large_annulus_invariants():
 def test_operator_from_partition_file(tmp_path):
     """A mesh read back from the reference's partition format gives the same
     operator (control points within 1e-15 relative after 16 digits on disk)."""
-    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 40, 33)
+    m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, 3, 40, 33, perturb=0.2, seed=6)
     fn = str(tmp_path / "part_0.txt")
     m.write_partition(fn)
     m2 = igamf.Mesh.read_partition(fn, 3, 3)

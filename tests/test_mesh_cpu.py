@@ -126,8 +126,9 @@ def test_partition_file_matches_reference_filemanager(tmp_path):
     the same bytes as the reference's writer for the same mesh, the
     reference's reader reads the library's file back, and the library's reader
     reads the reference's file back."""
-    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 10, 7)
+    m = igamf.Mesh.build(igamf.GEOM_PERTURBED_SQUARE, 3, 10, 7, perturb=0.2, seed=5)
     a = m.arrays()
+    assert a["W"] is None  # the format has no weights
     ours, refs = str(tmp_path / "ours_0.txt"), str(tmp_path / "ref_0.txt")
     m.write_partition(ours)
     O.ref_write_partition(refs, a)
@@ -157,6 +158,9 @@ def test_partition_reader_rejects_bad_files(tmp_path):
     assert e.value.status == 5
     with pytest.raises(igamf.IgaError):
         igamf.Mesh.read_partition(str(tmp_path / "missing.txt"), 2, 2)
+    with pytest.raises(igamf.IgaError) as e:  # weights cannot be written
+        igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 2, 6).write_partition(fn)
+    assert e.value.status == 6
     info = str(tmp_path / "info.txt")
     open(info, "w").write("p: 3\nq: 3\nLx: 1\nLy: 1.5\nnElemX: 64\nnElemY: 32\npart_num_1d: 1\n"
                           "dim: 2\nbase_name: part\n")
