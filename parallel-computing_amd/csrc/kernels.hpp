@@ -161,6 +161,25 @@ __device__ __forceinline__ void load_gq(const TG *G, int tile, int t, int NT, T 
 // tile.  X[b][a], ye[b][a]: b = row offset (eta), a = column offset (xi).
 // --------------------------------------------------------------------------
 
+// FP32 inputs of a polynomial or tensor-weight element are taken relative to
+// its first value: the basis is a partition of unity, so A_e 1 = 0 and the
+// element vector is unchanged, while the FP32 contractions see the local
+// variation of x instead of its magnitude.  Without it the FP32 operator's
+// rounding is relative to |x|, which for the smooth iterates of a CG is far
+// above the variation across an element, and the FP32 inner solve needs
+// about 20 % more iterations (quarter annulus, Jacobi).  Not for general
+// weights (their quotient rule uses the weighted value itself) or FP64.
+template <typename T, int P, bool RAT>
+__device__ __forceinline__ void centre_element(const T (&X0)[P + 1][P + 1],
+                                               T (&X)[P + 1][P + 1]) {
+  constexpr bool C = std::is_same_v<T, float> && !RAT;
+  const T c = C ? X0[0][0] : T(0);
+#pragma unroll
+  for (int b = 0; b <= P; ++b)
+#pragma unroll
+    for (int a = 0; a <= P; ++a) X[b][a] = C ? X0[b][a] - c : X0[b][a];
+}
+
 // Sum-factorised action ye = -sum_q gradR^T G_q gradR X.  The eta loop fuses
 // the forward eta contraction, the quadrature-point operator and the backward
 // eta contraction, so only one table row is live at a time.
@@ -169,11 +188,13 @@ __device__ __forceinline__ void load_gq(const TG *G, int tile, int t, int NT, T 
 // columns; the element vector is then the sum of the two threads' ye.
 template <typename T, int P, int NQ, int LX, int LY, bool RAT = false, int NI = NQ,
           typename GL>
-__device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
+__device__ __forceinline__ void element_sf(const T (&X0)[P + 1][P + 1],
                                            const T (*tbx)[LX], const T (*tby)[LY],
                                            int lx, int ly, const GL &gl,
                                            T (&ye)[P + 1][P + 1], int i0 = 0) {
   constexpr int NB = P + 1, TAB = NQ * NB;
+  T X[NB][NB];
+  centre_element<T, P, RAT>(X0, X);
   T t0[NB][NI], t1[NB][NI];
 #pragma unroll
   for (int il = 0; il < NI; ++il) {
@@ -267,11 +288,13 @@ __device__ __forceinline__ void element_sf(const T (&X)[P + 1][P + 1],
   }
 }
 template <typename T, int P, int NQ, int LX, int LY, bool RAT = false, typename GL>
-__device__ __forceinline__ void element_ebe(const T (&X)[P + 1][P + 1],
+__device__ __forceinline__ void element_ebe(const T (&X0)[P + 1][P + 1],
                                             const T (*tbx)[LX], const T (*tby)[LY],
                                             int lx, int ly, const GL &gl,
                                             T (&ye)[P + 1][P + 1]) {
   constexpr int NB = P + 1, TAB = NQ * NB;
+  T X[NB][NB];
+  centre_element<T, P, RAT>(X0, X);
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll

@@ -143,14 +143,17 @@ __device__ __forceinline__ bool finite(double x) { return x == x && !isinf(x); }
 // ---------------------------------------------------------------- CG kernels
 // init: u = 0, r = f*fscale, z = dinv r (or r), p = z when use_z;
 // partials r.r and r.z
-template <typename V, typename F, typename D>
-__global__ void k_cg_init(const F *__restrict__ f, double fscale, V *__restrict__ u,
+// TU: type of the iterate u (FP64 also for the FP32 inner recursion: an FP32
+// iterate's rounding returns through the true residual of every reliable
+// update and delays convergence)
+template <typename V, typename F, typename D, typename TU = V>
+__global__ void k_cg_init(const F *__restrict__ f, double fscale, TU *__restrict__ u,
                           V *__restrict__ r, V *__restrict__ z, V *__restrict__ p,
                           const D *__restrict__ dinv, int use_z, int64_t n, double *part) {
   double rr = 0.0, rz = 0.0;
   GRID_LOOP(i, n) {
     const V ri = (V)((double)f[i] * fscale);
-    u[i] = V(0);
+    u[i] = TU(0);
     r[i] = ri;
     const V zi = dinv ? (V)((double)ri * (double)dinv[i]) : ri;
     if (use_z) {
@@ -228,8 +231,8 @@ __global__ void k_cg_pap(const V *__restrict__ p, const V *__restrict__ Ap, int6
 // K2: alpha; u += alpha p; r -= alpha Ap (rold keeps r for the flexible
 // beta); z = dinv r when the preconditioner is local; partials of r.r (or
 // z.z with prec_norm) and r.z
-template <typename V, typename D, bool LOCAL, bool ROLD>
-__global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__restrict__ p,
+template <typename V, typename D, bool LOCAL, bool ROLD, typename TU = V>
+__global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__restrict__ p,
                             const V *__restrict__ Ap, V *__restrict__ z, V *__restrict__ rold,
                             const D *__restrict__ dinv, int prec_norm, int64_t n, double *part,
                             double *sc, int *ic, int nb, const int *outer_done, int64_t ld,
@@ -248,7 +251,7 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
   // overlaps the fold and the stores.
   constexpr int U = 4;
   const int64_t step = (int64_t)gridDim.x * RB * U;
-  auto load = [&](int64_t b, V (&uu)[U], V (&pv)[U], V (&rv)[U], V (&av)[U], D (&dv)[U]) {
+  auto load = [&](int64_t b, TU (&uu)[U], V (&pv)[U], V (&rv)[U], V (&av)[U], D (&dv)[U]) {
     if (b + (int64_t)(U - 1) * RB < n) {  // whole step in range: unconditional loads
 #pragma unroll
       for (int k = 0; k < U; ++k) {
@@ -264,7 +267,7 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
       for (int k = 0; k < U; ++k) {
         const int64_t i = b + (int64_t)k * RB;
         const bool in = i < n;
-        uu[k] = in ? u[i] : V(0);
+        uu[k] = in ? u[i] : TU(0);
         pv[k] = in ? p[i] : V(0);
         rv[k] = in ? r[i] : V(0);
         av[k] = in ? Ap[i] : V(0);
@@ -273,7 +276,8 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
     }
   };
   int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x;
-  V uu[U], pv[U], rv[U], av[U];
+  TU uu[U];
+  V pv[U], rv[U], av[U];
   D dv[U];
   if (b0 < n) load(b0, uu, pv, rv, av, dv);
   // p.Ap: partials of k_cg_pap, or the apply kernel's per-tile partials
@@ -286,14 +290,15 @@ __global__ void k_cg_update(V *__restrict__ u, V *__restrict__ r, const V *__res
   double rr = 0.0, rz = 0.0;
   while (b0 < n) {
     const int64_t bn = b0 + step;
-    V uu2[U], pv2[U], rv2[U], av2[U];
+    TU uu2[U];
+    V pv2[U], rv2[U], av2[U];
     D dv2[U];
     if (bn < n) load(bn, uu2, pv2, rv2, av2, dv2);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t i = b0 + (int64_t)k * RB;
       if (i >= n) continue;
-      u[i] = (V)((double)uu[k] + a * (double)pv[k]);
+      u[i] = (TU)((double)uu[k] + a * (double)pv[k]);
       if (ROLD) rold[i] = rv[k];
       const V ri = (V)((double)rv[k] - a * (double)av[k]);
       r[i] = ri;
@@ -447,11 +452,12 @@ __global__ void k_cg_rz0_fin(const double *part, int nb, double *sc, const int *
 }
 
 // ---------------------------------------------------------------- reliable update
-// x64 += x32, x32 = 0
-__global__ void k_ru_accum(double *x64, float *x32, int64_t n) {
+// x64 += x, x = 0
+template <typename TU>
+__global__ void k_ru_accum(double *x64, TU *x, int64_t n) {
   GRID_LOOP(i, n) {
-    x64[i] += (double)x32[i];
-    x32[i] = 0.f;
+    x64[i] += (double)x[i];
+    x[i] = TU(0);
   }
 }
 
@@ -512,8 +518,8 @@ enum { CG_RZO = 0, CG_PAP, CG_IT, CG_RR = 4, CG_RZ = CG_RR + RG, CG_US = CG_RZ +
 
 // init: x = 0, r = f*fscale, u = M r, p = s = 0; NEW partials of r.r, r.u
 // and u.s (= 0)
-template <typename V, typename F, typename D>
-__global__ void k_cgf_init(const F *__restrict__ f, double fscale, V *__restrict__ x,
+template <typename V, typename F, typename D, typename TU = V>
+__global__ void k_cgf_init(const F *__restrict__ f, double fscale, TU *__restrict__ x,
                            V *__restrict__ r, V *__restrict__ u, V *__restrict__ p,
                            V *__restrict__ sv, const D *__restrict__ dinv, int64_t n,
                            double *cgs) {
@@ -521,7 +527,7 @@ __global__ void k_cgf_init(const F *__restrict__ f, double fscale, V *__restrict
   GRID_LOOP(i, n) {
     const V ri = (V)((double)f[i] * fscale);
     const V ui = dinv ? (V)((double)ri * (double)dinv[i]) : ri;
-    x[i] = V(0);
+    x[i] = TU(0);
     r[i] = ri;
     u[i] = ui;
     p[i] = V(0);
@@ -573,8 +579,8 @@ __global__ void k_cgf_init_fin(double *cgs, int nb, double *sc, int *ic, double 
 // solve -- when r.r fell below the reliable-update threshold; the state is
 // then that of the last completed iteration.  zero_w: clear w for the next
 // scatter-add apply.
-template <typename V, typename D, bool JAC>
-__global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
+template <typename V, typename D, bool JAC, typename TU = V>
+__global__ void k_cgf_update(TU *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
                              V *__restrict__ sv, V *__restrict__ u, V *__restrict__ w,
                              const D *__restrict__ dinv, int zero_w, int64_t n, double *cgs,
                              const double *__restrict__ tdot, int ntiles, double *sc, int *ic,
@@ -585,7 +591,8 @@ __global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict
   constexpr int U = 4;
   const int64_t step = (int64_t)gridDim.x * RB * U;
   struct Step {
-    V u[U], w[U], p[U], s[U], x[U], r[U];
+    V u[U], w[U], p[U], s[U], r[U];
+    TU x[U];
     D d[U];
   };
   auto load = [&](int64_t b, Step &q) {
@@ -610,7 +617,7 @@ __global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict
         q.w[k] = in ? w[i] : V(0);
         q.p[k] = in ? p[i] : V(0);
         q.s[k] = in ? sv[i] : V(0);
-        q.x[k] = in ? x[i] : V(0);
+        q.x[k] = in ? x[i] : TU(0);
         q.r[k] = in ? r[i] : V(0);
         q.d[k] = (JAC && in) ? dinv[i] : D(1);
       }
@@ -663,7 +670,7 @@ __global__ void k_cgf_update(V *__restrict__ x, V *__restrict__ r, V *__restrict
       const V un = JAC ? (V)((double)rn * (double)q.d[k]) : rn;
       p[i] = pn;
       sv[i] = sn;
-      x[i] = (V)((double)q.x[k] + a * (double)pn);
+      x[i] = (TU)((double)q.x[k] + a * (double)pn);
       r[i] = rn;
       u[i] = un;
       if (zero_w) w[i] = V(0);
@@ -999,8 +1006,8 @@ struct iga_solver {
   float *dinv32 = nullptr;                // the same, rounded, for the FP32 inner PCG
   double *x64 = nullptr;                  // FP64 PCG iterate (owned: graphs capture it)
   // FP32 inner state
-  float *d32 = nullptr, *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *z32 = nullptr,
-        *rold32 = nullptr;
+  double *d64 = nullptr;  // inner iterate (FP64: see k_cg_init)
+  float *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *z32 = nullptr, *rold32 = nullptr;
   // scalars: [0, S_NS) main CG, [S_NS, 2 S_NS) inner PC CG, then outer slots
   double *sc = nullptr;
   int *ic = nullptr;
@@ -1143,8 +1150,8 @@ int apply_pc(iga_solver *s, bool f64, const V *r, V *z, hipStream_t st) {
 }
 
 // One CG iteration on vectors of type V (FP64 main solve or FP32 inner solve).
-template <typename V>
-int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold,
+template <typename V, typename TU>
+int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold,
                  hipStream_t st) {
   const int64_t n = s->n;
   const int nb = s->nb;
@@ -1168,10 +1175,10 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
                       ? nullptr
                       : (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32);
   if (loc)
-    LAUNCH((k_cg_update<V, V, true, false>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n, s->part,
+    LAUNCH((k_cg_update<V, V, true, false, TU>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n, s->part,
            s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles);
   else
-    LAUNCH((k_cg_update<V, V, false, true>), nb, u, r, p, Ap, z, rold, (const V *)nullptr, 0, n,
+    LAUNCH((k_cg_update<V, V, false, true, TU>), nb, u, r, p, Ap, z, rold, (const V *)nullptr, 0, n,
            s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles);
   if (!loc) {
     rc = apply_pc<V>(s, f64, r, z, st);
@@ -1183,10 +1190,10 @@ int cg_iteration(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
-template <typename V>
-int cg_chunk(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st) {
+template <typename V, typename TU>
+int cg_chunk(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st) {
   for (int k = 0; k < CHUNK; ++k) {
-    int rc = cg_iteration<V>(s, f64, u, r, p, Ap, z, rold, st);
+    int rc = cg_iteration<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
     if (rc) return rc;
   }
   return IGA_OK;
@@ -1194,8 +1201,8 @@ int cg_chunk(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hi
 
 // Run a CG to completion: init (k_cg_init + k_cg_init_fin) must already have
 // been enqueued on st.
-template <typename V>
-int cg_run(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
+template <typename V, typename TU>
+int cg_run(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
            hipGraphExec_t *gexec, int *iters, int *done) {
   const int64_t n = s->n;
   if (!local_pc(s)) {
@@ -1218,7 +1225,7 @@ int cg_run(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipS
       (void)hipStreamDestroy(cs);
       return IGA_ERR_HIP;
     }
-    int rc = cg_chunk<V>(s, f64, u, r, p, Ap, z, rold, cs);
+    int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, cs);
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
@@ -1234,7 +1241,7 @@ int cg_run(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipS
     if (s->o.use_graph) {
       if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
     } else {
-      int rc = cg_chunk<V>(s, f64, u, r, p, Ap, z, rold, st);
+      int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
       if (rc) return rc;
     }
     if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1248,15 +1255,15 @@ int cg_run(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipS
 }
 
 // Continue a CG whose device state was restarted by k_ru_fin.
-template <typename V>
-int cg_resume(iga_solver *s, bool f64, V *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
+template <typename V, typename TU>
+int cg_resume(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
               hipGraphExec_t *gexec, int *iters, int *done) {
   int hic[2];
   for (;;) {
     if (s->o.use_graph && *gexec) {
       if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
     } else {
-      int rc = cg_chunk<V>(s, f64, u, r, p, Ap, z, rold, st);
+      int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
       if (rc) return rc;
     }
     if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1277,8 +1284,8 @@ bool use_fused(const iga_solver *s, bool f64) {
 
 // one fused iteration: apply (+ seam kernel when gathering) and k_cgf_update;
 // u, w, p, sv: the recursion's M r, A u, direction and A p
-template <typename V>
-int cgf_iteration(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w,
+template <typename V, typename TU>
+int cgf_iteration(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w,
                   hipStream_t st) {
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
@@ -1289,19 +1296,19 @@ int cgf_iteration(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w,
   if (rc) return rc;
   const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
   if (s->o.pc == IGA_PC_JACOBI)
-    LAUNCH((k_cgf_update<V, V, true>), s->nb, x, r, p, sv, u, w,
+    LAUNCH((k_cgf_update<V, V, true, TU>), s->nb, x, r, p, sv, u, w,
            f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32,
            gather ? 0 : 1, s->n, s->cgs, s->tdot, ntiles, s->sc, s->ic, s->nb);
   else
-    LAUNCH((k_cgf_update<V, V, false>), s->nb, x, r, p, sv, u, w, (const V *)nullptr,
+    LAUNCH((k_cgf_update<V, V, false, TU>), s->nb, x, r, p, sv, u, w, (const V *)nullptr,
            gather ? 0 : 1, s->n, s->cgs, s->tdot, ntiles, s->sc, s->ic, s->nb);
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
-template <typename V>
-int cgf_chunk(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st) {
+template <typename V, typename TU>
+int cgf_chunk(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st) {
   for (int k = 0; k < CHUNK; ++k) {
-    int rc = cgf_iteration<V>(s, f64, x, r, p, sv, u, w, st);
+    int rc = cgf_iteration<V, TU>(s, f64, x, r, p, sv, u, w, st);
     if (rc) return rc;
   }
   return IGA_OK;
@@ -1309,8 +1316,8 @@ int cgf_chunk(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w, hipS
 
 // Run (or, after k_cgf_ru_reset + k_cgf_ru_fin, continue) a fused recursion
 // until its done flag is set; the init kernels must already be enqueued.
-template <typename V>
-int cgf_run(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st,
+template <typename V, typename TU>
+int cgf_run(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st,
             hipGraphExec_t *gexec, int *iters, int *done) {
   if (s->o.assembly != IGA_ASSEMBLY_INVLM_GATHER &&
       hipMemsetAsync(w, 0, (size_t)s->n * sizeof(V), st) != hipSuccess)
@@ -1323,7 +1330,7 @@ int cgf_run(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w, hipStr
       (void)hipStreamDestroy(cs);
       return IGA_ERR_HIP;
     }
-    int rc = cgf_chunk<V>(s, f64, x, r, p, sv, u, w, cs);
+    int rc = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, cs);
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
@@ -1339,7 +1346,7 @@ int cgf_run(iga_solver *s, bool f64, V *x, V *r, V *p, V *sv, V *u, V *w, hipStr
     if (s->o.use_graph) {
       if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
     } else {
-      int rc = cgf_chunk<V>(s, f64, x, r, p, sv, u, w, st);
+      int rc = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, st);
       if (rc) return rc;
     }
     if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1550,7 +1557,7 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   SA(s->part_in, (size_t)P_NP * RG * 8);
   if (o.kind == IGA_SOLVE_PCG_F32_REFINE) {
     SA(s->dinv32, n * 4);
-    SA(s->d32, n * 4);
+    SA(s->d64, n * 8);
     SA(s->r32, n * 4);
     SA(s->p32, n * 4);
     SA(s->Ap32, n * 4);
@@ -1679,7 +1686,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
              dinv, n, s->cgs);
       hipLaunchKernelGGL(k_cgf_init_fin, dim3(1), dim3(RB), 0, st, s->cgs, nb, s->sc, s->ic,
                          o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit, 0.0);
-      rc = cgf_run<double>(s, true, s->x64, s->r, s->p, s->rold, s->z, s->Ap, st, &s->gf64,
+      rc = cgf_run<double, double>(s, true, s->x64, s->r, s->p, s->rold, s->z, s->Ap, st, &s->gf64,
                            &iters, &done);
     } else {
       LAUNCH((k_cg_init<double, double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, dinv,
@@ -1687,7 +1694,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
       hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
                          o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit,
                          0.0);
-      rc = cg_run<double>(s, true, s->x64, s->r, s->p, s->Ap, s->z, s->rold, st, &s->g64, &iters,
+      rc = cg_run<double, double>(s, true, s->x64, s->r, s->p, s->Ap, s->z, s->rold, st, &s->g64, &iters,
                           &done);
     }
     if (rc) return rc;
@@ -1719,18 +1726,18 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
       status = IGA_OK;
     } else if (use_fused(s, false)) {
       // rold32 serves as s = A p of the fused recursion
-      LAUNCH((k_cgf_init<float, double, float>), nb, s->r, 1.0, s->d32, s->r32, s->z32, s->p32,
+      LAUNCH((k_cgf_init<float, double, float, double>), nb, s->r, 1.0, s->d64, s->r32, s->z32, s->p32,
              s->rold32, dinv32, n, s->cgs);
       hipLaunchKernelGGL(k_cgf_init_fin, dim3(1), dim3(RB), 0, st, s->cgs, nb, s->sc, s->ic,
                          tol2 / rr, 0.0, o.divtol * o.divtol * ff / rr, o.maxit, RU2);
       for (;;) {
-        rc = cgf_run<float>(s, false, s->d32, s->r32, s->p32, s->rold32, s->z32, s->Ap32, st,
+        rc = cgf_run<float, double>(s, false, s->d64, s->r32, s->p32, s->rold32, s->z32, s->Ap32, st,
                             &s->gf32, &iters, &done);
         if (rc) return rc;
         if (done == 3) { status = IGA_ERR_DIVERGED; break; }
         // reliable update
         ++outer;
-        LAUNCH(k_ru_accum, nb, s->x64, s->d32, n);
+        LAUNCH(k_ru_accum<double>, nb, s->x64, s->d64, n);
         rc = outer_residual(s, f, s->x64, s->r, h, st);
         if (rc) return rc;
         rr = h[0];
@@ -1741,7 +1748,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
         hipLaunchKernelGGL(k_cgf_ru_fin, dim3(1), dim3(64), 0, st, s->sc, s->ic, RU2 * rr);
       }
     } else {
-      LAUNCH((k_cg_init<float, double, float>), nb, s->r, 1.0, s->d32, s->r32, s->z32, s->p32,
+      LAUNCH((k_cg_init<float, double, float, double>), nb, s->r, 1.0, s->d64, s->r32, s->z32, s->p32,
              dinv32,
              loc ? 1 : 0, n, s->part);
       hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
@@ -1749,18 +1756,18 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
       bool first = true;
       for (;;) {
         if (first) {
-          rc = cg_run<float>(s, false, s->d32, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
+          rc = cg_run<float, double>(s, false, s->d64, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
                              &s->g32, &iters, &done);
           first = false;
         } else {
-          rc = cg_resume<float>(s, false, s->d32, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
+          rc = cg_resume<float, double>(s, false, s->d64, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
                                 &s->g32, &iters, &done);
         }
         if (rc) return rc;
         if (done == 3) { status = IGA_ERR_DIVERGED; break; }
         // reliable update
         ++outer;
-        LAUNCH(k_ru_accum, nb, s->x64, s->d32, n);
+        LAUNCH(k_ru_accum<double>, nb, s->x64, s->d64, n);
         rc = outer_residual(s, f, s->x64, s->r, h, st);
         if (rc) return rc;
         rr = h[0];
@@ -1883,7 +1890,7 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
       // reliable update of column j with the FP64 operator
       ++outer[j];
       double *xj = s->bx64 + (size_t)j * n;
-      LAUNCH(k_ru_accum, nb, xj, s->bd32 + (size_t)j * n, n);
+      LAUNCH(k_ru_accum<float>, nb, xj, s->bd32 + (size_t)j * n, n);
       double h[2];
       rc = outer_residual(s, F + (size_t)j * ld, xj, s->r, h, st);
       if (rc) return rc;
