@@ -234,7 +234,7 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
     IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used + 1], s));
     ++op->prof_used;
   }
-  if (gather) {
+  if (gather && !(hook && hook->defer_seam)) {
     if (f64)
       IGA_HIP_TRY(op->ops->seam64((double *)y, op->halo64, op->dirmask, op->nex, op->ney,
                                   op->nfx, op->nfy, L, 1, mask_out, skip, s));
@@ -243,6 +243,25 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
                                   op->nfx, op->nfy, L, 1, mask_out, skip, s));
   }
   return IGA_OK;
+}
+
+SeamGeom seam_geom(const iga_op *op, bool f64) {
+  const igak::Layout &L = f64 ? op->L64 : op->L32;
+  SeamGeom g{};
+  g.halo = f64 ? (const void *)op->halo64 : (const void *)op->halo32;
+  g.dirmask = op->dirmask;
+  g.nfx = op->nfx;
+  g.nfy = op->nfy;
+  g.nex = op->nex;
+  g.ney = op->ney;
+  g.p = op->p;
+  g.sx = __builtin_ctz((unsigned)L.tx);
+  g.sy = __builtin_ctz((unsigned)L.ty);
+  g.fx = L.tx + op->p;
+  g.fxy = g.fx * (L.ty + op->p);
+  g.ntx = L.ntx;
+  g.inv_nfx = 1.0 / op->nfx;
+  return g;
 }
 
 }  // namespace igai

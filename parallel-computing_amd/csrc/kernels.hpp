@@ -13,8 +13,9 @@
 //      order (P+1 passes, one element-row offset per pass);
 //   4. functions touched by this tile only are stored; functions on a tile
 //      seam are either atomically added (IEN scatter-add, y pre-zeroed on the
-//      seams) or parked in a per-tile halo slot and summed by k_seam in a
-//      fixed tile order (inverse-LM gather, bitwise reproducible).
+//      seams) or parked in a per-tile halo slot and summed in a fixed tile
+//      order by k_seam or by the CG update kernel that consumes y
+//      (inverse-LM gather, bitwise reproducible).
 // Dirichlet rows are written as zero; x is gathered for every function
 // (matrix_free/src/GlobalAssemblyMF.cpp:166-205).
 #pragma once

@@ -67,12 +67,88 @@ int dev_alloc(iga_op *op, void **p, size_t bytes);
 int set_device(int dev);
 // CG fusion of an apply (see igak::ApplyArgs): per-tile partials of x.y into
 // dot[ntiles], and the hand-off copy of hn doubles hsrc -> hdst
+// defer_seam (gather assembly): leave the seam functions of y unwritten;
+// the consumer of y sums them (seam_load, seam_value)
 struct CgHook {
   double *dot;
   const double *hsrc;
   double *hdst;
   int hn;
+  bool defer_seam;
 };
+// What a consumer of a defer_seam gather apply needs to sum the seam
+// functions itself: the per-tile halo slots ([tile][(TX+P)*(TY+P)], tile
+// row-major), the Dirichlet set and the tile layout of that precision.
+struct SeamGeom {
+  const void *halo;
+  const uint8_t *dirmask;  // nullptr: the boundary ring
+  int nfx, nfy, nex, ney, p;
+  int sx, sy;              // tiles of 2^sx x 2^sy elements
+  int fx, fxy, ntx;        // footprint row length, slots per tile, tiles per row
+  double inv_nfx;
+};
+SeamGeom seam_geom(const iga_op *op, bool f64);
+
+// The further partials of function i (see seam_load)
+template <typename V>
+struct SeamPart {
+  V a1, a2, a3;
+  unsigned f;   // SP_SEAM, SP_X, SP_Y, SP_DIR
+  unsigned dm;  // the Dirichlet mask byte (explicit masks)
+};
+enum { SP_SEAM = 1, SP_X = 2, SP_Y = 4, SP_DIR = 8 };
+
+// Loads of y at function i after a defer_seam gather apply: y[i] for a
+// function one tile touches, else the halo slot of the lowest touching tile
+// (returned) and the slots of the others in s (tile row, then tile column).
+// Branch-free: the addresses come from i alone and every lane issues four
+// loads (an absent partial reloads the first address, which the same cache
+// line serves), so all loads of a step are in flight together.  32-bit
+// function indices.
+template <typename V>
+__device__ __forceinline__ V seam_load(const V *__restrict__ y, const SeamGeom &g, int i,
+                                       SeamPart<V> &s) {
+  int gy = (int)((double)i * g.inv_nfx);  // within one of i / nfx
+  int gx = i - gy * g.nfx;
+  const bool under = gx < 0, over = gx >= g.nfx;
+  gy += over ? 1 : (under ? -1 : 0);
+  gx += under ? g.nfx : (over ? -g.nfx : 0);
+  const int xlo = max(gx - g.p, 0) >> g.sx, xhi = min(gx, g.nex - 1) >> g.sx;
+  const int ylo = max(gy - g.p, 0) >> g.sy, yhi = min(gy, g.ney - 1) >> g.sy;
+  const bool shx = xlo != xhi, shy = ylo != yhi, sh = shx || shy;
+  const V *h = static_cast<const V *>(g.halo);
+  const int64_t base = (int64_t)(ylo * g.ntx + xlo) * g.fxy + (gy - (ylo << g.sy)) * g.fx +
+                       (gx - (xlo << g.sx));
+  const int64_t dx = g.fxy - (1 << g.sx);                                   // next tile in x
+  const int64_t dy = (int64_t)g.ntx * g.fxy - (int64_t)g.fx * (1 << g.sy);  // next in y
+  const V *p0 = sh ? h + base : y + i;
+  const V *p1 = shx ? h + base + dx : p0;
+  const V *p2 = shy ? h + base + dy : p0;
+  const V *p3 = (shx && shy) ? h + base + dx + dy : p0;
+  const V a0 = *p0;
+  s.a1 = *p1;
+  s.a2 = *p2;
+  s.a3 = *p3;
+  // Dirichlet: the ring from the coordinates, an explicit mask by a load
+  // whose value is first needed in seam_value
+  const bool ring_dir = gx == 0 || gy == 0 || gx == g.nfx - 1 || gy == g.nfy - 1;
+  s.f = (sh ? SP_SEAM : 0u) | (shx ? SP_X : 0u) | (shy ? SP_Y : 0u) |
+        (!g.dirmask && ring_dir ? SP_DIR : 0u);
+  s.dm = g.dirmask ? (unsigned)g.dirmask[i] : 0u;
+  return a0;
+}
+
+// y at function i from what seam_load returned: bitwise the value k_seam
+// stores (an absent partial adds +0, which leaves the sum unchanged)
+template <typename V>
+__device__ __forceinline__ V seam_value(V a0, const SeamPart<V> &s) {
+  V acc = V(0) + a0;
+  acc += (s.f & SP_X) ? s.a1 : V(0);
+  acc += (s.f & SP_Y) ? s.a2 : V(0);
+  acc += ((s.f & SP_X) && (s.f & SP_Y)) ? s.a3 : V(0);
+  acc = ((s.f & SP_DIR) || s.dm) ? V(0) : acc;
+  return (s.f & SP_SEAM) ? acc : a0;
+}
 // whether the apply kernel configured for this precision supports CgHook
 bool cg_fusable(const iga_op *op, bool f64, bool ebe);
 bool cg_fusable_dt(const iga_op *op, int dt, bool ebe);

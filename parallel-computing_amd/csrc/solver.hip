@@ -231,12 +231,14 @@ __global__ void k_cg_pap(const V *__restrict__ p, const V *__restrict__ Ap, int6
 // K2: alpha; u += alpha p; r -= alpha Ap (rold keeps r for the flexible
 // beta); z = dinv r when the preconditioner is local; partials of r.r (or
 // z.z with prec_norm) and r.z
-template <typename V, typename D, bool LOCAL, bool ROLD, typename TU = V>
-__global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__restrict__ p,
+// SEAM: Ap comes from a gather apply that left its seam functions to this
+// kernel (igai::CgHook::defer_seam); sg describes that apply's halo.
+template <typename V, typename D, bool LOCAL, bool ROLD, typename TU = V, bool SEAM = false>
+__global__ __launch_bounds__(RB) void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__restrict__ p,
                             const V *__restrict__ Ap, V *__restrict__ z, V *__restrict__ rold,
                             const D *__restrict__ dinv, int prec_norm, int64_t n, double *part,
                             double *sc, int *ic, int nb, const int *outer_done, int64_t ld,
-                            const double *__restrict__ tdot, int ntiles) {
+                            const double *__restrict__ tdot, int ntiles, igai::SeamGeom sg) {
   CG_COLUMN(ld)
   u += voff_;
   r += voff_;
@@ -251,7 +253,15 @@ __global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__re
   // overlaps the fold and the stores.
   constexpr int U = 4;
   const int64_t step = (int64_t)gridDim.x * RB * U;
-  auto load = [&](int64_t b, TU (&uu)[U], V (&pv)[U], V (&rv)[U], V (&av)[U], D (&dv)[U]) {
+  using SP = igai::SeamPart<V>;
+  auto ldap = [&](int64_t i, SP &q) {
+    if constexpr (SEAM)
+      return igai::seam_load(Ap, sg, (int)i, q);
+    else
+      return Ap[i];
+  };
+  auto load = [&](int64_t b, TU (&uu)[U], V (&pv)[U], V (&rv)[U], V (&av)[U], D (&dv)[U],
+                  SP (&sp)[U]) {
     if (b + (int64_t)(U - 1) * RB < n) {  // whole step in range: unconditional loads
 #pragma unroll
       for (int k = 0; k < U; ++k) {
@@ -259,7 +269,7 @@ __global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__re
         uu[k] = u[i];
         pv[k] = p[i];
         rv[k] = r[i];
-        av[k] = Ap[i];
+        av[k] = ldap(i, sp[k]);
         if (LOCAL) dv[k] = dinv ? dinv[i] : D(1);
       }
     } else {
@@ -267,10 +277,11 @@ __global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__re
       for (int k = 0; k < U; ++k) {
         const int64_t i = b + (int64_t)k * RB;
         const bool in = i < n;
+        sp[k].f = sp[k].dm = 0;
         uu[k] = in ? u[i] : TU(0);
         pv[k] = in ? p[i] : V(0);
         rv[k] = in ? r[i] : V(0);
-        av[k] = in ? Ap[i] : V(0);
+        av[k] = in ? ldap(i, sp[k]) : V(0);
         if (LOCAL) dv[k] = (dinv && in) ? dinv[i] : D(1);
       }
     }
@@ -279,7 +290,8 @@ __global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__re
   TU uu[U];
   V pv[U], rv[U], av[U];
   D dv[U];
-  if (b0 < n) load(b0, uu, pv, rv, av, dv);
+  SP sp[U];
+  if (b0 < n) load(b0, uu, pv, rv, av, dv, sp);
   // p.Ap: partials of k_cg_pap, or the apply kernel's per-tile partials
   const double pap = tdot ? block_sum_all(fold_share(tdot, ntiles)) : fold(part + P_PAP * RG, nb);
   const double a = sc[S_RZ] / pap;
@@ -293,14 +305,16 @@ __global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__re
     TU uu2[U];
     V pv2[U], rv2[U], av2[U];
     D dv2[U];
-    if (bn < n) load(bn, uu2, pv2, rv2, av2, dv2);
+    SP sp2[U];
+    if (bn < n) load(bn, uu2, pv2, rv2, av2, dv2, sp2);
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const int64_t i = b0 + (int64_t)k * RB;
       if (i >= n) continue;
+      const V api = SEAM ? igai::seam_value(av[k], sp[k]) : av[k];
       u[i] = (TU)((double)uu[k] + a * (double)pv[k]);
       if (ROLD) rold[i] = rv[k];
-      const V ri = (V)((double)rv[k] - a * (double)av[k]);
+      const V ri = (V)((double)rv[k] - a * (double)api);
       r[i] = ri;
       if (LOCAL) {
         const V zi = (V)((double)ri * (double)dv[k]);
@@ -318,6 +332,7 @@ __global__ void k_cg_update(TU *__restrict__ u, V *__restrict__ r, const V *__re
       rv[k] = rv2[k];
       av[k] = av2[k];
       dv[k] = dv2[k];
+      sp[k] = sp2[k];
     }
     b0 = bn;
   }
@@ -579,12 +594,13 @@ __global__ void k_cgf_init_fin(double *cgs, int nb, double *sc, int *ic, double 
 // solve -- when r.r fell below the reliable-update threshold; the state is
 // then that of the last completed iteration.  zero_w: clear w for the next
 // scatter-add apply.
-template <typename V, typename D, bool JAC, typename TU = V>
-__global__ void k_cgf_update(TU *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
+// SEAM: as in k_cg_update, w's seam functions are summed here.
+template <typename V, typename D, bool JAC, typename TU = V, bool SEAM = false>
+__global__ __launch_bounds__(RB) void k_cgf_update(TU *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
                              V *__restrict__ sv, V *__restrict__ u, V *__restrict__ w,
                              const D *__restrict__ dinv, int zero_w, int64_t n, double *cgs,
                              const double *__restrict__ tdot, int ntiles, double *sc, int *ic,
-                             int nb) {
+                             int nb, igai::SeamGeom sg) {
   if (ic[I_DONE]) return;
   const double *cur = cgs;
   double *nw = cgs + CG_L;
@@ -594,6 +610,13 @@ __global__ void k_cgf_update(TU *__restrict__ x, V *__restrict__ r, V *__restric
     V u[U], w[U], p[U], s[U], r[U];
     TU x[U];
     D d[U];
+    igai::SeamPart<V> sp[U];
+  };
+  auto ldw = [&](int64_t i, igai::SeamPart<V> &q) {
+    if constexpr (SEAM)
+      return igai::seam_load(w, sg, (int)i, q);
+    else
+      return w[i];
   };
   auto load = [&](int64_t b, Step &q) {
     if (b + (int64_t)(U - 1) * RB < n) {  // whole step in range: unconditional loads
@@ -601,7 +624,7 @@ __global__ void k_cgf_update(TU *__restrict__ x, V *__restrict__ r, V *__restric
       for (int k = 0; k < U; ++k) {
         const int64_t i = b + (int64_t)k * RB;
         q.u[k] = u[i];
-        q.w[k] = w[i];
+        q.w[k] = ldw(i, q.sp[k]);
         q.p[k] = p[i];
         q.s[k] = sv[i];
         q.x[k] = x[i];
@@ -614,7 +637,8 @@ __global__ void k_cgf_update(TU *__restrict__ x, V *__restrict__ r, V *__restric
         const int64_t i = b + (int64_t)k * RB;
         const bool in = i < n;
         q.u[k] = in ? u[i] : V(0);
-        q.w[k] = in ? w[i] : V(0);
+        q.sp[k].f = q.sp[k].dm = 0;
+        q.w[k] = in ? ldw(i, q.sp[k]) : V(0);
         q.p[k] = in ? p[i] : V(0);
         q.s[k] = in ? sv[i] : V(0);
         q.x[k] = in ? x[i] : TU(0);
@@ -665,7 +689,8 @@ __global__ void k_cgf_update(TU *__restrict__ x, V *__restrict__ r, V *__restric
       const int64_t i = b0 + (int64_t)k * RB;
       if (i >= n) continue;
       const V pn = (V)((double)q.u[k] + b * (double)q.p[k]);
-      const V sn = (V)((double)q.w[k] + b * (double)q.s[k]);
+      const V wk = SEAM ? igai::seam_value(q.w[k], q.sp[k]) : q.w[k];
+      const V sn = (V)((double)wk + b * (double)q.s[k]);
       const V rn = (V)((double)q.r[k] - a * (double)sn);
       const V un = JAC ? (V)((double)rn * (double)q.d[k]) : rn;
       p[i] = pn;
@@ -1047,6 +1072,10 @@ struct iga_solver {
   std::vector<void *> ballocs;
   hipGraphExec_t gblk = nullptr;
   int gblk_k = 0, gblk_cfg = -1;
+  // pinned host copies of the done / iteration flags after two chunks in
+  // flight, and the events that mark them complete (run_chunks)
+  int *hflag = nullptr;
+  hipEvent_t fev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1092,7 +1121,8 @@ int fem_pc(iga_solver *s, const V *r, V *z, V *rin, V *w, V *pp, V *q, const V *
                        s->col, val, pp, q, s->fem_rows, ic + I_DONE, od);
     LAUNCH((k_cg_pap<V>), nb, pp, q, n, s->part_in, sc, ic, od, (int64_t)0);
     LAUNCH((k_cg_update<V, double, true, false>), nb, z, rin, pp, q, w, (V *)nullptr, s->fdinv,
-           1, n, s->part_in, sc, ic, nb, od, (int64_t)0, (const double *)nullptr, 0);
+           1, n, s->part_in, sc, ic, nb, od, (int64_t)0, (const double *)nullptr, 0,
+           igai::SeamGeom{});
     LAUNCH((k_cg_pupdate<V>), nb, pp, w, n, s->part_in, sc, ic, nb, 0, od, (int64_t)0,
            (V *)nullptr);
   }
@@ -1161,7 +1191,10 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   const int dt = apply_dt(s, f64);
   // p.Ap accumulated by the apply kernel itself when it can, else k_cg_pap
   const bool fdot = s->o.fused == 1 && s->tdot && igai::cg_fusable_dt(s->op, dt, ebe);
-  const igai::CgHook hook{s->tdot, nullptr, nullptr, 0};
+  // gather form with the fused p.Ap: k_cg_update sums the seam functions of
+  // Ap itself (no seam kernel, no pass over y)
+  const bool fseam = fdot && gather;
+  const igai::CgHook hook{s->tdot, nullptr, nullptr, 0, fseam};
   const double *tdot = fdot ? s->tdot : nullptr;
   const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
   // atomic form: Ap was zeroed by the previous k_cg_pupdate (or cg_run)
@@ -1174,12 +1207,19 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   const V *dinv = s->o.pc != IGA_PC_JACOBI
                       ? nullptr
                       : (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32);
-  if (loc)
+  const igai::SeamGeom sg = fseam ? igai::seam_geom(s->op, f64) : igai::SeamGeom{};
+  if (loc && fseam)
+    LAUNCH((k_cg_update<V, V, true, false, TU, true>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n,
+           s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
+  else if (loc)
     LAUNCH((k_cg_update<V, V, true, false, TU>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n, s->part,
-           s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles);
+           s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
+  else if (fseam)
+    LAUNCH((k_cg_update<V, V, false, true, TU, true>), nb, u, r, p, Ap, z, rold, (const V *)nullptr,
+           0, n, s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
   else
     LAUNCH((k_cg_update<V, V, false, true, TU>), nb, u, r, p, Ap, z, rold, (const V *)nullptr, 0, n,
-           s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles);
+           s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
   if (!loc) {
     rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
@@ -1190,6 +1230,38 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
+// Enqueue chunks (launch()) until the device's done flag is set.  The flags
+// after chunk k are copied to pinned memory and read while chunk k + 1 runs,
+// so the device does not idle for the host round trip between chunks; the
+// one chunk enqueued past the end finds the flag set and its kernels return
+// at once (later work on st stays behind it).
+template <typename L>
+int run_chunks(iga_solver *s, hipStream_t st, L &&launch, int *iters, int *done) {
+  auto issue = [&](int slot) -> int {
+    const int rc = launch();
+    if (rc) return rc;
+    if (hipMemcpyAsync(s->hflag + 2 * slot, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipEventRecord(s->fev[slot], st) != hipSuccess)
+      return IGA_ERR_HIP;
+    return IGA_OK;
+  };
+  int rc = issue(0);
+  if (rc) return rc;
+  for (int k = 0;; ++k) {
+    const int cur = k & 1;
+    rc = issue(cur ^ 1);
+    if (rc) return rc;
+    if (hipEventSynchronize(s->fev[cur]) != hipSuccess) return IGA_ERR_HIP;
+    const int *h = s->hflag + 2 * cur;
+    if (h[I_DONE]) {
+      *iters = h[I_ITER];
+      *done = h[I_DONE];
+      return IGA_OK;
+    }
+  }
+}
+
 template <typename V, typename TU>
 int cg_chunk(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st) {
   for (int k = 0; k < CHUNK; ++k) {
@@ -1198,6 +1270,10 @@ int cg_chunk(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, h
   }
   return IGA_OK;
 }
+
+template <typename V, typename TU>
+int cg_resume(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
+              hipGraphExec_t *gexec, int *iters, int *done);
 
 // Run a CG to completion: init (k_cg_init + k_cg_init_fin) must already have
 // been enqueued on st.
@@ -1236,44 +1312,21 @@ int cg_run(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hip
       return IGA_ERR_HIP;
     }
   }
-  int hic[2];
-  for (;;) {
-    if (s->o.use_graph) {
-      if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
-    } else {
-      int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
-      if (rc) return rc;
-    }
-    if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return IGA_ERR_HIP;
-    if (hic[I_DONE]) break;
-  }
-  *iters = hic[I_ITER];
-  *done = hic[I_DONE];
-  return IGA_OK;
+  return cg_resume<V, TU>(s, f64, u, r, p, Ap, z, rold, st, gexec, iters, done);
 }
 
 // Continue a CG whose device state was restarted by k_ru_fin.
 template <typename V, typename TU>
 int cg_resume(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
               hipGraphExec_t *gexec, int *iters, int *done) {
-  int hic[2];
-  for (;;) {
-    if (s->o.use_graph && *gexec) {
-      if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
-    } else {
-      int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
-      if (rc) return rc;
-    }
-    if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return IGA_ERR_HIP;
-    if (hic[I_DONE]) break;
-  }
-  *iters = hic[I_ITER];
-  *done = hic[I_DONE];
-  return IGA_OK;
+  return run_chunks(
+      s, st,
+      [&]() -> int {
+        if (s->o.use_graph && *gexec)
+          return hipGraphLaunch(*gexec, st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+        return cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
+      },
+      iters, done);
 }
 
 // ---- fused iteration (local preconditioners, single solves)
@@ -1282,26 +1335,37 @@ bool use_fused(const iga_solver *s, bool f64) {
          igai::cg_fusable_dt(s->op, apply_dt(s, f64), s->o.kernel == IGA_KERNEL_EBE);
 }
 
-// one fused iteration: apply (+ seam kernel when gathering) and k_cgf_update;
-// u, w, p, sv: the recursion's M r, A u, direction and A p
+// one fused iteration: apply and k_cgf_update (which, in the gather form,
+// also sums w's seam functions); u, w, p, sv: the recursion's M r, A u,
+// direction and A p
 template <typename V, typename TU>
 int cgf_iteration(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w,
                   hipStream_t st) {
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
-  igai::CgHook hook{s->tdot, s->cgs + CG_L, s->cgs, CG_L};
+  igai::CgHook hook{s->tdot, s->cgs + CG_L, s->cgs, CG_L, gather};
   // scatter-add form: w was cleared by the previous update (or cgf_run)
   int rc = igai::apply_raw_dt(s->op, apply_dt(s, f64), ebe, gather, u, w, 0, 1, st,
                               s->ic + I_DONE, /*y_zeroed=*/!gather, &hook);
   if (rc) return rc;
   const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
-  if (s->o.pc == IGA_PC_JACOBI)
-    LAUNCH((k_cgf_update<V, V, true, TU>), s->nb, x, r, p, sv, u, w,
-           f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32,
-           gather ? 0 : 1, s->n, s->cgs, s->tdot, ntiles, s->sc, s->ic, s->nb);
+  const V *dinv = s->o.pc == IGA_PC_JACOBI
+                      ? (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32)
+                      : nullptr;
+  const igai::SeamGeom sg = gather ? igai::seam_geom(s->op, f64) : igai::SeamGeom{};
+  const int zw = gather ? 0 : 1;
+  if (dinv && gather)
+    LAUNCH((k_cgf_update<V, V, true, TU, true>), s->nb, x, r, p, sv, u, w, dinv, zw, s->n, s->cgs,
+           s->tdot, ntiles, s->sc, s->ic, s->nb, sg);
+  else if (dinv)
+    LAUNCH((k_cgf_update<V, V, true, TU>), s->nb, x, r, p, sv, u, w, dinv, zw, s->n, s->cgs,
+           s->tdot, ntiles, s->sc, s->ic, s->nb, sg);
+  else if (gather)
+    LAUNCH((k_cgf_update<V, V, false, TU, true>), s->nb, x, r, p, sv, u, w, dinv, zw, s->n,
+           s->cgs, s->tdot, ntiles, s->sc, s->ic, s->nb, sg);
   else
-    LAUNCH((k_cgf_update<V, V, false, TU>), s->nb, x, r, p, sv, u, w, (const V *)nullptr,
-           gather ? 0 : 1, s->n, s->cgs, s->tdot, ntiles, s->sc, s->ic, s->nb);
+    LAUNCH((k_cgf_update<V, V, false, TU>), s->nb, x, r, p, sv, u, w, dinv, zw, s->n, s->cgs,
+           s->tdot, ntiles, s->sc, s->ic, s->nb, sg);
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
@@ -1341,22 +1405,14 @@ int cgf_run(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipSt
       return IGA_ERR_HIP;
     }
   }
-  int hic[2];
-  for (;;) {
-    if (s->o.use_graph) {
-      if (hipGraphLaunch(*gexec, st) != hipSuccess) return IGA_ERR_HIP;
-    } else {
-      int rc = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, st);
-      if (rc) return rc;
-    }
-    if (hipMemcpyAsync(hic, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return IGA_ERR_HIP;
-    if (hic[I_DONE]) break;
-  }
-  *iters = hic[I_ITER];
-  *done = hic[I_DONE];
-  return IGA_OK;
+  return run_chunks(
+      s, st,
+      [&]() -> int {
+        if (s->o.use_graph)
+          return hipGraphLaunch(*gexec, st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+        return cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, st);
+      },
+      iters, done);
 }
 
 // |f - A u|^2 and |f|^2 into h (host), r = f - A u
@@ -1428,7 +1484,8 @@ int bcg_chunk(iga_solver *s, int k, hipStream_t st) {
                        s->bsc, s->bic, (const int *)nullptr, n);
     hipLaunchKernelGGL((k_cg_update<float, float, true, false>), grid, dim3(RB), 0, st, s->bd32,
                        s->br32, s->bp32, s->bAp32, s->bz32, (float *)nullptr, dinv, 0, n, s->bpart,
-                       s->bsc, s->bic, nb, (const int *)nullptr, n, (const double *)nullptr, 0);
+                       s->bsc, s->bic, nb, (const int *)nullptr, n, (const double *)nullptr, 0,
+                       igai::SeamGeom{});
     hipLaunchKernelGGL((k_cg_pupdate<float>), grid, dim3(RB), 0, st, s->bp32, s->bz32, n,
                        s->bpart, s->bsc, s->bic, nb, 0, (const int *)nullptr, n,
                        (float *)nullptr);
@@ -1500,6 +1557,9 @@ int iga_solver_destroy(iga_solver *s) {
   if (s->gf64) (void)hipGraphExecDestroy(s->gf64);
   if (s->gf32) (void)hipGraphExecDestroy(s->gf32);
   if (s->gblk) (void)hipGraphExecDestroy(s->gblk);
+  for (hipEvent_t e : s->fev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->hflag) (void)hipHostFree(s->hflag);
   for (void *p : s->allocs) (void)hipFree(p);
   for (void *p : s->ballocs) (void)hipFree(p);
   delete s;
@@ -1564,6 +1624,10 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     SA(s->z32, n * 4);
     SA(s->rold32, n * 4);
   }
+  if (hipHostMalloc((void **)&s->hflag, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&s->fev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->fev[1], hipEventDisableTiming) != hipSuccess)
+    return fail(IGA_ERR_HIP);
   if (o.fused < 0 || o.fused > 2) return fail(IGA_ERR_INVALID_ARGUMENT);
   if (o.inner_f16_geometry &&
       (o.kind != IGA_SOLVE_PCG_F32_REFINE || !op->G16 || o.kernel == IGA_KERNEL_EBE))
