@@ -183,6 +183,9 @@ typedef struct {
                                    3: two threads per element (SF) on 16x8
                                    tiles, for high degrees and FP64; general
                                    rational meshes accept 0 and 3;
+                                   4 (FP32 only): the element operator on the
+                                   matrix cores (SF, p = 3, 16x16, not general
+                                   rational; EBE applies use 0);
                                    -1 (default): 3 for FP32 at p >= 5 and for
                                    FP64 at p = 3 (not general rational),
                                    else 0 */

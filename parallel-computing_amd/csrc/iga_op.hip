@@ -224,7 +224,7 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   const bool prof = op->prof_on && op->prof_used < op->prof_cap;
   if (prof) IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used], s));
   if (g16)
-    IGA_HIP_TRY(op->ops->apply16(a, tile, ntiles, s));
+    IGA_HIP_TRY(op->ops->apply16(a, tile, ntiles, op->variant32, s));
   else if (a.wf)
     IGA_HIP_TRY((f64 ? op->ops->rat64 : op->ops->rat32)(a, ebe, tile, ntiles, s));
   else
@@ -442,7 +442,7 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   };
   op->variant64 = o.variant_f64 < 0 ? dflt_variant(true) : o.variant_f64;
   op->variant32 = o.variant_f32 < 0 ? dflt_variant(false) : o.variant_f32;
-  if (o.variant_f64 < -1 || op->variant64 > 3 || o.variant_f32 < -1 || op->variant32 > 3) {
+  if (o.variant_f64 < -1 || op->variant64 > 3 || o.variant_f32 < -1 || op->variant32 > 4) {
     delete op;
     return IGA_ERR_INVALID_ARGUMENT;
   }
@@ -477,7 +477,8 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
     if (o.variant_f64 == 1 || o.variant_f64 == 2 || o.variant_f32 == 1 || o.variant_f32 == 2)
       return fail(IGA_ERR_UNSUPPORTED);
   }
-  if ((op->variant64 == 1 && !igak::pf_compiled(p, op->tile64)) ||
+  if ((op->variant32 == 4 && (rational == 2 || !igak::mfma_compiled(p, op->tile32))) ||
+      (op->variant64 == 1 && !igak::pf_compiled(p, op->tile64)) ||
       (op->variant32 == 1 && !igak::pf_compiled(p, op->tile32)) || op->variant64 == 2 ||
       (op->variant32 == 2 && !igak::w4_compiled(p, op->tile32, false)))
     return fail(IGA_ERR_UNSUPPORTED);
@@ -834,7 +835,7 @@ int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, vo
   IGA_HIP_TRY(hipEventCreate(&e0));
   IGA_HIP_TRY(hipEventCreate(&e1));
   auto launch = [&]() {
-    if (g16) return op->ops->apply16(a, tile, ntiles, s);
+    if (g16) return op->ops->apply16(a, tile, ntiles, op->variant32, s);
     return a.wf ? (f64 ? op->ops->rat64 : op->ops->rat32)(a, ebe, tile, ntiles, s)
                 : (f64 ? op->ops->apply64 : op->ops->apply32)(
                       a, ebe, tile, ntiles, f64 ? op->variant64 : op->variant32, s);

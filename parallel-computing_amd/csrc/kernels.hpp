@@ -405,8 +405,10 @@ struct TileStage {
     }
   }
 
+  // LX, LY: row strides of the LDS tables (TX, TY, or padded)
+  template <int LX = TX, int LY = TY>
   __device__ __forceinline__ void store(const ApplyArgs &A, int ex0, int ey0, int t, T *xs,
-                                        T (*ltx)[TX], T (*lty)[TY]) const {
+                                        T (*ltx)[LX], T (*lty)[LY]) const {
 #pragma unroll
     for (int o = 0; o < NOWN; ++o) {
       const int k = t + o * NT;
@@ -780,6 +782,192 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
     if (k < NFP)
       put_footprint<T, P, TX, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, k % FX, k / FX, acc[o],
                                        A.dot ? xs[k] : T(0), dacc, interior);
+  }
+  if (A.dot) store_tile_dot<NT>(A, tile, dacc);
+}
+
+// --------------------------------------------------------------------------
+// Matrix-core form of the sum-factorised apply, degree 3 (4 Gauss points per
+// direction), 16 x 16 tiles, FP32 arithmetic on FP32 or FP16-stored G_q
+// (polynomial and tensor-weight meshes).  The element operator is eight
+// 4x4x4 products per element, each taken by v_mfma_f32_4x4x1_16b_f32 (16
+// independent 4x4 blocks, rank-1 update per instruction): block = element,
+// so one instruction advances 16 elements (one tile row) by one k.  With the
+// block layout C[r][c] in lane 4e + c, register r (A[r] from lane 4e + r,
+// B[c] from lane 4e + c) every product's accumulator is the next product's
+// operand without any lane exchange:
+//   T1[b][i] = sum_a X[b][a] d1[i][a]     (T0 with n1)    lane i, reg b
+//   Ux[j][i] = sum_b n2[j][b] T1[b][i]    (Uy: d2, T0)    lane i, reg j
+//   Fx, Fy   = -G_q (Ux, Uy) at q = (j, i)                 pointwise
+//   S0[i][b] = sum_j Fx[j][i] n2[j][b]    (S1: Fy, d2)    lane b, reg i
+//   Ye[a][b] = sum_i d1[i][a] S0[i][b] + n1[i][a] S1[i][b] lane b, reg a
+// (the element vector ye[b][a] of element e in lane 4e + b, register a).
+// Wave w takes tile rows 4w .. 4w+3 in turn.  The FP32 G_q layout puts the
+// four points i of row j of one element in one 16-byte vector, so a row of
+// 16 elements loads as one contiguous 256-byte access per (component, j).
+// The element vectors are reduced over the footprint through LDS in a fixed
+// order (element row offset b, then column offset a): bitwise reproducible.
+// --------------------------------------------------------------------------
+// A pointer that is the same on every lane, as a value the compiler keeps in
+// scalar registers (loads from it plus a per-lane offset then need no
+// per-lane 64-bit address arithmetic).
+typedef const __attribute__((address_space(1))) char global_cchar;
+__device__ __forceinline__ global_cchar *wave_uniform(const void *p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (global_cchar *)(((unsigned long long)hi << 32) | lo);
+}
+
+// Register budget of four waves per SIMD (128 VGPRs, no spills): 4 workgroups
+// per CU.
+template <typename TG>
+__global__ __launch_bounds__(256, 4) void k_apply_mfma(ApplyArgs A) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  constexpr int P = 3, NQ = 4, NB = 4, TX = 16, TY = 16, NT = 256, FX = TX + P, FY = TY + P;
+  constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY, NOWN = (NFP + NT - 1) / NT;
+  constexpr int PS = TX * TY + 16;  // element-vector plane stride (conflict-free stores)
+  // tables with rows of 20 values: the rows 4r + a that the four lanes r of
+  // an element read at once fall in different LDS banks
+  constexpr int LT = 20;
+  __shared__ float xs[NFP];
+  __shared__ float tbx[2 * TAB][LT];
+  __shared__ float tby[2 * TAB][LT];
+  __shared__ float yb[NB * NB * PS];
+
+  if (A.skip && *A.skip) return;
+  cg_handoff(A, NT);
+  // w is wave-uniform: the G_q row bases then live in scalar registers
+  const int t = threadIdx.x, lane = t & 63, e = lane >> 2, r = lane & 3;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % A.ntx, ty = tile / A.ntx;
+  const int ex0 = tx * TX, ey0 = ty * TY;
+  const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
+                        (!A.mask_out || !A.dirmask);
+
+  // x footprint and tables, then this lane's G_q values of its four rows:
+  // point (j, r) of element e of row 4w + k.  gq_index(tile, c, 4j + r,
+  // 16(4w + k) + e) is a wave-uniform base plus a per-lane offset 4e + r
+  // (FP32: the four points of an element row are one 16-byte vector) or
+  // 8e + r (FP16: rows j, j + 1 share an 8-value vector).
+  TileStage<float, P, NQ, TX, TY, false, NT> stg;
+  stg.load(A, ex0, ey0, t);
+  const TG *Gt = static_cast<const TG *>(A.G);
+  constexpr bool F32G = std::is_same_v<TG, float>;
+  const unsigned goff = (F32G ? (unsigned)lane : (unsigned)(8 * e + r)) * (unsigned)sizeof(TG);
+  float g[4][3][NQ];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const size_t base = (size_t)(tile * 3 + c) * NQ2 * NT +
+                            (F32G ? (size_t)(j * NT + (4 * w + k) * TX) * 4
+                                  : (size_t)((j / 2) * NT + (4 * w + k) * TX) * 8 + (j % 2) * 4);
+        g[k][c][j] = (float)*(const __attribute__((address_space(1))) TG *)(
+            wave_uniform(Gt + base) + goff);
+      }
+  stg.template store<LT, LT>(A, ex0, ey0, t, xs, tbx, tby);
+  __syncthreads();
+
+  // x-direction tables of column e: B operands of the first products, A
+  // operands of the last
+  float b1n[NB], b1d[NB], a5n[NQ], a5d[NQ];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    b1n[a] = tbx[r * NB + a][e];
+    b1d[a] = tbx[TAB + r * NB + a][e];
+  }
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    a5n[i] = tbx[i * NB + r][e];
+    a5d[i] = tbx[TAB + i * NB + r][e];
+  }
+  const bool col_ok = ex0 + e < A.nex;
+  const float gs = F32G ? 1.0f : A.gscale;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ly = 4 * w + k;
+    float a2n[NB], a2d[NB], b4n[NQ], b4d[NQ], xa[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      a2n[b] = tby[r * NB + b][ly];
+      a2d[b] = tby[TAB + r * NB + b][ly];
+    }
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      b4n[j] = tby[j * NB + r][ly];
+      b4d[j] = tby[TAB + j * NB + r][ly];
+    }
+    // row r of the element's inputs, relative to its first value (the basis
+    // is a partition of unity: the element vector is unchanged, the FP32
+    // contractions see the local variation of x)
+    const float x0 = xs[ly * FX + e];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) xa[a] = xs[(ly + r) * FX + e + a] - x0;
+    f4 t0 = {0.f, 0.f, 0.f, 0.f}, t1 = t0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      t0 = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[a], b1n[a], t0, 0, 0, 0);
+      t1 = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[a], b1d[a], t1, 0, 0, 0);
+    }
+    f4 ux = {0.f, 0.f, 0.f, 0.f}, uy = ux;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      ux = __builtin_amdgcn_mfma_f32_4x4x1f32(a2n[b], t1[b], ux, 0, 0, 0);
+      uy = __builtin_amdgcn_mfma_f32_4x4x1f32(a2d[b], t0[b], uy, 0, 0, 0);
+    }
+    f4 fx, fy;
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      fx[j] = -(g[k][0][j] * ux[j] + g[k][1][j] * uy[j]);
+      fy[j] = -(g[k][1][j] * ux[j] + g[k][2][j] * uy[j]);
+    }
+    f4 s0 = {0.f, 0.f, 0.f, 0.f}, s1 = s0;
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      s0 = __builtin_amdgcn_mfma_f32_4x4x1f32(fx[j], b4n[j], s0, 0, 0, 0);
+      s1 = __builtin_amdgcn_mfma_f32_4x4x1f32(fy[j], b4d[j], s1, 0, 0, 0);
+    }
+    f4 ye = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      ye = __builtin_amdgcn_mfma_f32_4x4x1f32(a5d[i], s0[i], ye, 0, 0, 0);
+      ye = __builtin_amdgcn_mfma_f32_4x4x1f32(a5n[i], s1[i], ye, 0, 0, 0);
+    }
+    // ye[a] = element vector (b = r, a) of element (e, ly); elements outside
+    // the mesh (partial tiles) contribute nothing
+    const bool ok = col_ok && ey0 + ly < A.ney;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) yb[(a * NB + r) * PS + ly * TX + e] = ok ? ye[a] * gs : 0.f;
+  }
+  __syncthreads();
+
+  float *y = static_cast<float *>(A.y);
+  float *halo = static_cast<float *>(A.halo);
+  double dacc = 0.0;
+#pragma unroll
+  for (int o = 0; o < NOWN; ++o) {
+    const int k = t + o * NT;
+    if (k < NFP) {
+      const int fx = k % FX, fy = k / FX;
+      float v = 0.f;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int ly = fy - b;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          const int lx = fx - a;
+          const bool in = ly >= 0 && ly < TY && lx >= 0 && lx < TX;
+          const float u = yb[(a * NB + b) * PS + (in ? ly * TX + lx : 0)];
+          v += in ? u : 0.f;
+        }
+      }
+      put_footprint<float, P, TX, TY, false>(A, y, halo, nullptr, tile, ex0, ey0, fx, fy, v,
+                                             A.dot ? xs[k] : 0.f, dacc, interior);
+    }
   }
   if (A.dot) store_tile_dot<NT>(A, tile, dacc);
 }

@@ -37,6 +37,8 @@ inline bool tile_compiled(int p, int id) {
 inline bool pf_compiled(int p, int id) { return p <= 4 || id == TILE_16x16; }
 inline bool w4_compiled(int p, int id, bool f64) { return p <= 4 && id == TILE_16x16 && !f64; }
 inline bool g16_compiled(int id) { return id == TILE_16x16 || id == TILE_16x8; }
+// matrix-core variant (4): degree 3, 16x16 tiles, FP32 arithmetic
+inline bool mfma_compiled(int p, int id) { return p == 3 && id == TILE_16x16; }
 inline bool rat_compiled(int p, int id) {
   return id == TILE_16x16 || id == TILE_16x8 || (p <= 4 && id == TILE_32x16);
 }
@@ -52,10 +54,21 @@ struct Launch {
                        dim3(TX * TY * SPLIT), 0, s, a);
     return hipGetLastError();
   }
+  // matrix-core SF apply (variant 4, degree 3, 16x16 tiles) on G_q stored as TG
+  template <typename TG>
+  static hipError_t apply_mfma_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
+    if constexpr (P == 3) {
+      hipLaunchKernelGGL((k_apply_mfma<TG>), dim3(ntiles), dim3(256), 0, s, a);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
   // FP32 SF apply on FP16-stored G_q (polynomial / tensor-weight meshes):
-  // the 16x16 one-workgroup-per-tile kernel, or the split-element one on
-  // 16x8 tiles
-  static hipError_t apply_g16(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {
+  // the 16x16 one-workgroup-per-tile kernel (its matrix-core form for
+  // variant 4), or the split-element one on 16x8 tiles
+  static hipError_t apply_g16(const ApplyArgs &a, int tile, int ntiles, int variant,
+                              hipStream_t s) {
+    if (variant == 4 && mfma_compiled(P, tile)) return apply_mfma_t<_Float16>(a, ntiles, s);
     if (tile == TILE_16x16) return apply_t<float, 16, 16, false, false, 1, 1, _Float16>(a, ntiles, s);
     if (tile == TILE_16x8) return apply_t<float, 16, 8, false, false, 1, 2, _Float16>(a, ntiles, s);
     return hipErrorInvalidValue;
@@ -115,7 +128,7 @@ struct Launch {
   // variant: 0 one workgroup per tile, 1 persistent pipelined (SF only),
   // 2 one workgroup per tile at four waves per SIMD (FP32 SF, 16x16, p <= 4)
   // 3 two threads per element (SF; 16x8 tiles, which the EBE apply of such
-  // an operator also uses).  One precision per call, so that FP32 and FP64
+  // an operator also uses), 4 matrix-core SF (FP32, 16x16, p = 3).  One precision per call, so that FP32 and FP64
   // kernels compile in separate translation units.
   template <typename T>
   static hipError_t apply_prec(const ApplyArgs &a, bool ebe, int tile, int ntiles, int variant,
@@ -126,7 +139,9 @@ struct Launch {
     if constexpr (P <= 4 && sizeof(T) == 4)
       if (variant == 2 && !ebe && tile == TILE_16x16)
         return apply_t<float, 16, 16, false, false, 4>(a, ntiles, s);
-    if (variant == 2) variant = 0;  // EBE applies of a variant-2 operator
+    if constexpr (sizeof(T) == 4)
+      if (variant == 4 && !ebe && mfma_compiled(P, tile)) return apply_mfma_t<float>(a, ntiles, s);
+    if (variant == 2 || variant == 4) variant = 0;  // EBE applies of a variant-2 / 4 operator
     if (variant == 1 && !ebe) return apply_pf_tile<T>(a, tile, ntiles, s);
     return ebe ? apply_tile<T, true>(a, tile, ntiles, s) : apply_tile<T, false>(a, tile, ntiles, s);
   }
@@ -257,7 +272,7 @@ struct Ops {
   hipError_t (*rat64)(const ApplyArgs &, bool ebe, int tile, int ntiles, hipStream_t);
   hipError_t (*rat32)(const ApplyArgs &, bool ebe, int tile, int ntiles, hipStream_t);
   hipError_t (*block)(const BlockArgs &, int cfg, hipStream_t);
-  hipError_t (*apply16)(const ApplyArgs &, int tile, int ntiles, hipStream_t);
+  hipError_t (*apply16)(const ApplyArgs &, int tile, int ntiles, int variant, hipStream_t);
   hipError_t (*g16conv)(const float *, _Float16 *, const Layout &, float, hipStream_t);
 };
 
@@ -292,7 +307,7 @@ const Ops *ops_for_degree(int p);
   hipError_t rat64_p##P(const ApplyArgs &, bool, int, int, hipStream_t);                  \
   hipError_t rat32_p##P(const ApplyArgs &, bool, int, int, hipStream_t);                  \
   hipError_t block_p##P(const BlockArgs &, int, hipStream_t);                             \
-  hipError_t apply16_p##P(const ApplyArgs &, int, int, hipStream_t);                      \
+  hipError_t apply16_p##P(const ApplyArgs &, int, int, int, hipStream_t);                 \
   hipError_t g16conv_p##P(const float *, _Float16 *, const Layout &, float, hipStream_t); \
   const Ops *ops_p##P() {                                                                 \
     static const Ops o = [] {                                                             \
@@ -321,8 +336,9 @@ const Ops *ops_for_degree(int p);
 // FP32 apply kernels on FP16-stored G_q of degree P (kern_p<P>_g16.hip)
 #define IGAK_DEFINE_G16(P)                                                                \
   namespace igak {                                                                        \
-  hipError_t apply16_p##P(const ApplyArgs &a, int tile, int ntiles, hipStream_t s) {      \
-    return Launch<P>::apply_g16(a, tile, ntiles, s);                                      \
+  hipError_t apply16_p##P(const ApplyArgs &a, int tile, int ntiles, int variant,          \
+                          hipStream_t s) {                                                \
+    return Launch<P>::apply_g16(a, tile, ntiles, variant, s);                             \
   }                                                                                       \
   hipError_t g16conv_p##P(const float *G32, _Float16 *G16, const Layout &L, float inv,    \
                           hipStream_t s) {                                                \

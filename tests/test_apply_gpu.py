@@ -228,6 +228,56 @@ def test_split_element_variant(built, name, kw):
         igamf.Operator(m, variant_f32=3, tile_f32=(16, 16))
 
 
+MFMA_CASES = [c for c in CASES if c[1]["p"] == 3 and not c[0].startswith("rational")]
+
+
+@pytest.mark.parametrize("name,kw", MFMA_CASES, ids=[c[0] for c in MFMA_CASES])
+def test_matrix_core_variant(built, name, kw):
+    """Variant 4 (degree 3: the element operator on the matrix cores) against
+    the oracle on FP32 and FP16-stored G_q, both assemblies, its x.Ax
+    product, bitwise reproducibility of the gather form; EBE applies of such
+    an operator and its FP64 applies use their own kernels."""
+    m, om, _ = built(name, kw)
+    op = igamf.Operator(m, variant_f32=4, f16_geometry=True)
+    assert op.info.variant_f32 == 4 and (op.info.tile_x_f32, op.info.tile_y_f32) == (16, 16)
+    x = igamf.rng_normal(7, op.nfunc)
+    y_ref = O.apply_mf(om, x)
+    xd = torch.from_numpy(x).cuda()
+    x32 = xd.float()
+    for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
+        y32 = op.apply(x32, assembly=asm).double().cpu().numpy()
+        assert rel(y32, y_ref) < TOL32, (name, asm, rel(y32, y_ref))
+        y16 = op.apply(x32, assembly=asm, g16=True).double().cpu().numpy()
+        assert rel(y16, y_ref) < TOL16, (name, asm, rel(y16, y_ref))
+        y, xy = op.apply_dot(x32, assembly=asm)
+        e = abs(xy.item() - float(x @ y_ref)) / abs(float(x @ y_ref))
+        assert e < 2e-6, (name, asm, e)
+        ye = op.apply(x32, kernel=igamf.EBE, assembly=asm).double().cpu().numpy()
+        assert rel(ye, y_ref) < TOL32, (name, asm, rel(ye, y_ref))
+        y64 = op.apply(xd, assembly=asm).cpu().numpy()
+        assert rel(y64, y_ref) < TOL64, (name, asm, rel(y64, y_ref))
+    a = op.apply(x32)
+    b = op.apply(x32)
+    assert torch.equal(a, b)
+    # Dirichlet rows exactly zero
+    assert np.all(op.apply(x32).cpu().numpy()[om.Dir] == 0.0)
+
+
+def test_matrix_core_variant_unsupported():
+    """Variant 4 exists for degree 3 on 16x16 tiles, FP32, not general rational."""
+    for kw in (dict(kind=igamf.GEOM_UNIT_SQUARE, p=2, nex=20),
+               dict(kind=igamf.GEOM_UNIT_SQUARE, p=4, nex=20),
+               RATIONAL[1][1]):
+        with pytest.raises(igamf.IgaError) as e:
+            igamf.Operator(igamf.Mesh.build(**kw), variant_f32=4)
+        assert e.value.status == 6, kw
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 20)
+    with pytest.raises(igamf.IgaError):
+        igamf.Operator(m, variant_f32=4, tile_f32=(32, 16))
+    with pytest.raises(igamf.IgaError):
+        igamf.Operator(m, variant_f64=4)
+
+
 def test_general_rational_mode_and_tiles():
     """Non-tensor weights select the general rational kernels (5 geometry
     components per point); both of their tiles agree with the oracle; the

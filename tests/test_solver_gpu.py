@@ -169,6 +169,32 @@ def test_fused_iteration_matches_classic(geom):
         assert st.status == 0 and st.rel_residual <= 1.0001e-10
 
 
+def test_matrix_core_inner_solve():
+    """FP32 inner solves on the matrix-core apply (variant 4, degree 3), on
+    FP32 and FP16-stored G_q, with both assemblies and the fused iteration:
+    the same converged solution as the default operator's FP64 PCG, and
+    iteration counts within 12 % of the default FP32 kernel's."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 45, 38)
+    op0 = igamf.Operator(m, f16_geometry=True)
+    op4 = igamf.Operator(m, variant_f32=4, f16_geometry=True)
+    F = op0.load(igamf.LOAD_CONST, 1.0)
+    s = igamf.Solver(op0, None, kind=igamf.SOLVE_PCG_F64, rtol=1e-10)
+    u64, st = s.solve(F)
+    assert st.status == 0
+    for g16 in (0, 1):
+        for asm in (igamf.INVLM_GATHER, igamf.IEN_ATOMIC):
+            its = []
+            for op in (op0, op4):
+                s = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F32_REFINE, rtol=1e-10,
+                                 assembly=asm, inner_f16_geometry=g16)
+                u, st = s.solve(F)
+                assert st.status == 0 and st.rel_residual <= 1.0001e-10, (g16, asm, st)
+                d = ((u - u64).abs().max() / u64.abs().max()).item()
+                assert d < 1e-8, (g16, asm, d)
+                its.append(st.iterations)
+            assert abs(its[1] - its[0]) <= max(3, 0.12 * its[0]), (g16, asm, its)
+
+
 @pytest.mark.parametrize("asm", [igamf.INVLM_GATHER, igamf.IEN_ATOMIC])
 def test_f16_geometry_inner_solve(asm):
     """FP32 inner PCG on the FP16-geometry operator under FP64 reliable
