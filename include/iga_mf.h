@@ -352,7 +352,9 @@ typedef struct {
   double rtol, abstol, divtol;  /* on ||f - A u||_2 (FP64) */
   int maxit;                    /* total inner iterations cap */
   double inner_rtol;            /* F32_REFINE: residual reduction between
-                                   reliable (FP64) updates (default 0.5) */
+                                   reliable (FP64) updates (default 0.1:
+                                   fewer FP64 residuals than 0.5 at the same
+                                   inner iteration count) */
   int inner_maxit;
   int pc_inner_maxit;           /* FEM_CG: inner CG iteration cap */
   double pc_inner_rtol;         /* FEM_CG: inner CG tolerance */

@@ -1540,7 +1540,7 @@ void iga_solver_opts_default(iga_solver_opts *o) {
   o->abstol = 0.0;
   o->divtol = 1e4;
   o->maxit = 100000;
-  o->inner_rtol = 0.5;
+  o->inner_rtol = 0.1;
   o->inner_maxit = 100000;
   o->pc_inner_maxit = 20;
   o->pc_inner_rtol = 1e-2;
