@@ -101,10 +101,12 @@ enum { SP_SEAM = 1, SP_X = 2, SP_Y = 4, SP_DIR = 8 };
 // Loads of y at function i after a defer_seam gather apply: y[i] for a
 // function one tile touches, else the halo slot of the lowest touching tile
 // (returned) and the slots of the others in s (tile row, then tile column).
-// Branch-free: the addresses come from i alone and every lane issues four
-// loads (an absent partial reloads the first address, which the same cache
-// line serves), so all loads of a step are in flight together.  32-bit
-// function indices.
+// The addresses come from i alone, so all loads of a step are in flight
+// together; within a wave the loads are unconditional (an absent partial
+// reloads the first address, which the same cache line serves), and a wave
+// with no function on a horizontal seam skips the two loads only those
+// need (wave vote: most waves of a row-major sweep).  32-bit function
+// indices.
 template <typename V>
 __device__ __forceinline__ V seam_load(const V *__restrict__ y, const SeamGeom &g, int i,
                                        SeamPart<V> &s) {
@@ -127,8 +129,11 @@ __device__ __forceinline__ V seam_load(const V *__restrict__ y, const SeamGeom &
   const V *p3 = (shx && shy) ? h + base + dx + dy : p0;
   const V a0 = *p0;
   s.a1 = *p1;
-  s.a2 = *p2;
-  s.a3 = *p3;
+  s.a2 = s.a3 = V(0);
+  if (__any(shy)) {
+    s.a2 = *p2;
+    s.a3 = *p3;
+  }
   // Dirichlet: the ring from the coordinates, an explicit mask by a load
   // whose value is first needed in seam_value
   const bool ring_dir = gx == 0 || gy == 0 || gx == g.nfx - 1 || gy == g.nfy - 1;

@@ -1066,7 +1066,8 @@ struct iga_solver {
   // per-column scalars / flags / partial slots, graph of one chunk
   int bk = 0;
   double *bx64 = nullptr;
-  float *bd32 = nullptr, *br32 = nullptr, *bp32 = nullptr, *bAp32 = nullptr, *bz32 = nullptr;
+  double *bd64 = nullptr;  // FP32 recursions' iterates, held in FP64 (see k_cg_init)
+  float *br32 = nullptr, *bp32 = nullptr, *bAp32 = nullptr, *bz32 = nullptr;
   double *bsc = nullptr, *bpart = nullptr;
   int *bic = nullptr;
   std::vector<void *> ballocs;
@@ -1459,7 +1460,7 @@ int block_buffers(iga_solver *s, int k) {
   };
   int rc = 0;
   rc = rc ? rc : al((void **)&s->bx64, nk * 8);
-  rc = rc ? rc : al((void **)&s->bd32, nk * 4);
+  rc = rc ? rc : al((void **)&s->bd64, nk * 8);
   rc = rc ? rc : al((void **)&s->br32, nk * 4);
   rc = rc ? rc : al((void **)&s->bp32, nk * 4);
   rc = rc ? rc : al((void **)&s->bAp32, nk * 4);
@@ -1482,7 +1483,7 @@ int bcg_chunk(iga_solver *s, int k, hipStream_t st) {
     if (rc) return rc;
     hipLaunchKernelGGL((k_cg_pap<float>), grid, dim3(RB), 0, st, s->bp32, s->bAp32, n, s->bpart,
                        s->bsc, s->bic, (const int *)nullptr, n);
-    hipLaunchKernelGGL((k_cg_update<float, float, true, false>), grid, dim3(RB), 0, st, s->bd32,
+    hipLaunchKernelGGL((k_cg_update<float, float, true, false, double>), grid, dim3(RB), 0, st, s->bd64,
                        s->br32, s->bp32, s->bAp32, s->bz32, (float *)nullptr, dinv, 0, n, s->bpart,
                        s->bsc, s->bic, nb, (const int *)nullptr, n, (const double *)nullptr, 0,
                        igai::SeamGeom{});
@@ -1909,7 +1910,7 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
     rrf[j] = h[0];
     const double tol = std::fmax(o.rtol * std::sqrt(ff[j]), o.abstol);
     tol2[j] = tol * tol;
-    LAUNCH((k_cg_init<float, double, float>), nb, s->r, 1.0, s->bd32 + (size_t)j * n,
+    LAUNCH((k_cg_init<float, double, float, double>), nb, s->r, 1.0, s->bd64 + (size_t)j * n,
            s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, s->bp32 + (size_t)j * n, dinv32, 1,
            n, colp(j));
     const double rr = h[0];
@@ -1954,7 +1955,7 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
       // reliable update of column j with the FP64 operator
       ++outer[j];
       double *xj = s->bx64 + (size_t)j * n;
-      LAUNCH(k_ru_accum<float>, nb, xj, s->bd32 + (size_t)j * n, n);
+      LAUNCH(k_ru_accum<double>, nb, xj, s->bd64 + (size_t)j * n, n);
       double h[2];
       rc = outer_residual(s, F + (size_t)j * ld, xj, s->r, h, st);
       if (rc) return rc;
