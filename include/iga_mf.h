@@ -386,6 +386,10 @@ typedef struct {
 } iga_solve_stats;
 
 typedef struct iga_solver iga_solver;
+/* Defaults: PCG_F64, Jacobi, SF kernel, INVLM gather; rtol 1e-8, abstol 0,
+   divtol 1e4, maxit 100000; inner_rtol 0.1, inner_maxit 100000;
+   pc_inner_maxit 20, pc_inner_rtol 1e-2, pc_cheb_degree 8; use_graph 1,
+   inner_f16_geometry 0, fused 1.  Host code only (no device call). */
 void iga_solver_opts_default(iga_solver_opts *o);
 int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o,
                       iga_solver **out);

@@ -167,3 +167,17 @@ def test_partition_reader_rejects_bad_files(tmp_path):
     d = igamf.read_preprocess_info(info)
     assert (d["p"], d["q"], d["nelem_x"], d["nelem_y"], d["Ly"], d["base_name"]) == \
         (3, 3, 64, 32, 1.5, "part")
+
+
+def test_solver_opts_defaults():
+    """iga_solver_opts_default (host code only) fills the values the header
+    documents; reading them back through the ctypes mirror also checks that
+    the mirror's field layout matches the C struct."""
+    o = igamf.SolverOpts()
+    igamf.lib().iga_solver_opts_default(C.byref(o))
+    assert (o.kind, o.pc, o.kernel, o.assembly) == (
+        igamf.SOLVE_PCG_F64, igamf.PC_JACOBI, igamf.SF, igamf.INVLM_GATHER)
+    assert (o.rtol, o.abstol, o.divtol, o.maxit) == (1e-8, 0.0, 1e4, 100000)
+    assert (o.inner_rtol, o.inner_maxit) == (0.1, 100000)
+    assert (o.pc_inner_maxit, o.pc_inner_rtol, o.pc_cheb_degree) == (20, 1e-2, 8)
+    assert (o.use_graph, o.inner_f16_geometry, o.fused) == (1, 0, 1)
