@@ -256,7 +256,12 @@ int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream);
  * per-point kappa field on the device. */
 int iga_op_qp_coords(iga_op *op, double *xq, void *stream);
 /* ---- per-qp conductivity and its gradient (extension; kappa = 1 is the
- * reference).  kappa: [E][nqp] FP64 device; G_q is rebuilt on the device. */
+ * reference).  kappa: [E][nqp] FP64 device; G_q is rebuilt on the device in
+ * every precision the op keeps.  The FP16 copy (f16_geometry) gets a new
+ * power-of-two scale, which is read back to the host: the call then
+ * synchronises the stream.  IGA_ERR_UNSUPPORTED for an op without FP64
+ * geometry (want_f64 = 0); IGA_ERR_NOT_FINITE when the new factors are not
+ * finite. */
 int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream);
 /* For y = A(kappa) x and an output cotangent g: grad_x = A^T g
  * (= K~ P_int g; equals A g on interior rows when g vanishes on Dirichlet

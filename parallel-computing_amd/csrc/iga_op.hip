@@ -277,6 +277,32 @@ __global__ void k_absmax_bits(const float *v, int64_t n, unsigned *out) {
   if (m != m) m = __builtin_inff();
   atomicMax(out, __float_as_uint(m));
 }
+
+// The FP16 copy of G32 (IGA_F32_G16): G32 / 2^e, e chosen so that the
+// largest magnitude is at most 2^14 (range margin below 65504; values 2^28
+// times smaller stay normal).  op->G16 and op->g16max must exist.  Reads the
+// maximum back to the host, so it synchronises the stream.
+int build_g16(iga_op *op, hipStream_t s) {
+  const igak::Layout &L = op->L32;
+  const size_t n = (size_t)L.ntx * L.nty * op->nc * op->nqp * L.tx * L.ty;
+  IGA_HIP_TRY(hipMemsetAsync(op->g16max, 0, sizeof(unsigned), s));
+  hipLaunchKernelGGL(k_absmax_bits, dim3(1024), dim3(256), 0, s, op->G32, (int64_t)n,
+                     op->g16max);
+  unsigned hb = 0;
+  IGA_HIP_TRY(hipMemcpyAsync(&hb, op->g16max, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  IGA_HIP_TRY(hipStreamSynchronize(s));
+  float gmax;
+  std::memcpy(&gmax, &hb, sizeof(float));
+  if (!std::isfinite(gmax)) return IGA_ERR_NOT_FINITE;
+  int e = 0;
+  if (gmax > 0.f) {
+    (void)std::frexp(gmax, &e);  // gmax < 2^e
+    e -= 14;
+  }
+  op->g16_scale = std::ldexp(1.0f, e);
+  IGA_HIP_TRY(op->ops->g16conv(op->G32, op->G16, L, std::ldexp(1.0f, -e), s));
+  return IGA_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -561,30 +587,12 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
       return fail(IGA_ERR_HIP);
     ALLOC(op->halo32, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 4);
     if (o.f16_geometry) {
-      // FP16 copy for IGA_F32_G16: G32 / 2^e with the largest magnitude at
-      // most 2^14 (range margin below 65504; values 2^28 times smaller stay
-      // normal)
       if (rational == 2 || !igak::g16_compiled(op->tile32) || !ops->apply16)
         return fail(IGA_ERR_UNSUPPORTED);
-      unsigned *dmax = nullptr;
-      ALLOC(dmax, sizeof(unsigned));
-      if (hipMemset(dmax, 0, sizeof(unsigned)) != hipSuccess) return fail(IGA_ERR_HIP);
-      hipLaunchKernelGGL(k_absmax_bits, dim3(1024), dim3(256), 0, s, op->G32, (int64_t)n, dmax);
-      unsigned hb = 0;
-      if (hipMemcpy(&hb, dmax, sizeof(unsigned), hipMemcpyDeviceToHost) != hipSuccess)
-        return fail(IGA_ERR_HIP);
-      float gmax;
-      std::memcpy(&gmax, &hb, sizeof(float));
-      if (!std::isfinite(gmax)) return fail(IGA_ERR_NOT_FINITE);
-      int e = 0;
-      if (gmax > 0.f) {
-        (void)std::frexp(gmax, &e);  // gmax < 2^e
-        e -= 14;
-      }
-      op->g16_scale = std::ldexp(1.0f, e);
+      ALLOC(op->g16max, sizeof(unsigned));
       ALLOC(op->G16, n * 2);
-      if (ops->g16conv(op->G32, op->G16, L, std::ldexp(1.0f, -e), s) != hipSuccess)
-        return fail(IGA_ERR_HIP);
+      const int rc16 = build_g16(op, s);
+      if (rc16) return fail(rc16);
     }
   }
   ALLOC(op->tdot, (size_t)std::max(op->L64.ntx * op->L64.nty, op->L32.ntx * op->L32.nty) * 8);
@@ -918,6 +926,7 @@ int iga_op_qp_coords(iga_op *op, double *xq, void *stream) {
 
 int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream) {
   if (!op || !kappa) return IGA_ERR_INVALID_ARGUMENT;
+  if (!op->G64) return IGA_ERR_UNSUPPORTED;  // kappa is applied to the FP64 factors
   int rc = igai::set_device(op->device);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -941,6 +950,8 @@ int iga_op_set_kappa(iga_op *op, const double *kappa, void *stream) {
     IGA_HIP_TRY(op->ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
                                 op->kappa, op->nex, op->ney, op->nfx, op->L32, s));
   ++op->gen;
+  // the FP16 copy follows G32, with its power-of-two scale chosen again
+  if (op->G16) return build_g16(op, s);
   return IGA_OK;
 }
 

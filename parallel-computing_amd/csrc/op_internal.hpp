@@ -42,6 +42,7 @@ struct iga_op {
   float *G32 = nullptr;
   _Float16 *G16 = nullptr;          // FP16 copy of G32 / g16_scale (f16_geometry)
   float g16_scale = 1.0f;
+  unsigned *g16max = nullptr;       // device word: bits of max |G32| (FP16 scale)
   double *kappa = nullptr;          // device [E][nqp] or nullptr
   double *halo64 = nullptr;
   float *halo32 = nullptr;

@@ -54,8 +54,9 @@ constexpr int CHUNK = 16;  // iterations per enqueued chunk / graph
 
 // scalar slots (doubles) of one CG instance
 enum { S_RZ = 0, S_RZ_OLD, S_RR, S_RR0, S_TOL2, S_DIV2, S_ALPHA, S_BETA, S_UPD2, S_NS };
-// integer slots
-enum { I_DONE = 0, I_ITER, I_NEXT, I_MAXIT, I_NI };
+// integer slots; I_TOK: one per workgroup of k_cg_pupdate, the iteration it
+// last processed (see there)
+enum { I_DONE = 0, I_ITER, I_NEXT, I_MAXIT, I_TOK, I_NI = I_TOK + RG };
 // I_DONE values: 0 running, 1 converged, 2 iteration limit, 3 diverged /
 // breakdown, 4 residual reduced by the reliable-update factor (FP32 solve)
 // partial-sum slots (each RG doubles)
@@ -184,6 +185,7 @@ __global__ void k_cg_init_fin(const double *part, int nb, double *sc, int *ic, d
     ic[I_MAXIT] = maxit;
     ic[I_DONE] = !finite(rr) ? 3 : (rr <= sc[S_TOL2] ? 1 : 0);
   }
+  for (int i = threadIdx.x; i < RG; i += blockDim.x) ic[I_TOK + i] = 0;
 }
 
 // K1: partials of p.Ap
@@ -365,6 +367,12 @@ __global__ void k_cg_rz(const V *r, const V *z, const V *rold, int64_t n, double
 // With Apz, also zero Ap for the next scatter-add apply (IEN atomic form),
 // which then needs no seam-zeroing kernel; Ap is dead once this kernel runs.
 // When the iteration stops, Ap is left as it is: cg_run zeroes it on entry.
+// Workgroup 0 writes the done flag, and on a reliable-update stop (st = 4)
+// the other workgroups must still form p: so this kernel never reads the
+// done flag.  It runs for iteration I_NEXT when K2 ran (K2 advances I_NEXT
+// and returns at once on a set flag), which each workgroup tells from its
+// own token, the iteration it last processed (no other workgroup reads or
+// writes it).
 template <typename V>
 __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t n,
                              const double *part, double *sc, int *ic, int nb, int flexible,
@@ -373,7 +381,9 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
   p += voff_;
   z += voff_;
   if (Apz) Apz += voff_;
-  if (ic[I_DONE] || (outer_done && *outer_done)) return;
+  if (outer_done && *outer_done) return;
+  const int it = ic[I_NEXT];
+  if (ic[I_TOK + blockIdx.x] == it) return;
   constexpr int U = 4;
   const int64_t step = (int64_t)gridDim.x * RB * U;
   auto load = [&](int64_t bq, V (&zv)[U], V (&pv)[U]) {
@@ -409,9 +419,9 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
     f[1] = a1;
     f[2] = a2;
   }
-  block_sum_n<3>(f);
+  block_sum_n<3>(f);  // its barriers order every thread's token read before the write
+  if (threadIdx.x == 0) ic[I_TOK + blockIdx.x] = it;
   const double rr = f[0], rz = f[1], ro = f[2];
-  const int it = ic[I_NEXT];
   int st = 0;
   if (rr <= sc[S_TOL2]) st = 1;
   else if (!finite(rr) || rr > sc[S_DIV2]) st = 3;
@@ -941,6 +951,7 @@ __global__ void k_pc_init_fin(const double *part, int nb, double *sc, int *ic, d
     ic[I_MAXIT] = maxit;
     ic[I_DONE] = (ww <= sc[S_TOL2]) ? 1 : 0;
   }
+  for (int i = threadIdx.x; i < RG; i += blockDim.x) ic[I_TOK + i] = 0;
 }
 
 inline int nblocks(int64_t n) {
@@ -1613,7 +1624,7 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   SA(s->dinv, n * 8);
   SA(s->x64, n * 8);
   SA(s->sc, 64 * 8);
-  SA(s->ic, 32 * 4);
+  SA(s->ic, 2 * I_NI * 4);
   SA(s->part, (size_t)P_NP * RG * 8);
   SA(s->part_in, (size_t)P_NP * RG * 8);
   if (o.kind == IGA_SOLVE_PCG_F32_REFINE) {
@@ -1639,7 +1650,7 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     SA(s->tdot, (size_t)std::max(s->ntiles64, s->ntiles32) * 8);
     if (o.fused == 2 && !fem_pc_kind) SA(s->cgs, (size_t)2 * CG_L * 8);
   }
-  if (hipMemset(s->ic, 0, 32 * 4) != hipSuccess || hipMemset(s->sc, 0, 64 * 8) != hipSuccess)
+  if (hipMemset(s->ic, 0, 2 * I_NI * 4) != hipSuccess || hipMemset(s->sc, 0, 64 * 8) != hipSuccess)
     return fail(IGA_ERR_HIP);
   if (fem_pc_kind) {
     iga_csr_view v;

@@ -194,6 +194,24 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
+def check_vec(t, dtype, n, device, name):
+    """Raise ValueError unless t is a contiguous tensor of n values of dtype
+    on the CUDA device `device` (the C ABI takes bare device pointers and
+    trusts their size and type)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_cuda or t.device != device:
+        raise ValueError(f"{name} must live on {device}, got {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() != n:
+        raise ValueError(f"{name} must have {n} values, got {t.numel()}")
+    return t
+
+
 # --------------------------------------------------------------------------- mesh
 class Mesh:
     """Host mesh of one patch, built by the library (iga_mesh_build) or wrapped
@@ -326,19 +344,30 @@ class Operator:
     @staticmethod
     def _dtype(x, g16):
         import torch
-        dt = {torch.float64: F64, torch.float32: F32}[x.dtype]
+        dt = {torch.float64: F64, torch.float32: F32}.get(getattr(x, "dtype", None))
+        if dt is None:
+            raise ValueError("vectors must be float64 or float32 tensors")
         if g16:
-            assert dt == F32, "FP16 geometry applies take float32 vectors"
+            if dt != F32:
+                raise ValueError("FP16 geometry applies take float32 vectors")
             dt = F32_G16
         return dt
 
-    def apply(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None, g16=False):
-        """y = A x (iga_op_apply); g16: FP32 arithmetic on the FP16 G_q."""
+    def _xy(self, x, y, g16):
+        """Checked (dtype code, x, y) of an apply; y allocated when None."""
         import torch
         dt = self._dtype(x, g16)
-        assert x.is_cuda and x.is_contiguous() and x.numel() == self.nfunc
+        check_vec(x, x.dtype, self.nfunc, self.device, "x")
         if y is None:
             y = torch.empty_like(x)
+        check_vec(y, x.dtype, self.nfunc, self.device, "y")
+        if y.data_ptr() == x.data_ptr():
+            raise ValueError("x and y must not alias")
+        return dt, x, y
+
+    def apply(self, x, y=None, kernel=SF, assembly=INVLM_GATHER, stream=None, g16=False):
+        """y = A x (iga_op_apply); g16: FP32 arithmetic on the FP16 G_q."""
+        dt, x, y = self._xy(x, y, g16)
         check(_lib().iga_op_apply(self._h, dt, kernel, assembly, x.data_ptr(), y.data_ptr(),
                                   _stream_ptr(stream)), "op_apply")
         return y
@@ -349,10 +378,7 @@ class Operator:
         """y = A x and x.y (float64 device scalar) from one kernel
         (iga_op_apply_dot)."""
         import torch
-        dt = self._dtype(x, g16)
-        assert x.is_cuda and x.is_contiguous() and x.numel() == self.nfunc
-        if y is None:
-            y = torch.empty_like(x)
+        dt, x, y = self._xy(x, y, g16)
         xy = torch.empty((), dtype=torch.float64, device=x.device)
         check(_lib().iga_op_apply_dot(self._h, dt, kernel, assembly, x.data_ptr(), y.data_ptr(),
                                       xy.data_ptr(), _stream_ptr(stream)), "op_apply_dot")
@@ -379,7 +405,7 @@ class Operator:
                     g16=False):
         """Average ms of the element kernel alone over nlaunch back-to-back
         launches (y receives partial results)."""
-        dt = self._dtype(x, g16)
+        dt, x, y = self._xy(x, y, g16)
         ms = C.c_double()
         check(_lib().iga_op_time_kernel(self._h, dt, kernel, assembly, x.data_ptr(), y.data_ptr(),
                                         nlaunch, _stream_ptr(stream), C.byref(ms)), "time_kernel")
@@ -393,12 +419,18 @@ class Operator:
         return xq
 
     def set_kappa(self, kappa, stream=None):
-        assert kappa.dtype.is_floating_point and kappa.numel() == self.nelem * self.nqp
-        k = kappa.contiguous().double()
+        """kappa: float64 CUDA tensor of nelem * nqp values on the op's device
+        (element-major, quadrature points in the reference's order)."""
+        import torch
+        k = check_vec(kappa.reshape(-1) if hasattr(kappa, "reshape") else kappa,
+                      torch.float64, self.nelem * self.nqp, self.device, "kappa")
         check(_lib().iga_op_set_kappa(self._h, k.data_ptr(), _stream_ptr(stream)), "set_kappa")
 
     def backward(self, x, g, want_x=True, want_kappa=True, stream=None):
         import torch
+        check_vec(g, torch.float64, self.nfunc, self.device, "g")
+        if want_kappa or x is not None:
+            check_vec(x, torch.float64, self.nfunc, self.device, "x")
         gx = torch.empty_like(g) if want_x else None
         gk = (torch.empty(self.nelem * self.nqp, dtype=torch.float64, device=self.device)
               if want_kappa else None)
@@ -408,10 +440,11 @@ class Operator:
 
     def _block_args(self, X, Y):
         import torch
-        if X.dtype != torch.float32 or not X.is_cuda or X.dim() != 2:
-            raise ValueError("X must be a 2-D float32 CUDA tensor [k, nfunc]")
+        if (X.dtype != torch.float32 or not X.is_cuda or X.dim() != 2
+                or X.device != self.device):
+            raise ValueError("X must be a 2-D float32 tensor [k, nfunc] on the op's device")
         k, n = X.shape
-        if n != self.nfunc or X.stride(1) != 1:
+        if n != self.nfunc or X.stride(1) != 1 or (k > 1 and X.stride(0) < n):
             raise ValueError("X rows must be contiguous vectors of length nfunc")
         if Y is None:
             Y = torch.empty_like(X)
@@ -544,8 +577,12 @@ class Solver:
         raises IgaError unless check_status is False (stats.status then
         carries the code)."""
         import torch
+        check_vec(f, torch.float64, self.op.nfunc, self.op.device, "f")
         if u is None:
             u = torch.zeros_like(f)
+        check_vec(u, torch.float64, self.op.nfunc, self.op.device, "u")
+        if u.data_ptr() == f.data_ptr():
+            raise ValueError("f and u must not alias")
         st = SolveStats()
         rc = _lib().iga_solver_solve(self._h, f.data_ptr(), u.data_ptr(), _stream_ptr(stream),
                                      C.byref(st))
@@ -558,9 +595,13 @@ class Solver:
         contiguous; a row stride above nfunc is the leading dimension).
         Returns (U, [stats per row])."""
         import torch
-        if F.dtype != torch.float64 or F.dim() != 2 or F.stride(1) != 1:
-            raise ValueError("F must be a 2-D float64 CUDA tensor with contiguous rows")
+        if (F.dtype != torch.float64 or F.dim() != 2 or F.stride(1) != 1 or not F.is_cuda
+                or F.device != self.op.device):
+            raise ValueError("F must be a 2-D float64 tensor on the op's device with "
+                             "contiguous rows")
         k, n = F.shape
+        if n != self.op.nfunc or k < 1 or (k > 1 and F.stride(0) < n):
+            raise ValueError("F rows must be vectors of length nfunc")
         if U is None:  # same leading dimension as F
             U = torch.zeros((k, F.stride(0)), dtype=F.dtype, device=F.device)[:, :n]
         if U.shape != F.shape or U.stride() != F.stride():

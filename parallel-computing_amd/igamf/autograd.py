@@ -30,6 +30,8 @@ class _ApplyKappa(torch.autograd.Function):
         want_x, want_k = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         gx, gk = op.backward(x.detach().contiguous(), gy.contiguous(), want_x=want_x,
                              want_kappa=want_k)
+        if gk is not None:
+            gk = gk.view_as(kappa)
         return gx, gk, None
 
 
@@ -60,7 +62,7 @@ class _SolveKappa(torch.autograd.Function):
         gk = None
         if ctx.needs_input_grad[1]:
             _, gk = op.backward(u, lam, want_x=False, want_kappa=True)
-            gk = -gk
+            gk = -gk.view_as(kappa)
         return lam, gk, None
 
 

@@ -161,3 +161,23 @@ def test_qp_coords_match_oracle(geom):
         for ex in range(om.nex):
             ref = O.element_qp(om, ex, ey)[4]
             assert np.abs(xq[ey * om.nex + ex] - ref).max() < 1e-14
+
+
+def test_set_kappa_refreshes_f16_geometry():
+    """set_kappa rebuilds the FP16 copy of G_q (and its power-of-two scale):
+    the FP16-geometry apply follows the new kappa, also one 1000x larger
+    than the first (beyond the first scale's FP16 range)."""
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 20)
+    op = igamf.Operator(m, f16_geometry=True)
+    x = torch.from_numpy(igamf.rng_normal(4, op.nfunc)).cuda()
+    k1 = torch.from_numpy(kappa_field(m, 1)).cuda()
+    s0 = op.info.f16_scale
+    for kap in (k1, 1000.0 * k1):
+        op.set_kappa(kap)
+        y64 = op.apply(x).cpu().numpy()
+        y16 = op.apply(x.float(), g16=True).double().cpu().numpy()
+        assert rel(y16, y64) < 3e-3, rel(y16, y64)
+    ref = igamf.Operator(m, kappa=(1000.0 * k1).cpu().numpy(), f16_geometry=True)
+    y_ref = ref.apply(x.float(), g16=True)
+    assert torch.equal(op.apply(x.float(), g16=True), y_ref)
+    assert s0 > 0

@@ -181,3 +181,35 @@ def test_solver_opts_defaults():
     assert (o.inner_rtol, o.inner_maxit) == (0.1, 100000)
     assert (o.pc_inner_maxit, o.pc_inner_rtol, o.pc_cheb_degree) == (20, 1e-2, 8)
     assert (o.use_graph, o.inner_f16_geometry, o.fused) == (1, 0, 1)
+
+
+def test_binding_rejects_bad_vectors_before_any_device_call():
+    """The C ABI trusts bare device pointers: the binding checks dtype,
+    device, contiguity and size first (no library call is reached here)."""
+    import torch
+    op = igamf.Operator.__new__(igamf.Operator)
+    op._h, op.nfunc, op.nelem, op.nqp = None, 10, 2, 4
+    op.device = torch.device("cuda", 0)
+    cpu64 = torch.zeros(10, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        op.apply(cpu64)
+    with pytest.raises(ValueError):
+        op.apply(torch.zeros(10, dtype=torch.int32))
+    with pytest.raises(ValueError):
+        op.apply(cpu64, g16=True)
+    with pytest.raises(ValueError):
+        op.set_kappa(torch.ones(8, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        op.backward(cpu64, cpu64)
+    with pytest.raises(ValueError):
+        op.apply_block(torch.zeros((2, 10), dtype=torch.float32))
+    s = igamf.Solver.__new__(igamf.Solver)
+    s._h, s.op = None, op
+    with pytest.raises(ValueError):
+        s.solve(torch.zeros(10, dtype=torch.float32))
+    with pytest.raises(ValueError):
+        s.solve(cpu64)
+    with pytest.raises(ValueError):
+        s.solve_block(torch.zeros((2, 10), dtype=torch.float64))
+    with pytest.raises(ValueError):
+        igamf.check_vec(cpu64[::2], torch.float64, 5, op.device, "x")

@@ -371,3 +371,23 @@ def test_fem_chebyshev_preconditioned_solve():
     with pytest.raises(igamf.IgaError) as e:
         igamf.Solver(op, fem, pc=igamf.PC_FEM_CHEB, pc_cheb_degree=0)
     assert e.value.status == 1
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_refine_solve_repeatable_on_full_vector_grid(fused):
+    """A mesh whose vector kernels run the full grid of 512 workgroups: the
+    FP32 + FP64 reliable-update solve (which stops its recursion with the
+    p-update kernel's done flag) is bitwise repeatable and matches FP64."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 384)
+    op = igamf.Operator(m)
+    assert op.nfunc >= 512 * 256
+    F = op.load(igamf.LOAD_CONST, 1.0)
+    u64, _ = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F64, rtol=1e-9).solve(F)
+    s = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F32_REFINE, rtol=1e-9, fused=fused)
+    runs = [s.solve(F) for _ in range(3)]
+    u0, st0 = runs[0]
+    assert st0.status == 0 and st0.outer_iterations > 2, st0.outer_iterations
+    for u, st in runs[1:]:
+        assert torch.equal(u, u0) and st.iterations == st0.iterations
+    d = ((u0 - u64).abs().max() / u64.abs().max()).item()
+    assert d < 1e-7, d
