@@ -416,6 +416,16 @@ int iga_solver_destroy(iga_solver *s);
 /* ---- device vector helpers (FP64 / FP32, used by drivers and tests) ----- */
 int iga_dot_f64(const double *a, const double *b, int64_t n, double *out_host,
                 void *stream);
+/* Allocation-free form of iga_dot_f64.  No counterpart in the reference; it
+ * is for callers that cannot allocate or synchronise (code under HIP graph
+ * capture, tight serving loops).  The partial sums (one per 1,024 values)
+ * live in a static device array of the library, 65,536 doubles (512 KiB) per
+ * GPU held until the process ends, so n <= 67,108,864 (IGA_ERR_SIZE_LIMIT
+ * above).  a . b goes to *out (one double, device memory) on `stream`,
+ * asynchronously: nothing is allocated and nothing synchronises.  Calls that
+ * overlap on one GPU (other streams or threads) share the array and race. */
+int iga_dot_f64_noalloc(const double *a, const double *b, int64_t n, double *out,
+                        void *stream);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,8 @@ GEOM_UNIT_SQUARE, GEOM_QUARTER_ANNULUS, GEOM_PERTURBED_SQUARE = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_FEM_CG, PC_FEM_CHEB, PC_FEM_ILU = 0, 1, 2, 3, 4
 SOLVE_PCG_F64, SOLVE_PCG_F32_REFINE = 0, 1
 LOAD_REF_POLY, LOAD_CONST, LOAD_SINSIN = 0, 1, 2
-IGA_ERR_DIVERGED, IGA_ERR_NOT_CONVERGED = 10, 11
+IGA_ERR_INVALID_ARGUMENT, IGA_ERR_UNSUPPORTED = 1, 6
+IGA_ERR_NOT_FINITE, IGA_ERR_DIVERGED, IGA_ERR_NOT_CONVERGED, IGA_ERR_SIZE_LIMIT = 9, 10, 11, 12
 
 
 class IgaError(RuntimeError):
@@ -158,6 +159,7 @@ def _lib():
                                              C.POINTER(SolveStats)]
         L.iga_solver_destroy.argtypes = [VP]
         L.iga_dot_f64.argtypes = [VP, VP, C.c_int64, C.POINTER(C.c_double), VP]
+        L.iga_dot_f64_noalloc.argtypes = [VP, VP, C.c_int64, VP, VP]
         _LIB = L
     return _LIB
 
