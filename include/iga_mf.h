@@ -249,6 +249,13 @@ int iga_op_profile_read(iga_op *op, int *count, double *total_ms, double *min_ms
  * 3-29).  kind: 0 f = x(1-x)y(1-y) (the reference's), 1 f = value,
  * 2 f = value*sin(pi x)sin(pi y). */
 int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream);
+/* The same load vector for a right-hand side given by its values at the
+ * quadrature points: fq [E][nqp] FP64 device, elements and points in the
+ * reference's order (those of iga_op_qp_coords).  The reference hard-codes
+ * f (matrix_free/include/LocalAssemblyMF.hpp:52-55, matrix_free_gpu/src/
+ * mult.cu:228-231); this form takes any f, e.g. one defined on the
+ * parametric coordinates. */
+int iga_op_load_qp(iga_op *op, const double *fq, double *F, void *stream);
 
 /* Physical coordinates of every quadrature point, xq[e][q][2] (FP64,
  * device, E * nqp * 2 values), elements and points in the reference's order

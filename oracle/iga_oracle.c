@@ -731,7 +731,22 @@ int orc_apply_sf_ref(const orc_mesh *m, const double *x, double *y) {
  * matrix_free/src/LocalAssemblyMF.cpp:3-29 and GlobalAssemblyMF.cpp:26-80.
  * kind 0: the reference's f = x(1-x)y(1-y) (LocalAssemblyMF.hpp:52-55);
  * kind 1: f = value; kind 2: f = value*sin(pi x)sin(pi y). */
+static void load_impl(const orc_mesh *m, int kind, double value, const double *fq,
+                      double *F);
+
 void orc_load(const orc_mesh *m, int kind, double value, double *F) {
+  load_impl(m, kind, value, NULL, F);
+}
+
+/* The same with f given at the quadrature points, fq[e][qp] (element and
+ * point order of orc_element_qp): AssemLocalLoad's sum with f(x_q) replaced
+ * by the given value (extension: the reference hard-codes f). */
+void orc_load_qp(const orc_mesh *m, const double *fq, double *F) {
+  load_impl(m, 0, 0.0, fq, F);
+}
+
+static void load_impl(const orc_mesh *m, int kind, double value, const double *fq,
+                      double *F) {
   const int nloc = (m->p + 1) * (m->q + 1);
   const int nqp = m->nq1 * m->nq2;
   const double pi = 3.14159265358979323846264338327950288;
@@ -752,7 +767,8 @@ void orc_load(const orc_mesh *m, int kind, double value, double *F) {
       for (int qp = 0; qp < nqp; ++qp) {
         const double X = xq[2 * qp], Y = xq[2 * qp + 1];
         double f;
-        if (kind == 0) f = X * (1 - X) * Y * (1 - Y);
+        if (fq) f = fq[((int64_t)ey * m->nex + ex) * nqp + qp];
+        else if (kind == 0) f = X * (1 - X) * Y * (1 - Y);
         else if (kind == 1) f = value;
         else f = value * sin(pi * X) * sin(pi * Y);
         for (int l = 0; l < nloc; ++l) Fe[l] += R[qp * nloc + l] * f * JxW[qp];
@@ -966,8 +982,11 @@ void orc_fem_elem2coo(int nfx, int nfy, const int *rows, const int *cols,
   free(off);
 }
 
-/* dir2coo[i] = position of the first COO entry in row Dir_i:
- * matrix_free_gpu/src/Elem2COOGenerator.cpp:57-69 (std::find over rows). */
+/* dir2coo[i] = position of the first COO entry in row Dir_i, nnz when the
+ * row is empty: matrix_free_gpu/src/Elem2COOGenerator.cpp:57-69, which finds
+ * it with std::find over rows.  rows is sorted, so the first entry of row d
+ * is the lower bound of d (a binary search: the linear scan is O(ndir nnz),
+ * minutes at 2048^2). */
 int orc_fem_dir2coo(int nfx, int nfy, const int *rows, int64_t nnz,
                     int *dir2coo) {
   int nd = 0;
@@ -975,9 +994,12 @@ int orc_fem_dir2coo(int nfx, int nfy, const int *rows, int64_t nnz,
     for (int i = 0; i < nfx; ++i)
       if (i == 0 || j == 0 || i == nfx - 1 || j == nfy - 1) {
         const int d = j * nfx + i;
-        int64_t k = 0;
-        while (k < nnz && rows[k] != d) ++k;
-        dir2coo[nd++] = (int)k;
+        int64_t lo = 0, hi = nnz;  /* first k with rows[k] >= d */
+        while (lo < hi) {
+          const int64_t mid = lo + (hi - lo) / 2;
+          if (rows[mid] < d) lo = mid + 1; else hi = mid;
+        }
+        dir2coo[nd++] = (int)((lo < nnz && rows[lo] == d) ? lo : nnz);
       }
   return nd;
 }

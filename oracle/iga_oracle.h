@@ -72,6 +72,7 @@ int orc_apply_mf_colored(const orc_mesh *m, const double *x, double *y,
                          int nthreads);
 int orc_apply_sf_ref(const orc_mesh *m, const double *x, double *y);
 void orc_load(const orc_mesh *m, int kind, double value, double *F);
+void orc_load_qp(const orc_mesh *m, const double *fq, double *F);
 void orc_kappa_grad(const orc_mesh *m, const double *u, const double *g,
                     double *grad_kappa);
 int64_t orc_element_matrices(const orc_mesh *m, int64_t cap, double *Ke);

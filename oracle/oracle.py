@@ -52,6 +52,7 @@ def lib():
         L.orc_apply_mf_colored.argtypes = [C.POINTER(_Mesh), _F64P, _F64P, C.c_int]
         L.orc_apply_sf_ref.argtypes = [C.POINTER(_Mesh), _F64P, _F64P]
         L.orc_load.argtypes = [C.POINTER(_Mesh), C.c_int, C.c_double, _F64P]
+        L.orc_load_qp.argtypes = [C.POINTER(_Mesh), _F64P, _F64P]
         L.orc_kappa_grad.argtypes = [C.POINTER(_Mesh), _F64P, _F64P, _F64P]
         L.orc_element_matrices.argtypes = [C.POINTER(_Mesh), C.c_int64, _F64P]
         L.orc_element_matrices.restype = C.c_int64
@@ -256,6 +257,15 @@ def load(m, kind=LOAD_REF_POLY, value=0.0):
     F = np.empty(m.nfunc)
     s = m.struct()
     lib().orc_load(C.byref(s), kind, value, F)
+    return F
+
+
+def load_qp(m, fq):
+    """Load vector of f given at the quadrature points ([E][nqp], the
+    order of element_qp)."""
+    F = np.empty(m.nfunc)
+    s = m.struct()
+    lib().orc_load_qp(C.byref(s), np.ascontiguousarray(fq, np.float64).reshape(-1), F)
     return F
 
 

@@ -900,8 +900,9 @@ int iga_op_profile_read(iga_op *op, int *count, double *total_ms, double *min_ms
   return IGA_OK;
 }
 
-int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream) {
-  if (!op || !F || kind < 0 || kind > 2) return IGA_ERR_INVALID_ARGUMENT;
+namespace {
+int load_common(iga_op *op, int kind, double value, const double *fq, double *F,
+                void *stream) {
   int rc = igai::set_device(op->device);
   if (rc) return rc;
   if (!op->fJ) {
@@ -909,10 +910,20 @@ int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream) {
     if (rc) return rc;
   }
   IGA_HIP_TRY(op->ops->load(F, op->fJ, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
-                            op->dirmask,
-                            op->nex, op->ney, op->nfx, op->nfy, kind, value,
+                            op->dirmask, op->nex, op->ney, op->nfx, op->nfy, kind, value, fq,
                             (hipStream_t)stream));
   return IGA_OK;
+}
+}  // namespace
+
+int iga_op_load(iga_op *op, int kind, double value, double *F, void *stream) {
+  if (!op || !F || kind < 0 || kind > 2) return IGA_ERR_INVALID_ARGUMENT;
+  return load_common(op, kind, value, nullptr, F, stream);
+}
+
+int iga_op_load_qp(iga_op *op, const double *fq, double *F, void *stream) {
+  if (!op || !fq || !F) return IGA_ERR_INVALID_ARGUMENT;
+  return load_common(op, 0, 0.0, fq, F, stream);
 }
 
 int iga_op_qp_coords(iga_op *op, double *xq, void *stream) {

@@ -1257,9 +1257,11 @@ __global__ void k_geometry(T *G, const double *tabx, const double *taby,
 // is the per-function form of the reference's load kernel).
 // --------------------------------------------------------------------------
 template <int P, int NQ>
+// fq: values of f at the quadrature points ([E][NQ*NQ], reference order),
+// used instead of kind / value when not null
 __global__ void k_load_qp(double *fJ, const double *tabx, const double *taby,
                           const double *cp, const double *wts, const double *wq, int nex,
-                          int ney, int nfx, int kind, double value) {
+                          int ney, int nfx, int kind, double value, const double *fq) {
   constexpr int NB = P + 1, NQ2 = NQ * NQ, TAB = NQ * NB;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nex * ney) return;
@@ -1273,7 +1275,8 @@ __global__ void k_load_qp(double *fJ, const double *tabx, const double *taby,
       map_q<P, NQ>(cp, wts, nfx, ex, ey, t1, t2, i, j, m);
       const double x = m[0], y = m[1];
       double f;
-      if (kind == 0) f = x * (1.0 - x) * y * (1.0 - y);
+      if (fq) f = fq[(size_t)e * NQ2 + j * NQ + i];
+      else if (kind == 0) f = x * (1.0 - x) * y * (1.0 - y);
       else if (kind == 1) f = value;
       else f = value * sin(pi * x) * sin(pi * y);
       // general rational: R_A = w_A B_A / W, the 1/W here, w_A in the gather

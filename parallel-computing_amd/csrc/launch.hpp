@@ -183,11 +183,11 @@ struct Launch {
   static hipError_t load(double *F, double *fJ, const double *tabx, const double *taby,
                          const double *cp, const double *wts, const double *wq,
                          const uint8_t *mask, int nex, int ney, int nfx, int nfy, int kind,
-                         double value, hipStream_t s) {
+                         double value, const double *fq, hipStream_t s) {
     const int bs = 256;
     const int ne = nex * ney, nf = nfx * nfy;
     hipLaunchKernelGGL((k_load_qp<P, NQ>), dim3((ne + bs - 1) / bs), dim3(bs), 0, s, fJ, tabx,
-                       taby, cp, wts, wq, nex, ney, nfx, kind, value);
+                       taby, cp, wts, wq, nex, ney, nfx, kind, value, fq);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_load_gather<P, NQ>), dim3((nf + bs - 1) / bs), dim3(bs), 0, s, F, fJ,
@@ -263,7 +263,7 @@ struct Ops {
                        const Layout &, hipStream_t);
   hipError_t (*load)(double *, double *, const double *, const double *, const double *,
                      const double *, const double *, const uint8_t *, int, int, int, int, int,
-                     double, hipStream_t);
+                     double, const double *, hipStream_t);
   hipError_t (*kappa_grad)(double *, const double *, const double *, const double *,
                            const double *, const double *, const double *, const uint8_t *, int,
                            int, int, int, const Layout &, hipStream_t);

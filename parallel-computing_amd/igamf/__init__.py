@@ -132,6 +132,7 @@ def _lib():
         L.iga_op_apply.argtypes = [VP, C.c_int, C.c_int, C.c_int, VP, VP, VP]
         L.iga_op_apply_dot.argtypes = [VP, C.c_int, C.c_int, C.c_int, VP, VP, VP, VP]
         L.iga_op_load.argtypes = [VP, C.c_int, C.c_double, VP, VP]
+        L.iga_op_load_qp.argtypes = [VP, VP, VP, VP]
         L.iga_op_profile.argtypes = [VP, C.c_int, C.c_int]
         L.iga_op_profile_read.argtypes = [VP, C.POINTER(C.c_int)] + [C.POINTER(C.c_double)] * 3
         L.iga_op_time_kernel.argtypes = [VP, C.c_int, C.c_int, C.c_int, VP, VP, C.c_int, VP,
@@ -401,6 +402,16 @@ class Operator:
         F = torch.empty(self.nfunc, dtype=torch.float64, device=self.device)
         check(_lib().iga_op_load(self._h, kind, value, F.data_ptr(), _stream_ptr(stream)),
               "op_load")
+        return F
+
+    def load_qp(self, fq, stream=None):
+        """Load vector of f given at the quadrature points (float64 CUDA,
+        nelem * nqp values, the order of qp_coords)."""
+        import torch
+        fq = check_vec(fq.reshape(-1), torch.float64, self.nelem * self.nqp, self.device, "fq")
+        F = torch.empty(self.nfunc, dtype=torch.float64, device=self.device)
+        check(_lib().iga_op_load_qp(self._h, fq.data_ptr(), F.data_ptr(), _stream_ptr(stream)),
+              "op_load_qp")
         return F
 
     def time_kernel(self, x, y, nlaunch=50, kernel=SF, assembly=INVLM_GATHER, stream=None,

@@ -345,6 +345,11 @@ def test_load_matches_oracle(built, name, kw):
         F = op.load(kind, val).cpu().numpy()
         Fr = O.load(om, kind, val)
         assert rel(F, Fr) < TOL64, (name, kind, rel(F, Fr))
+    # f given at the quadrature points (any values, here random)
+    fq = igamf.rng_normal(31, op.nelem * op.nqp)
+    F = op.load_qp(torch.from_numpy(fq).cuda()).cpu().numpy()
+    Fr = O.load_qp(om, fq)
+    assert rel(F, Fr) < TOL64, (name, "qp", rel(F, Fr))
 
 
 def test_large_annulus_invariants():

@@ -25,11 +25,13 @@ def rel(a, b):
 
 
 def kappa_field(m, seed):
-    """kappa_q = exp(0.5 s(x_q)), s a sum of 16 Fourier modes with N(0, 1/(m^2+n^2))
-    coefficients; x_q from the oracle's element geometry."""
+    """kappa_q = exp(0.5 s(x_q)), s a sum of 16 Fourier modes
+    sin(a pi x) sin(b pi y), a, b = 1..4, with N(0, 1/(a^2+b^2)) coefficients
+    (splitmix64 normals of the seed, a-major); x_q from the oracle's element
+    geometry.  The benchmark builds the same field on the device."""
     om = _om(m)
-    rng = np.random.default_rng(seed)
-    coef = {(a, b): rng.normal(0, 1 / np.sqrt(a * a + b * b)) for a in range(1, 5)
+    z = igamf.rng_normal(seed, 16)
+    coef = {(a, b): z[(a - 1) * 4 + b - 1] / np.sqrt(a * a + b * b) for a in range(1, 5)
             for b in range(1, 5)}
     out = []
     for ey in range(om.ney):

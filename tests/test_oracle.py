@@ -182,6 +182,22 @@ def test_load_matches_reference_golden(name):
     assert rel(O.load(golden_mesh(g), O.LOAD_REF_POLY), g["F_ref"]) < 1e-13
 
 
+@pytest.mark.parametrize("name", MF_GOLD)
+def test_load_qp_reproduces_the_reference_load(name):
+    """The load from values at the quadrature points, given the reference's
+    f = x(1-x)y(1-y) at the oracle's x_q, is the reference's load vector."""
+    g = gold(name)
+    m = golden_mesh(g)
+    fq = []
+    for ey in range(m.ney):
+        for ex in range(m.nex):
+            xq = O.element_qp(m, ex, ey)[4]
+            fq.append(xq[:, 0] * (1 - xq[:, 0]) * xq[:, 1] * (1 - xq[:, 1]))
+    F = O.load_qp(m, np.concatenate(fq))
+    assert rel(F, g["F_ref"]) < 1e-13
+    assert np.array_equal(F, O.load(m, O.LOAD_REF_POLY))
+
+
 def _bspline_1d_mats(p, n, L=1.0):
     """1D stiffness and mass matrices of the open uniform B-spline space by
     high-order Gauss quadrature with scipy B-splines (independent of the
