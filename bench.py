@@ -12,14 +12,19 @@ assembly).  Alongside, in the same JSON line:
             both assemblies, the three iteration forms (separate p.Ap kernel,
             p.Ap inside the apply, single-reduction iteration);
   block     multi-RHS apply, quarter annulus p = 4, 512 x 512, k = 32
-            (autotuned over k in {8, 16, 32});
+            (autotuned over k in {8, 16, 32}) and the block solve on the load
+            vectors of the k Fourier right-hand sides;
   c5        unit square p = 5, 2048 x 2048, perturbed interior points and
             per-function weights U[0.8, 1.2]: FP32 SF and FP64 EBE applies and
-            the GPU FEM-preconditioner assembly;
-  c1 / c4   the plain method (p = 2, 32^2, FP64 EBE + FEM preconditioner) and
-            the kappa adjoint (p = 3, 256^2) with their checks;
+            the GPU FEM-preconditioner assembly, its indices compared bit for
+            bit with the CPU's;
+  c1        the plain method (p = 2, 32^2, FP64 EBE + FEM preconditioner);
+  c4        the kappa adjoint (p = 3, 256^2, Fourier kappa of seed 1, u_obs
+            of seed 2 plus noise) with the gradient against central
+            differences along 16 directions;
   cpu_baseline  the oracle's OpenMP restatement of the reference apply on the
-            host cores.
+            host cores (all cores and one core), the C1 CPU solve and the C5
+            CPU FEM assembly.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-solve] ...
 """
@@ -49,7 +54,7 @@ def parse():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the C1 and C4 checks")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-applies", type=int, default=3)
+    ap.add_argument("--cpu-applies", type=int, default=10)
     ap.add_argument("--assembly", choices=["gather", "atomic"], default="gather")
     ap.add_argument("--kernel", choices=["sf", "ebe"], default="sf")
     ap.add_argument("--detail", action="store_true", help="also time every apply variant")
@@ -317,10 +322,39 @@ def solves(out, op, only=(), reps=3):
     out["solve"] = res
 
 
+def c3_coeffs(j):
+    """a_mn ~ N(0,1) of right-hand side j (splitmix64 seed j), a[m-1][n-1]."""
+    import igamf
+    return igamf.rng_normal(j, 64).reshape(8, 8)
+
+
+def c3_greville_columns(p, ne, k):
+    """f_j = sum_{m,n<=8} a_mn sin(m pi xi) sin(n pi eta) at the Greville
+    abscissae of the open uniform knots (row eta, column xi)."""
+    import numpy as np
+    nf = ne + p
+    knots = np.concatenate([np.zeros(p), np.linspace(0, 1, ne + 1), np.ones(p)])
+    gr = np.array([knots[i + 1:i + p + 1].mean() for i in range(nf)])
+    S = np.sin(np.pi * np.outer(gr, np.arange(1, 9)))
+    return np.stack([(S @ c3_coeffs(j) @ S.T).T.reshape(-1) for j in range(k)])
+
+
+def c3_qp_values(p, ne, j):
+    """f_j at the quadrature points' parametric coordinates, [E][nqp] in the
+    reference's element and point order (the input of iga_op_load_qp)."""
+    import numpy as np
+    nq = p + 1
+    g = (np.polynomial.legendre.leggauss(nq)[0] + 1) / 2
+    xi = ((np.arange(ne)[:, None] + g[None, :]) / ne).reshape(-1)
+    S = np.sin(np.pi * np.outer(xi, np.arange(1, 9)))
+    T = S @ c3_coeffs(j) @ S.T
+    return T.reshape(ne, nq, ne, nq).transpose(2, 0, 3, 1).reshape(-1)
+
+
 def block(out):
-    """Multi-RHS apply, quarter annulus p = 4, 512^2, 5 x 5 Gauss; RHS j is
-    sum_{m,n<=8} a_mn sin(m pi xi) sin(n pi eta) at the control points'
-    parameters (Greville abscissae), a_mn ~ N(0,1) from seed j."""
+    """Multi-RHS apply, quarter annulus p = 4, 512^2, 5 x 5 Gauss, k = 32:
+    the apply on f_j sampled at the Greville abscissae, the block solve on the
+    load vectors of f_j (a_mn ~ N(0,1) from seed j)."""
     import numpy as np
     import torch
     import igamf
@@ -328,17 +362,7 @@ def block(out):
     mesh = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, p, ne, r0=1.0, r1=2.0)
     op = igamf.Operator(mesh)
     n = op.nfunc
-    nf = ne + p
-    # Greville abscissae of the open uniform knot vector on [0, 1]
-    knots = np.concatenate([np.zeros(p), np.linspace(0, 1, ne + 1), np.ones(p)])
-    gr = np.array([knots[i + 1:i + p + 1].mean() for i in range(nf)])
-    m_ = np.arange(1, 9)
-    S = np.sin(np.pi * np.outer(gr, m_))          # [nf][8]
-    X = np.empty((k, n))
-    for j in range(k):
-        amn = igamf.rng_normal(j, 64).reshape(8, 8)
-        X[j] = (S @ amn @ S.T).T.reshape(-1)      # row eta, column xi
-    X = torch.from_numpy(X).cuda().float()
+    X = torch.from_numpy(c3_greville_columns(p, ne, k)).cuda().float()
     Y = torch.empty_like(X)
     x0 = X[0].contiguous()
     t_single, _ = ev_time(lambda: op.apply(x0), 20, 3)
@@ -365,11 +389,10 @@ def block(out):
         "note": "VALU (non-packed FP32) kernel; the flop count is the sum-factorised "
                 "16(p+1)^3 + 6(p+1)^2 per element and rhs"}
     log(f"block k={k}: {t_block * 1e3 / k:.1f} us/rhs, {tf:.1f} TFLOP/s")
-    # block PCG: the k right-hand sides F_j = P_int f_j (the same Greville
-    # values, Dirichlet rows zeroed) in one block solve, against k single
-    # FP32 + refinement solves and k plain FP64 solves, Jacobi, rtol 1e-8
-    mask = torch.from_numpy(mesh.arrays()["ID"] >= 0).cuda()
-    F = (X.double() * mask).contiguous()
+    # block PCG: the k load vectors F_j of f_j in one block solve, against k
+    # single FP32 + refinement solves and k plain FP64 solves, Jacobi, rtol 1e-8
+    F = torch.stack([op.load_qp(torch.from_numpy(c3_qp_values(p, ne, j)).cuda())
+                     for j in range(k)]).contiguous()
     sb = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F32_REFINE, pc=igamf.PC_JACOBI, rtol=1e-8,
                       maxit=200000)
     sb.solve_block(F)  # captures the block chunk graph
@@ -394,7 +417,7 @@ def block(out):
     ref = res["fp64"][1]
     out["block"]["solve"] = {
         "k": k, "rtol": 1e-8, "preconditioner": "jacobi",
-        "rhs": "F_j = f_j at the Greville points with Dirichlet rows zeroed",
+        "rhs": "F_j = load vector of f_j (iga_op_load_qp at the 5x5 Gauss points)",
         "block_fp32_refine_ms": round(t_bs, 2),
         "single_fp32_refine_ms_total": round(res["fp32_refine"][0], 2),
         "single_fp64_ms_total": round(res["fp64"][0], 2),
@@ -410,7 +433,8 @@ def block(out):
 
 def c5(out):
     """p = 5, 2048^2 unit square, interior points moved U[-0.1h, 0.1h] (seed
-    3), weights U[0.8, 1.2]: FP32 SF and FP64 EBE applies, FEM assembly."""
+    3), weights U[0.8, 1.2]: FP32 SF and FP64 EBE applies, FEM assembly.
+    Returns the assembled FEM matrix for the CPU check of its indices."""
     import torch
     import igamf
     p, ne = 5, 2048
@@ -430,6 +454,7 @@ def c5(out):
     t64e, _ = ev_time(lambda: op.apply(x, y64, kernel=igamf.EBE), 5, 1)
     nc = 5 if info.rational == 2 else 3
     b32 = 4 * (2 * op.nfunc) + 4 * nc * op.nelem * op.nqp + 4 * op.nfunc
+    del op
     nodes = igamf.fem_reference_nodes(p, p, ne, ne)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -448,14 +473,16 @@ def c5(out):
         "fem_assembly_ms": round(t_fem * 1e3, 2), "fem_nnz": fem.view.nnz,
         "setup_s": round(setup, 2)}
     log(f"c5 f32 sf {t32 * 1e3:.0f} us, f64 ebe {t64e * 1e3:.0f} us")
+    return fem, nodes, p, ne
 
 
-def c1_c4(out):
+def c1(out):
+    """C1: FP64 EBE apply (100 applies, x ~ N(0,1) seed 0) and the solve to
+    rtol 1e-10 inside CG with the Q1 FEM preconditioner: ILU(0) (the
+    configuration's) and the reference GPU driver's inner CG + Jacobi."""
     import numpy as np
     import torch
     import igamf
-    from igamf import autograd as IA
-    # C1: FP64 EBE apply inside CG with the FEM (Q1) preconditioner, rtol 1e-10
     p, ne = 2, 32
     mesh = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, ne)
     op = igamf.Operator(mesh)
@@ -482,41 +509,107 @@ def c1_c4(out):
         out["c1"][name] = {"solve_ms": round(st.solve_ms, 2), "iterations": st.iterations,
                            "rel_residual": st.rel_residual,
                            "max_err_vs_minus_sinsin_at_cp": err}
-    # C4: kappa adjoint gradient against central differences
+    log(f"c1 solve ilu {out['c1']['ilu']['iterations']} its, fem-cg "
+        f"{out['c1']['fem_cg']['iterations']} its")
+
+
+def c4_kappa(op, seed):
+    """kappa_q = exp(0.5 s(x_q)), s = sum_{a,b<=4} c_ab sin(a pi x) sin(b pi y),
+    c_ab ~ N(0, 1/(a^2+b^2)) (splitmix64 normals of the seed, a-major), at
+    the physical quadrature points; float64 CUDA [E * nqp]."""
+    import numpy as np
+    import torch
+    import igamf
+    xq = op.qp_coords().reshape(-1, 2)
+    z = igamf.rng_normal(seed, 16)
+    s = torch.zeros(xq.shape[0], dtype=torch.float64, device=xq.device)
+    for a in range(1, 5):
+        for b in range(1, 5):
+            c = z[(a - 1) * 4 + b - 1] / np.sqrt(a * a + b * b)
+            s += c * torch.sin(a * np.pi * xq[:, 0]) * torch.sin(b * np.pi * xq[:, 1])
+    return torch.exp(0.5 * s).contiguous()
+
+
+def c4(out):
+    """C4: -div(kappa grad u) = f on the unit square, p = 3, 256^2, 4 x 4
+    Gauss, f = 1; kappa from seed 1, u_obs = the solution for the seed-2 kappa
+    plus N(0, 1e-3) noise (seed 9) on the interior; J = 1/2 |u - u_obs|^2.
+    Forward solve, adjoint solve and the grad_kappa kernel, timed; the
+    gradient against FP64 central differences (step 1e-6) along 16 random
+    directions (seeds 300..315)."""
+    import torch
+    import igamf
+    from igamf import autograd as IA
     p, ne = 3, 256
     mesh = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, ne)
     op = igamf.Operator(mesh)
-    E, nqp = op.nelem, op.nqp
-    kap = torch.ones(E * nqp, dtype=torch.float64, device="cuda")
-    op.set_kappa(kap)
-    F = op.load(igamf.LOAD_SINSIN, 2 * np.pi ** 2)
-    s = igamf.Solver(op, None, pc=igamf.PC_JACOBI, rtol=1e-12, maxit=200000)
-    uobs = torch.from_numpy(igamf.rng_normal(2, op.nfunc)).cuda() * 1e-3
-    k = kap.clone().requires_grad_()
+    f = op.load(igamf.LOAD_CONST, 1.0)
+    s = igamf.Solver(op, None, pc=igamf.PC_JACOBI, rtol=1e-13, maxit=200000)
+    k1, k2 = c4_kappa(op, 1), c4_kappa(op, 2)
+    interior = IA.interior_mask(op)
+    with torch.no_grad():
+        u_obs = IA.solve_kappa(s, f, k2)
+        u_obs = u_obs + 1e-3 * torch.from_numpy(igamf.rng_normal(9, op.nfunc)).cuda() * interior
+    IA.solve_kappa(s, f, k1)  # graph capture outside the timed region
+    torch.cuda.synchronize()
+    k = k1.clone().requires_grad_()
     t0 = time.perf_counter()
-    u = IA.solve_kappa(s, F, k)
-    J = 0.5 * ((u - uobs) ** 2).sum()
+    u = IA.solve_kappa(s, f, k)
+    J = 0.5 * ((u - u_obs) ** 2).sum()
     J.backward()
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    grad = k.grad.clone()
+    h = 1e-6
     errs = []
     with torch.no_grad():
-        for d in range(4):
-            dk = torch.from_numpy(igamf.rng_normal(100 + d, E * nqp)).cuda()
-            h = 1e-6
-            jp = 0.5 * ((IA.solve_kappa(s, F, kap + h * dk) - uobs) ** 2).sum().item()
-            jm = 0.5 * ((IA.solve_kappa(s, F, kap - h * dk) - uobs) ** 2).sum().item()
+        for d in range(16):
+            dk = torch.from_numpy(igamf.rng_normal(300 + d, k1.numel())).cuda()
+            jp = 0.5 * ((IA.solve_kappa(s, f, k1 + h * dk) - u_obs) ** 2).sum().item()
+            jm = 0.5 * ((IA.solve_kappa(s, f, k1 - h * dk) - u_obs) ** 2).sum().item()
             fd = (jp - jm) / (2 * h)
-            an = torch.dot(k.grad, dk).item()
+            an = torch.dot(grad, dk).item()
             errs.append(abs(fd - an) / max(abs(fd), 1e-300))
-    out["c4"] = {"workload": "unit square p=3 256x256, kappa = 1, J = 0.5|u - u_obs|^2",
-                 "forward_adjoint_grad_s": round(t_grad, 3),
-                 "fd_rel_err_max_4dirs": max(errs)}
-    log(f"c1 solve ilu {out['c1']['ilu']['iterations']} its, fem-cg "
-        f"{out['c1']['fem_cg']['iterations']} its; c4 fd err {max(errs):.2e}")
+    out["c4"] = {"workload": "unit square p=3 256x256, 4x4 Gauss, kappa = exp(0.5 s(x_q)) with "
+                             "16 Fourier modes (seed 1), u_obs from the seed-2 kappa + N(0,1e-3) "
+                             "noise, J = 0.5|u - u_obs|^2, f = 1, Jacobi PCG to rtol 1e-13",
+                 "J": J.item(),
+                 "forward_adjoint_grad_s": round(t_grad, 4),
+                 "fd_directions": 16, "fd_step": h,
+                 "fd_rel_err_max": max(errs), "fd_rel_err_median": median(errs)}
+    log(f"c4 forward+adjoint+gradient {t_grad * 1e3:.1f} ms; FD rel err max {max(errs):.2e} "
+        f"over 16 directions")
+
+
+def _cpu_pcg_ilu(O, m, F, rowptr, col, lu, diag, rtol):
+    """CG on the oracle's apply with the oracle's ILU(0) of the Q1 matrix (the
+    reference CPU driver's KSPCG + PCILU; zero initial guess, ||f - A u|| /
+    ||f|| <= rtol)."""
+    import numpy as np
+    u = np.zeros_like(F)
+    r = F.copy()
+    z = O.ilu0_solve(rowptr, col, lu, diag, r)
+    pv = z.copy()
+    rz = r @ z
+    nf = np.linalg.norm(F)
+    for it in range(1, 10000):
+        Ap = O.apply_mf(m, pv)
+        a = rz / (pv @ Ap)
+        u += a * pv
+        r -= a * Ap
+        if np.linalg.norm(r) <= rtol * nf:
+            return u, it
+        z = O.ilu0_solve(rowptr, col, lu, diag, r)
+        rz, rz_old = r @ z, rz
+        pv = z + (rz / rz_old) * pv
+    return u, it
 
 
 def cpu_baseline(a, out, mesh, x64, op, kern, asm, n_int):
+    """The oracle's restatement of the reference CPU path on the host cores:
+    C2 applies on all cores (median of --cpu-applies), one apply on one core
+    of a 1024 x 128 element strip of the same annulus, C1 applies (100) and
+    the C1 solve (CG + ILU(0) of the Q1 matrix, rtol 1e-10)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -541,6 +634,60 @@ def cpu_baseline(a, out, mesh, x64, op, kern, asm, n_int):
     y64 = op.apply(x64, kernel=kern, assembly=asm).cpu().numpy()
     out["cpu_baseline"]["gpu_f64_rel_diff"] = float(np.abs(y64 - yc).max() / np.abs(yc).max())
     log(f"cpu baseline {tc:.2f} s per apply on {used} threads")
+    # one core, bounded sample: a strip of 1024 x 128 elements of the annulus
+    sm = O.mesh_quarter_annulus(a.p, a.nelem, a.nelem // 8)
+    sx = O.rng_normal(0, sm.nfunc)
+    t0 = time.perf_counter()
+    O.apply_mf_colored(sm, sx, 1)
+    t1c = time.perf_counter() - t0
+    s_int = (a.nelem + a.p - 2) * (a.nelem // 8 + a.p - 2)
+    out["cpu_baseline"]["one_core"] = {
+        "value": round(s_int / t1c / 1e9, 6), "unit": "GDoF/s", "cores": 1,
+        "sample": f"one FP64 apply of a {a.nelem}x{a.nelem // 8} element strip of the same "
+                  f"annulus, p={a.p} ({t1c:.2f} s)"}
+    # C1: 100 applies on one core and on all, the CPU solve
+    m1 = O.mesh_unit_square(2, 32)
+    x1 = O.rng_normal(0, m1.nfunc)
+    c1 = {}
+    for nt in (1, threads):
+        t0 = time.perf_counter()
+        for _ in range(100):
+            O.apply_mf_colored(m1, x1, nt)
+        c1[f"apply_ms_{nt}_cores"] = round((time.perf_counter() - t0) * 1e3 / 100, 4)
+    F1 = O.load(m1, O.LOAD_SINSIN, 2 * np.pi ** 2)
+    rows, cols, _, _, val = O.fem_assemble(34, 34, O.fem_nodes(2, 2, 32, 32))
+    rowptr = np.searchsorted(rows, np.arange(34 * 34 + 1))
+    t0 = time.perf_counter()
+    lu, diag = O.ilu0(rowptr, cols, val)
+    u1, its = _cpu_pcg_ilu(O, m1, F1, rowptr, cols, lu, diag, 1e-10)
+    c1["solve_s"] = round(time.perf_counter() - t0, 4)
+    c1["solve_iterations"] = its
+    c1["solve"] = "CG on the oracle apply + ILU(0) of the Q1 matrix (Python triangular solves)"
+    out["cpu_baseline"]["c1"] = c1
+    log(f"cpu one core {t1c:.2f} s on the strip; c1 cpu solve {its} its")
+
+
+def cpu_check_fem(out, fem, nodes, p, ne):
+    """C5 FEM assembly on the CPU (the oracle's restatement of elem2coo /
+    dir2coo / the Q1 values), timed, and the GPU indices compared bit for
+    bit with it."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nf = ne + p
+    t0 = time.perf_counter()
+    rows, cols, e2c, d2c, val = O.fem_assemble(nf, nf, nodes)
+    t_cpu = time.perf_counter() - t0
+    t = fem.to_torch()
+    same = (np.array_equal(t["coo_row"].cpu().numpy(), rows) and
+            np.array_equal(t["col"].cpu().numpy(), cols) and
+            np.array_equal(t["elem2coo"].cpu().numpy().reshape(-1), e2c) and
+            np.array_equal(t["dir2coo"].cpu().numpy(), d2c))
+    vd = float(np.abs(t["val"].cpu().numpy() - val).max() / np.abs(val).max())
+    out["c5"]["fem_indices_match_cpu"] = bool(same)
+    out["c5"]["fem_values_max_rel_diff_cpu"] = vd
+    out["cpu_baseline"]["fem_assembly_c5_s"] = round(t_cpu, 3)
+    log(f"c5 fem indices match the CPU: {same}; CPU assembly {t_cpu:.1f} s")
 
 
 def main():
@@ -568,9 +715,13 @@ def main():
         if not a.no_block:
             block(out)
         if not a.no_c5:
-            c5(out)
+            fem, nodes, p5, ne5 = c5(out)
+            if not a.no_cpu:
+                cpu_check_fem(out, fem, nodes, p5, ne5)
+            del fem
         if not a.no_extra:
-            c1_c4(out)
+            c1(out)
+            c4(out)
     print(json.dumps(out), flush=True)
 
 
