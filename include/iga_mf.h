@@ -129,6 +129,53 @@ typedef struct {
  * base_name + "_" + r + ".txt" (:450-452). */
 int iga_read_preprocess_info(const char *filename, iga_preprocess_info *info);
 
+/* ---- the FEM preconditioner's files --------------------------------------
+ * Replaces FileManager's readers of the GPU driver's FEM set-up
+ * (matrix_free_gpu/matrixfreegpu.cu:139-188).
+ *
+ * A FEM partition file, the short WritePartition form
+ * (matrix_free/src/FileManager.cpp:71-112, read by ReadPartition :282-373):
+ * "nlocalfunc: N", "nlocalelemx: N", "nlocalelemy: N", then the sections ID,
+ * Dir, IEN (4 local indices per Q1 element) and CP (x, y; 16 significant
+ * digits).  ID and CP list the owned functions first, then the ghosts
+ * (matrix_free_gpu/src/Partition.cpp:395-466). */
+typedef struct iga_fem_partition iga_fem_partition;
+typedef struct {
+  int nlocalfunc, nlocalelemx, nlocalelemy;
+  int nid;            /* owned + ghost functions: entries of id, pairs of cp */
+  int ndir;
+  const int *id;      /* [nid] partition-numbered IDs, -1 = Dirichlet */
+  const int *dir;     /* [ndir] */
+  const int *ien;     /* [nlocalelemx * nlocalelemy][4] indices into id / cp */
+  const double *cp;   /* [nid][2] */
+} iga_fem_partition_desc;
+int iga_fem_read_partition(const char *filename, iga_fem_partition **out);
+int iga_fem_partition_view(const iga_fem_partition *fp, iga_fem_partition_desc *desc);
+int iga_fem_partition_destroy(iga_fem_partition *fp);
+/* The dry run's sparsity file coordinate_nonzero_<rank>.txt ("nnz: N",
+ * "rows: ...", "cols: ...": matrix_free_gpu/src/FileManager.cpp:684-749,
+ * written by GlobalAssemblyDR.cpp:3-26).  *nnz receives the count; with
+ * cap < *nnz nothing is copied and IGA_ERR_SIZE_LIMIT returned (call with
+ * cap = 0 to size the buffers). */
+int iga_read_nonzero_coordinate(const char *filename, int64_t cap, int *rows, int *cols,
+                                int64_t *nnz);
+int iga_write_nonzero_coordinate(const char *filename, int64_t nnz, const int *rows,
+                                 const int *cols);
+/* new_to_old_mapping.txt, one line of the partition numbering's original
+ * indices (matrix_free_gpu/src/FileManager.cpp:751-786, written by
+ * Partition.cpp:342-365); same size protocol. */
+int iga_read_new_to_old(const char *filename, int64_t cap, int *map, int64_t *n);
+int iga_write_new_to_old(const char *filename, int64_t n, const int *map);
+/* The GPU driver's merge (matrix_free_gpu/matrixfreegpu.cu:149-188): the
+ * coordinate_nonzero_<r>.txt files of ranks 0..nparts-1 in directory dir,
+ * mapped through dir/new_to_old_mapping.txt to the natural numbering and
+ * sorted lexicographically: the FEM matrix's COO pattern, the one
+ * iga_fem_assemble computes.  The serial preprocessing writes no mapping
+ * file; without one the numbering is taken as natural (the reference would
+ * read an empty map).  Same size protocol. */
+int iga_fem_pattern_from_files(const char *dir, int nparts, int64_t cap, int *rows, int *cols,
+                               int64_t *nnz);
+
 /* splitmix64 -> Box-Muller, the generator of every synthetic input */
 int iga_rng_normal(uint64_t seed, int64_t n, double *out);
 int iga_rng_uniform(uint64_t seed, int64_t n, double lo, double hi, double *out);

@@ -421,6 +421,17 @@ def ref_gpu():
         L.ref_fem_assemble.restype = C.c_long
         L.ref_invlm.argtypes = [C.c_int] * 3 + [_I32P] * 5
         L.ref_invlm.restype = C.c_long
+        L.ref_fem_write_partitions.argtypes = [C.c_char_p] + [C.c_int] * 5 + [_I32P]
+        L.ref_fem_read_partition.argtypes = [C.c_char_p, C.c_long, _I32P, _I32P, _I32P,
+                                             _I32P, _F64P]
+        L.ref_write_nonzero.argtypes = [C.c_char_p, C.c_int, _I32P, _I32P]
+        L.ref_write_nonzero.restype = None
+        L.ref_read_nonzero.argtypes = [C.c_char_p, C.c_long, _I32P, _I32P]
+        L.ref_read_nonzero.restype = C.c_long
+        L.ref_write_new_to_old.argtypes = [C.c_char_p, C.c_int, _I32P]
+        L.ref_write_new_to_old.restype = None
+        L.ref_read_new_to_old.argtypes = [C.c_char_p, C.c_long, _I32P]
+        L.ref_read_new_to_old.restype = C.c_long
     return _ref_gpu
 
 

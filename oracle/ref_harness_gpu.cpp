@@ -1,25 +1,30 @@
 // ref_harness_gpu.cpp -- C entry points over the reference's FEM-preconditioner
 // and inverse-LM host code (matrix_free_gpu/src/{ElementFEM,Elem2COOGenerator,
-// AbscissaeGenerator,QuadraturePoint}.cpp and matrix_free_gpu_dbd/src/
-// InvLM.cpp, compiled from /root/reference by oracle/Makefile into
-// oracle/_ref/libref_gpu.so).  TEST INFRASTRUCTURE ONLY.
+// AbscissaeGenerator,QuadraturePoint,FileManager,Partition}.cpp and
+// matrix_free_gpu_dbd/src/InvLM.cpp, compiled from /root/reference by
+// oracle/Makefile into oracle/_ref/libref_gpu.so).  TEST INFRASTRUCTURE ONLY.
 //
 // Restated here (the reference runs them inside PETSc or as CUDA kernels):
 //   - the FEM mesh set-up of matrix_free_gpu/matrixfreegpu.cu:104-137,
 //   - the COO pattern that the dry run writes (GlobalAssemblyDR.cpp:33-94),
 //   - the value loop of AssembleStiffnessKernel / DirichletBCKKernel
 //     (matrix_free_gpu/src/assembly.cu:41-99), run sequentially.
+#include <unistd.h>
+
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <utility>
 #include <vector>
 
 #include "AbscissaeGenerator.hpp"
 #include "Elem2COOGenerator.hpp"
 #include "ElementFEM.hpp"
+#include "FileManager.hpp"
 #include "IDGenerator.hpp"
 #include "IENGenerator.hpp"
 #include "InvLM.hpp"
+#include "Partition.hpp"
 #include "QuadraturePoint.hpp"
 
 namespace {
@@ -156,6 +161,93 @@ long ref_invlm(int nloc, int nelem, int nfunc, const int *IEN, int *elemNum,
   std::memcpy(elemIdx, c.data(), sizeof(int) * c.size());
   std::memcpy(baseIdx, d.data(), sizeof(int) * d.size());
   return (long)c.size();
+}
+
+// The FEM preprocessing of matrix_free_gpu/preprocess_fem.cpp:15-70 on
+// [0,1]^2 (Greville nodes of degree p-2, Q1 IEN, ID with -1 on the boundary),
+// partitioned by the reference's Partition::GeneratePartition into
+// part_fem_<r>.txt and new_to_old_mapping.txt in directory dir (both written
+// relative to the working directory, hence the chdir).  Returns the node
+// count per direction in nf[0..1].
+int ref_fem_write_partitions(const char *dir, int p, int q, int nElemX, int nElemY,
+                             int part_num_1d, int *nf) {
+  char old[4096];
+  if (!getcwd(old, sizeof(old)) || chdir(dir) != 0) return -1;
+  const double Lx = 1.0, Ly = 1.0;
+  const double hx = Lx / (nElemX + 2), hy = Ly / (nElemY + 2);
+  const int nFuncX = p + nElemX, nFuncY = q + nElemY;
+  std::vector<double> S, T;
+  for (int i = 0; i < p - 1; ++i) S.push_back(0.0);
+  for (int i = 1; i < nElemX + 2; ++i) S.push_back(i * hx);
+  for (int i = 0; i < p - 1; ++i) S.push_back(Lx);
+  for (int i = 0; i < q - 1; ++i) T.push_back(0.0);
+  for (int i = 1; i < nElemY + 2; ++i) T.push_back(i * hy);
+  for (int i = 0; i < q - 1; ++i) T.push_back(Ly);
+  AbscissaeGenerator ag;
+  std::vector<double> CP = ag.GenerateAbscissae2D(S, T, p - 2, q - 2);
+  IENGenerator ig;
+  std::vector<int> IEN = ig.GenerateIEN2D(nFuncX - 1, nFuncY - 1);
+  IDGenerator idg;
+  std::vector<int> ID = idg.GenerateID2D(nFuncX, nFuncY);
+  Partition part(part_num_1d, part_num_1d, 2, "part_fem");
+  part.GeneratePartition(nFuncX - 1, nFuncY - 1, CP, IEN, ID);
+  nf[0] = nFuncX;
+  nf[1] = nFuncY;
+  return chdir(old) == 0 ? 0 : -1;
+}
+
+// FileManager::ReadPartition (short form).  counts: nlocalfunc, nlocalelemx,
+// nlocalelemy, |ID|, |Dir|, |IEN|, |CP|; arrays copied when they fit cap.
+int ref_fem_read_partition(const char *fn, long cap, int *counts, int *id, int *dir,
+                           int *ien, double *cp) {
+  FileManager fm;
+  int nf = 0, nx = 0, ny = 0;
+  std::vector<double> CP;
+  std::vector<int> ID, Dir, IEN;
+  fm.ReadPartition(fn, nf, nx, ny, CP, ID, Dir, IEN);
+  const long sz[7] = {nf, nx, ny, (long)ID.size(), (long)Dir.size(), (long)IEN.size(),
+                      (long)CP.size()};
+  for (int k = 0; k < 7; ++k) counts[k] = (int)sz[k];
+  if ((long)ID.size() > cap || (long)Dir.size() > cap || (long)IEN.size() > cap ||
+      (long)CP.size() > cap)
+    return -1;
+  std::copy(ID.begin(), ID.end(), id);
+  std::copy(Dir.begin(), Dir.end(), dir);
+  std::copy(IEN.begin(), IEN.end(), ien);
+  std::copy(CP.begin(), CP.end(), cp);
+  return 0;
+}
+
+void ref_write_nonzero(const char *fn, int nnz, const int *rows, const int *cols) {
+  FileManager fm;
+  fm.WriteNonZeroCoordinate(fn, nnz, std::vector<int>(rows, rows + nnz),
+                            std::vector<int>(cols, cols + nnz));
+}
+
+// FileManager::ReadNonZeroCoordinate; returns nnz (negative when over cap)
+long ref_read_nonzero(const char *fn, long cap, int *rows, int *cols) {
+  FileManager fm;
+  int nnz = 0;
+  std::vector<int> r, c;
+  fm.ReadNonZeroCoordinate(fn, nnz, r, c);
+  if ((long)r.size() > cap || r.size() != c.size()) return -(long)r.size() - 1;
+  std::copy(r.begin(), r.end(), rows);
+  std::copy(c.begin(), c.end(), cols);
+  return nnz;
+}
+
+void ref_write_new_to_old(const char *fn, int n, const int *map) {
+  FileManager fm;
+  fm.WriteNewToOldMapping(fn, std::vector<int>(map, map + n));
+}
+
+long ref_read_new_to_old(const char *fn, long cap, int *map) {
+  FileManager fm;
+  std::vector<int> v;
+  fm.ReadNewToOldMapping(fn, v);
+  if ((long)v.size() > cap) return -(long)v.size() - 1;
+  std::copy(v.begin(), v.end(), map);
+  return (long)v.size();
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 // level.  Geometry coefficients of polynomial maps (the Greville points of
 // the identity map, the homogeneous quarter circle) come from Marsden's
 // identity: coefficient i is the blossom at (t_{i+1}, ..., t_{i+p}).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -30,6 +31,12 @@ struct iga_mesh {
   std::vector<double> cp, w, C1, C2, h1, h2;
   std::vector<int> ien, id, dir;
   bool has_w = false;
+};
+
+struct iga_fem_partition {
+  int nlocalfunc = 0, nex = 0, ney = 0;
+  std::vector<int> id, dir, ien;
+  std::vector<double> cp;
 };
 
 namespace {
@@ -271,21 +278,19 @@ int iga_mesh_destroy(iga_mesh *m) {
   return IGA_OK;
 }
 
-int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out) {
-  if (!filename || !out) return IGA_ERR_INVALID_ARGUMENT;
-  *out = nullptr;
-  if (p < 1 || p > 8 || q < 1 || q > 8) return IGA_ERR_INVALID_DEGREE;
+}  // extern "C"
+
+namespace {
+// The lines of a text file (without line ends); false when it cannot be read.
+bool read_lines(const char *filename, std::vector<std::string> &lines) {
   std::FILE *f = std::fopen(filename, "r");
-  if (!f) return IGA_ERR_INVALID_ARGUMENT;
+  if (!f) return false;
   std::string text;
-  {
-    char buf[1 << 16];
-    size_t n;
-    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
-    std::fclose(f);
-  }
-  // split into lines
-  std::vector<std::string> lines;
+  char buf[1 << 16];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+  std::fclose(f);
+  lines.clear();
   for (size_t a = 0; a <= text.size();) {
     size_t b = text.find('\n', a);
     if (b == std::string::npos) b = text.size();
@@ -294,33 +299,65 @@ int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out) 
     lines.push_back(l);
     a = b + 1;
   }
-  auto header = [&](size_t i, const char *key, long &v) {
-    const size_t k = std::strlen(key);
-    if (i >= lines.size() || lines[i].compare(0, k, key) != 0) return false;
+  return true;
+}
+
+// "key: N" on line i
+bool header_value(const std::vector<std::string> &lines, size_t i, const char *key, long &v) {
+  const size_t k = std::strlen(key);
+  if (i >= lines.size() || lines[i].compare(0, k, key) != 0) return false;
+  char *end = nullptr;
+  v = std::strtol(lines[i].c_str() + k, &end, 10);
+  return end != lines[i].c_str() + k;
+}
+
+// The whitespace-separated numbers of a string, from offset `from`.
+template <typename T>
+void numbers(const std::string &l, size_t from, std::vector<T> &v) {
+  v.clear();
+  const char *c = l.c_str() + (from < l.size() ? from : l.size());
+  for (;;) {
     char *end = nullptr;
-    v = std::strtol(lines[i].c_str() + k, &end, 10);
-    return end != lines[i].c_str() + k;
-  };
-  long nfunc = 0, nex = 0, ney = 0;
-  if (!header(0, "nlocalfunc: ", nfunc) || !header(1, "nlocalelemx: ", nex) ||
-      !header(2, "nlocalelemy: ", ney) || nfunc < 1 || nex < 1 || ney < 1)
-    return IGA_ERR_BAD_MESH;
+    const double d = std::strtod(c, &end);
+    if (end == c) break;
+    v.push_back((T)d);
+    c = end;
+  }
+}
+
+// Named sections after the three header lines of a partition file: a label
+// line, then one line of values (FileManager::WritePartition's layout).
+std::map<std::string, std::vector<double>> sections(const std::vector<std::string> &lines) {
   std::map<std::string, std::vector<double>> sec;
   for (size_t i = 3; i + 1 < lines.size(); ++i) {
     const std::string &name = lines[i];
     if (name.empty()) continue;
-    std::vector<double> v;
-    const char *c = lines[i + 1].c_str();
-    for (;;) {
-      char *end = nullptr;
-      const double d = std::strtod(c, &end);
-      if (end == c) break;
-      v.push_back(d);
-      c = end;
-    }
-    sec[name] = std::move(v);
+    numbers(lines[i + 1], 0, sec[name]);
     ++i;
   }
+  return sec;
+}
+
+void to_ints(const std::vector<double> &v, std::vector<int> &o) {
+  o.resize(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = (int)v[i];
+}
+}  // namespace
+
+extern "C" {
+
+int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out) {
+  if (!filename || !out) return IGA_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (p < 1 || p > 8 || q < 1 || q > 8) return IGA_ERR_INVALID_DEGREE;
+  std::vector<std::string> lines;
+  if (!read_lines(filename, lines)) return IGA_ERR_INVALID_ARGUMENT;
+  long nfunc = 0, nex = 0, ney = 0;
+  if (!header_value(lines, 0, "nlocalfunc: ", nfunc) ||
+      !header_value(lines, 1, "nlocalelemx: ", nex) ||
+      !header_value(lines, 2, "nlocalelemy: ", ney) || nfunc < 1 || nex < 1 || ney < 1)
+    return IGA_ERR_BAD_MESH;
+  std::map<std::string, std::vector<double>> sec = sections(lines);
   const char *need[] = {"ID", "Dir", "IEN", "ElemSize1", "ElemSize2", "CP", "NURBSExtraction1",
                         "NURBSExtraction2"};
   for (const char *k : need)
@@ -343,13 +380,9 @@ int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out) 
   m->h2 = sec["ElemSize2"];
   m->C1 = sec["NURBSExtraction1"];
   m->C2 = sec["NURBSExtraction2"];
-  auto ints = [](const std::vector<double> &v, std::vector<int> &o) {
-    o.resize(v.size());
-    for (size_t i = 0; i < v.size(); ++i) o[i] = (int)v[i];
-  };
-  ints(sec["ID"], m->id);
-  ints(sec["Dir"], m->dir);
-  ints(sec["IEN"], m->ien);
+  to_ints(sec["ID"], m->id);
+  to_ints(sec["Dir"], m->dir);
+  to_ints(sec["IEN"], m->ien);
   *out = m;
   return IGA_OK;
 }
@@ -411,6 +444,178 @@ int iga_read_preprocess_info(const char *filename, iga_preprocess_info *info) {
   }
   std::fclose(f);
   return seen == 15 ? IGA_OK : IGA_ERR_BAD_MESH;
+}
+
+int iga_fem_read_partition(const char *filename, iga_fem_partition **out) {
+  if (!filename || !out) return IGA_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::vector<std::string> lines;
+  if (!read_lines(filename, lines)) return IGA_ERR_INVALID_ARGUMENT;
+  long nfunc = 0, nex = 0, ney = 0;
+  if (!header_value(lines, 0, "nlocalfunc: ", nfunc) ||
+      !header_value(lines, 1, "nlocalelemx: ", nex) ||
+      !header_value(lines, 2, "nlocalelemy: ", ney) || nfunc < 1 || nex < 1 || ney < 1)
+    return IGA_ERR_BAD_MESH;
+  std::map<std::string, std::vector<double>> sec = sections(lines);
+  for (const char *k : {"ID", "Dir", "IEN", "CP"})
+    if (!sec.count(k)) return IGA_ERR_BAD_MESH;
+  // owned functions first, then ghosts: ID and CP cover both
+  const size_t nid = sec["ID"].size();
+  if (nid < (size_t)nfunc || sec["CP"].size() != 2 * nid ||
+      sec["IEN"].size() != (size_t)nex * ney * 4)
+    return IGA_ERR_BAD_MESH;
+  iga_fem_partition *fp = new (std::nothrow) iga_fem_partition();
+  if (!fp) return IGA_ERR_OUT_OF_MEMORY;
+  fp->nlocalfunc = (int)nfunc;
+  fp->nex = (int)nex;
+  fp->ney = (int)ney;
+  to_ints(sec["ID"], fp->id);
+  to_ints(sec["Dir"], fp->dir);
+  to_ints(sec["IEN"], fp->ien);
+  fp->cp = sec["CP"];
+  for (int a : fp->ien)
+    if (a < 0 || (size_t)a >= nid) {
+      delete fp;
+      return IGA_ERR_BAD_MESH;
+    }
+  *out = fp;
+  return IGA_OK;
+}
+
+int iga_fem_partition_view(const iga_fem_partition *fp, iga_fem_partition_desc *d) {
+  if (!fp || !d) return IGA_ERR_INVALID_ARGUMENT;
+  d->nlocalfunc = fp->nlocalfunc;
+  d->nlocalelemx = fp->nex;
+  d->nlocalelemy = fp->ney;
+  d->nid = (int)fp->id.size();
+  d->ndir = (int)fp->dir.size();
+  d->id = fp->id.data();
+  d->dir = fp->dir.data();
+  d->ien = fp->ien.data();
+  d->cp = fp->cp.data();
+  return IGA_OK;
+}
+
+int iga_fem_partition_destroy(iga_fem_partition *fp) {
+  delete fp;
+  return IGA_OK;
+}
+
+int iga_read_nonzero_coordinate(const char *filename, int64_t cap, int *rows, int *cols,
+                                int64_t *nnz) {
+  if (!filename || !nnz || cap < 0 || (cap > 0 && (!rows || !cols)))
+    return IGA_ERR_INVALID_ARGUMENT;
+  std::vector<std::string> lines;
+  if (!read_lines(filename, lines)) return IGA_ERR_INVALID_ARGUMENT;
+  long n = -1;
+  std::vector<long> r, c;
+  bool have_r = false, have_c = false;
+  for (size_t i = 0; i < lines.size(); ++i) {
+    const std::string &l = lines[i];
+    if (l.compare(0, 5, "nnz: ") == 0) (void)header_value(lines, i, "nnz: ", n);
+    else if (l.compare(0, 6, "rows: ") == 0) numbers(l, 6, r), have_r = true;
+    else if (l.compare(0, 6, "cols: ") == 0) numbers(l, 6, c), have_c = true;
+  }
+  if (n < 0 || !have_r || !have_c || r.size() != c.size() || (long)r.size() != n)
+    return IGA_ERR_BAD_MESH;
+  *nnz = n;
+  if (n > cap) return IGA_ERR_SIZE_LIMIT;
+  for (long k = 0; k < n; ++k) {
+    rows[k] = (int)r[k];
+    cols[k] = (int)c[k];
+  }
+  return IGA_OK;
+}
+
+int iga_write_nonzero_coordinate(const char *filename, int64_t nnz, const int *rows,
+                                 const int *cols) {
+  if (!filename || nnz < 0 || (nnz > 0 && (!rows || !cols))) return IGA_ERR_INVALID_ARGUMENT;
+  std::FILE *f = std::fopen(filename, "w");
+  if (!f) return IGA_ERR_INVALID_ARGUMENT;
+  std::fprintf(f, "nnz: %lld\nrows: ", (long long)nnz);
+  for (int64_t k = 0; k < nnz; ++k) std::fprintf(f, "%d ", rows[k]);
+  std::fprintf(f, "\ncols: ");
+  for (int64_t k = 0; k < nnz; ++k) std::fprintf(f, "%d ", cols[k]);
+  std::fprintf(f, "\n");
+  return std::fclose(f) == 0 ? IGA_OK : IGA_ERR_INVALID_ARGUMENT;
+}
+
+int iga_read_new_to_old(const char *filename, int64_t cap, int *map, int64_t *n) {
+  if (!filename || !n || cap < 0 || (cap > 0 && !map)) return IGA_ERR_INVALID_ARGUMENT;
+  std::vector<std::string> lines;
+  if (!read_lines(filename, lines)) return IGA_ERR_INVALID_ARGUMENT;
+  std::vector<long> v;
+  if (!lines.empty()) numbers(lines[0], 0, v);  // the first line only, as the reference
+  *n = (int64_t)v.size();
+  if ((int64_t)v.size() > cap) return IGA_ERR_SIZE_LIMIT;
+  for (size_t i = 0; i < v.size(); ++i) map[i] = (int)v[i];
+  return IGA_OK;
+}
+
+int iga_write_new_to_old(const char *filename, int64_t n, const int *map) {
+  if (!filename || n < 0 || (n > 0 && !map)) return IGA_ERR_INVALID_ARGUMENT;
+  std::FILE *f = std::fopen(filename, "w");
+  if (!f) return IGA_ERR_INVALID_ARGUMENT;
+  for (int64_t i = 0; i < n; ++i) std::fprintf(f, "%d ", map[i]);
+  std::fprintf(f, "\n");
+  return std::fclose(f) == 0 ? IGA_OK : IGA_ERR_INVALID_ARGUMENT;
+}
+
+int iga_fem_pattern_from_files(const char *dir, int nparts, int64_t cap, int *rows, int *cols,
+                               int64_t *nnz) {
+  if (!dir || !nnz || nparts < 1 || cap < 0 || (cap > 0 && (!rows || !cols)))
+    return IGA_ERR_INVALID_ARGUMENT;
+  const std::string base(dir);
+  auto path = [&](const std::string &name) { return base + "/" + name; };
+  std::vector<int> r, c;
+  for (int k = 0; k < nparts; ++k) {
+    const std::string fn = path("coordinate_nonzero_" + std::to_string(k) + ".txt");
+    int64_t n = 0;
+    int rc = iga_read_nonzero_coordinate(fn.c_str(), 0, nullptr, nullptr, &n);
+    if (rc != IGA_OK && rc != IGA_ERR_SIZE_LIMIT) return rc;
+    const size_t at = r.size();
+    r.resize(at + n);
+    c.resize(at + n);
+    rc = iga_read_nonzero_coordinate(fn.c_str(), n, r.data() + at, c.data() + at, &n);
+    if (rc) return rc;
+  }
+  // new -> old numbering: the partitions' map; a serial preprocessing writes
+  // none (its numbering is the natural one), so a missing file is the identity
+  std::vector<int> map;
+  const std::string mfn = path("new_to_old_mapping.txt");
+  if (std::FILE *f = std::fopen(mfn.c_str(), "r")) {
+    std::fclose(f);
+    int64_t m = 0;
+    int rc = iga_read_new_to_old(mfn.c_str(), 0, nullptr, &m);
+    if (rc != IGA_OK && rc != IGA_ERR_SIZE_LIMIT) return rc;
+    map.resize(m);
+    rc = iga_read_new_to_old(mfn.c_str(), m, map.data(), &m);
+    if (rc) return rc;
+  }
+  const int64_t total = (int64_t)r.size();
+  std::vector<int64_t> key(total);
+  const int64_t N = (int64_t)1 << 31;
+  for (int64_t k = 0; k < total; ++k) {
+    int a = r[k], b = c[k];
+    if (!map.empty()) {
+      if (a < 0 || b < 0 || (size_t)a >= map.size() || (size_t)b >= map.size())
+        return IGA_ERR_BAD_MESH;
+      a = map[a];
+      b = map[b];
+    }
+    if (a < 0 || b < 0) return IGA_ERR_BAD_MESH;
+    key[k] = (int64_t)a * N + b;
+  }
+  // lexicographic sort of the (row, col) pairs (duplicates kept, as std::sort
+  // of the pairs does)
+  std::sort(key.begin(), key.end());
+  *nnz = total;
+  if (total > cap) return IGA_ERR_SIZE_LIMIT;
+  for (int64_t k = 0; k < total; ++k) {
+    rows[k] = (int)(key[k] / N);
+    cols[k] = (int)(key[k] % N);
+  }
+  return IGA_OK;
 }
 
 }  // extern "C"

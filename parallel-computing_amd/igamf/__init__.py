@@ -82,6 +82,98 @@ class PreprocessInfo(C.Structure):
                 ("dim", C.c_int), ("base_name", C.c_char * 256)]
 
 
+class FemPartitionDesc(C.Structure):
+    _fields_ = [("nlocalfunc", C.c_int), ("nlocalelemx", C.c_int), ("nlocalelemy", C.c_int),
+                ("nid", C.c_int), ("ndir", C.c_int), ("id", C.c_void_p), ("dir", C.c_void_p),
+                ("ien", C.c_void_p), ("cp", C.c_void_p)]
+
+
+def _sized(call, what):
+    """Two-call size protocol of the file readers: query the count, then
+    read into arrays of that size."""
+    n = C.c_int64()
+    rc = call(0, None, None, C.byref(n))
+    if rc not in (0, 12):  # IGA_ERR_SIZE_LIMIT: the count is known
+        raise IgaError(rc, what)
+    return n.value
+
+
+def read_fem_partition(filename):
+    """A FEM partition file (part_fem_<r>.txt): dict of the header counts
+    and the ID, Dir, IEN, CP arrays."""
+    L = _lib()
+    h = VP()
+    check(L.iga_fem_read_partition(filename.encode(), C.byref(h)), "fem_read_partition")
+    try:
+        d = FemPartitionDesc()
+        check(L.iga_fem_partition_view(h, C.byref(d)), "fem_partition_view")
+
+        def arr(ptr, n, ct, dt):
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), (n,)).astype(dt).copy() \
+                if n else np.zeros(0, dt)
+        nel = d.nlocalelemx * d.nlocalelemy
+        return {"nlocalfunc": d.nlocalfunc, "nlocalelemx": d.nlocalelemx,
+                "nlocalelemy": d.nlocalelemy,
+                "ID": arr(d.id, d.nid, C.c_int, np.int32),
+                "Dir": arr(d.dir, d.ndir, C.c_int, np.int32),
+                "IEN": arr(d.ien, nel * 4, C.c_int, np.int32),
+                "CP": arr(d.cp, 2 * d.nid, C.c_double, np.float64)}
+    finally:
+        L.iga_fem_partition_destroy(h)
+
+
+def read_nonzero_coordinate(filename):
+    """(rows, cols) of a coordinate_nonzero_<r>.txt file."""
+    L = _lib()
+    fn = filename.encode()
+    n = _sized(lambda c, a, b, p: L.iga_read_nonzero_coordinate(fn, c, a, b, p),
+               "read_nonzero_coordinate")
+    rows = np.empty(n, np.int32)
+    cols = np.empty(n, np.int32)
+    m = C.c_int64()
+    check(L.iga_read_nonzero_coordinate(fn, n, rows.ctypes.data, cols.ctypes.data,
+                                        C.byref(m)), "read_nonzero_coordinate")
+    return rows, cols
+
+
+def write_nonzero_coordinate(filename, rows, cols):
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    assert rows.shape == cols.shape
+    check(_lib().iga_write_nonzero_coordinate(filename.encode(), len(rows), rows.ctypes.data,
+                                              cols.ctypes.data), "write_nonzero_coordinate")
+
+
+def read_new_to_old(filename):
+    L = _lib()
+    fn = filename.encode()
+    n = _sized(lambda c, a, b, p: L.iga_read_new_to_old(fn, c, a, p), "read_new_to_old")
+    out = np.empty(n, np.int32)
+    m = C.c_int64()
+    check(L.iga_read_new_to_old(fn, n, out.ctypes.data, C.byref(m)), "read_new_to_old")
+    return out
+
+
+def write_new_to_old(filename, m):
+    m = np.ascontiguousarray(m, np.int32)
+    check(_lib().iga_write_new_to_old(filename.encode(), len(m), m.ctypes.data),
+          "write_new_to_old")
+
+
+def fem_pattern_from_files(directory, nparts):
+    """The GPU driver's merged, renumbered and sorted FEM COO pattern."""
+    L = _lib()
+    d = directory.encode()
+    n = _sized(lambda c, a, b, p: L.iga_fem_pattern_from_files(d, nparts, c, a, b, p),
+               "fem_pattern_from_files")
+    rows = np.empty(n, np.int32)
+    cols = np.empty(n, np.int32)
+    m = C.c_int64()
+    check(L.iga_fem_pattern_from_files(d, nparts, n, rows.ctypes.data, cols.ctypes.data,
+                                       C.byref(m)), "fem_pattern_from_files")
+    return rows, cols
+
+
 def read_preprocess_info(filename):
     """The reference's preprocess info file as a dict (iga_read_preprocess_info)."""
     info = PreprocessInfo()
@@ -122,6 +214,16 @@ def _lib():
         L.iga_mesh_read_partition.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(VP)]
         L.iga_mesh_write_partition.argtypes = [VP, C.c_char_p]
         L.iga_read_preprocess_info.argtypes = [C.c_char_p, C.POINTER(PreprocessInfo)]
+        L.iga_fem_read_partition.argtypes = [C.c_char_p, C.POINTER(VP)]
+        L.iga_fem_partition_view.argtypes = [VP, C.POINTER(FemPartitionDesc)]
+        L.iga_fem_partition_destroy.argtypes = [VP]
+        L.iga_read_nonzero_coordinate.argtypes = [C.c_char_p, C.c_int64, VP, VP,
+                                                  C.POINTER(C.c_int64)]
+        L.iga_write_nonzero_coordinate.argtypes = [C.c_char_p, C.c_int64, VP, VP]
+        L.iga_read_new_to_old.argtypes = [C.c_char_p, C.c_int64, VP, C.POINTER(C.c_int64)]
+        L.iga_write_new_to_old.argtypes = [C.c_char_p, C.c_int64, VP]
+        L.iga_fem_pattern_from_files.argtypes = [C.c_char_p, C.c_int, C.c_int64, VP, VP,
+                                                 C.POINTER(C.c_int64)]
         L.iga_rng_normal.argtypes = [C.c_uint64, C.c_int64, VP]
         L.iga_rng_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, VP]
         L.iga_op_opts_default.argtypes = [C.POINTER(OpOpts)]

@@ -213,3 +213,95 @@ def test_binding_rejects_bad_vectors_before_any_device_call():
         s.solve_block(torch.zeros((2, 10), dtype=torch.float64))
     with pytest.raises(ValueError):
         igamf.check_vec(cpu64[::2], torch.float64, 5, op.device, "x")
+
+
+# --------------------------------------------------------------------------- FEM-side files
+def _ref_gpu_or_skip():
+    if not O.ref_available():
+        pytest.skip("reference CPU units not built (oracle/_ref)")
+    return O.ref_gpu()
+
+
+def _ref_fem_partition(R, fn, cap=1 << 20):
+    counts = np.zeros(7, np.int32)
+    ids = np.empty(cap, np.int32); dirs = np.empty(cap, np.int32)
+    ien = np.empty(cap, np.int32); cp = np.empty(cap, np.float64)
+    assert R.ref_fem_read_partition(fn.encode(), cap, counts, ids, dirs, ien, cp) == 0
+    nf, nx, ny, nid, nd, nien, ncp = (int(c) for c in counts)
+    return {"nlocalfunc": nf, "nlocalelemx": nx, "nlocalelemy": ny, "ID": ids[:nid],
+            "Dir": dirs[:nd], "IEN": ien[:nien], "CP": cp[:ncp]}
+
+
+@pytest.mark.parametrize("p,nex,ney,parts", [(3, 10, 12, 2), (4, 13, 9, 3)])
+def test_fem_partition_files_match_reference(tmp_path, p, nex, ney, parts):
+    """The reference's Partition::GeneratePartition writes part_fem_<r>.txt and
+    new_to_old_mapping.txt; our readers return what its FileManager reads.
+    The dry run's coordinate_nonzero_<r>.txt files (PETSc's MatGetRowIJ of
+    each rank's rows, which cannot be built here) are written with the
+    reference's FileManager from the oracle's pattern in partition numbering;
+    the driver's merge (renumber, sort) then gives back the oracle's pattern
+    bit for bit, and our writers write byte-identical files."""
+    R = _ref_gpu_or_skip()
+    d = str(tmp_path)
+    nf = np.zeros(2, np.int32)
+    assert R.ref_fem_write_partitions(d.encode(), p, p, nex, ney, parts, nf) == 0
+    nfx, nfy = int(nf[0]), int(nf[1])
+    nparts = parts * parts
+    owned = []
+    for r in range(nparts):
+        fn = os.path.join(d, f"part_fem_{r}.txt")
+        ours = igamf.read_fem_partition(fn)
+        ref = _ref_fem_partition(R, fn)
+        for k in ("nlocalfunc", "nlocalelemx", "nlocalelemy"):
+            assert ours[k] == ref[k], k
+        for k in ("ID", "Dir", "IEN"):
+            assert np.array_equal(ours[k], ref[k]), k
+        assert np.array_equal(ours["CP"], ref["CP"])
+        owned.append(ours["nlocalfunc"])
+    assert sum(owned) == nfx * nfy
+    mfn = os.path.join(d, "new_to_old_mapping.txt")
+    n2o = igamf.read_new_to_old(mfn)
+    ref_map = np.empty(nfx * nfy, np.int32)
+    assert R.ref_read_new_to_old(mfn.encode(), nfx * nfy, ref_map) == nfx * nfy
+    assert np.array_equal(n2o, ref_map)
+    assert np.array_equal(np.sort(n2o), np.arange(nfx * nfy))
+    # the dry run's per-rank pattern in partition numbering: rank r owns a
+    # contiguous block of new indices, each row's columns sorted
+    rows, cols, _, _, _ = O.fem_assemble(nfx, nfy, O.fem_nodes(p, p, nex, ney))
+    o2n = np.empty_like(n2o)
+    o2n[n2o] = np.arange(len(n2o), dtype=np.int32)
+    nr, nc = o2n[rows], o2n[cols]
+    order = np.lexsort((nc, nr))
+    nr, nc = nr[order], nc[order]
+    start = np.concatenate([[0], np.cumsum(owned)])
+    for r in range(nparts):
+        sel = (nr >= start[r]) & (nr < start[r + 1])
+        fn = os.path.join(d, f"coordinate_nonzero_{r}.txt")
+        R.ref_write_nonzero(fn.encode(), int(sel.sum()), np.ascontiguousarray(nr[sel]),
+                            np.ascontiguousarray(nc[sel]))
+        a, b = igamf.read_nonzero_coordinate(fn)
+        assert np.array_equal(a, nr[sel]) and np.array_equal(b, nc[sel])
+        ours_fn = os.path.join(d, f"ours_{r}.txt")
+        igamf.write_nonzero_coordinate(ours_fn, a, b)
+        assert open(ours_fn, "rb").read() == open(fn, "rb").read()
+    ours_map = os.path.join(d, "ours_map.txt")
+    igamf.write_new_to_old(ours_map, n2o)
+    assert open(ours_map, "rb").read() == open(mfn, "rb").read()
+    pr, pc = igamf.fem_pattern_from_files(d, nparts)
+    assert np.array_equal(pr, rows) and np.array_equal(pc, cols)
+
+
+def test_fem_file_readers_reject_bad_input(tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("nnz: 3\nrows: 1 2 3 \ncols: 1 2 \n")
+    with pytest.raises(igamf.IgaError) as e:
+        igamf.read_nonzero_coordinate(str(bad))
+    assert e.value.status == 5  # IGA_ERR_BAD_MESH
+    with pytest.raises(igamf.IgaError):
+        igamf.read_fem_partition(str(tmp_path / "missing.txt"))
+    short = tmp_path / "part.txt"
+    short.write_text("nlocalfunc: 4\nnlocalelemx: 1\nnlocalelemy: 1\nID\n0 1 2 3 \nDir\n\n"
+                     "IEN\n0 1 2 9 \nCP\n0 0 1 0 0 1 1 1 \n")
+    with pytest.raises(igamf.IgaError) as e:  # IEN entry outside the partition
+        igamf.read_fem_partition(str(short))
+    assert e.value.status == 5
