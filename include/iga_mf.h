@@ -103,6 +103,14 @@ int iga_mesh_build(const iga_mesh_params *params, iga_mesh **out);
  * iga_mesh_destroy). */
 int iga_mesh_describe(const iga_mesh *m, iga_mesh_desc *desc);
 int iga_mesh_destroy(iga_mesh *m);
+/* The patch's geometry map at n parametric points uv[n][2] (host), xy[n][2]
+ * out: x(u,v) = sum_A w_A N_A(u,v) P_A / sum_A w_A N_A(u,v), the B-splines
+ * from the mesh's Bezier extraction operators, the parameter domain
+ * [0, sum h1] x [0, sum h2].  No counterpart in the reference, whose domains
+ * are rectangles; it maps the reference's FEM-preconditioner node grid
+ * (iga_fem_reference_nodes on [0,1]^2) onto curved patches such as the
+ * quarter annulus.  IGA_ERR_INVALID_ARGUMENT for a point outside the domain. */
+int iga_mesh_eval(const iga_mesh *m, int64_t n, const double *uv, double *xy);
 
 /* ---- the reference's on-disk partition format --------------------------
  * A partition file of FileManager::WritePartition
@@ -389,12 +397,23 @@ typedef enum {
                             matrix (a linear preconditioner; interval
                             [lmax/30, lmax], lmax the Gershgorin bound of
                             D^-1 K); no counterpart in the reference */
-  IGA_PC_FEM_ILU = 4     /* ILU(0) of the Q1 matrix in natural ordering (the
+  IGA_PC_FEM_ILU = 4,    /* ILU(0) of the Q1 matrix in natural ordering (the
                             -pc_type ilu of the reference's plain-method runs,
                             PETSc PCILU with 0 levels), factored on the host
                             at set-up; triangular solves level-scheduled in
                             one workgroup, so meant for small meshes (tens of
                             thousands of functions) */
+  IGA_PC_FEM_MG = 5      /* one V-cycle of geometric multigrid on the Q1
+                            matrix: the approximation of K^-1 that the
+                            reference's inner CG computes, at a cost linear in
+                            the grid (structured 9-point stencils, Galerkin
+                            coarse grids, Chebyshev-Jacobi smoothing of degree
+                            2, coarsest grid solved exactly); a symmetric
+                            linear preconditioner, so the outer CG keeps the
+                            standard beta.  The Q1 matrix's Dirichlet set must
+                            be the boundary ring (as iga_fem_assemble builds
+                            it); FP64 V-cycle in the FP64 solve, FP32 in the
+                            FP32 inner solve */
 } iga_pc_kind;
 
 typedef enum {
@@ -465,6 +484,14 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
  * IGA_OK or the status of the first column that failed. */
 int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double *U,
                            void *stream, iga_solve_stats *stats);
+/* z = M^-1 r: one application of the solver's FEM preconditioner, the
+ * PCSHELL apply callback MyPCApply(PC, Vec, Vec) of the reference
+ * (matrix_free_gpu/matrixfreegpu.cu:18-24, 273-289), for callers running
+ * their own Krylov method.  dt: IGA_F64 for a PCG_F64 solver, IGA_F32 for a
+ * PCG_F32_REFINE one (the precision its inner solve uses); r, z nfunc device
+ * values, not aliased.  Asynchronous on `stream`; must not overlap a solve
+ * of the same solver.  IGA_ERR_UNSUPPORTED for Jacobi and none. */
+int iga_solver_apply_pc(iga_solver *s, int dt, const void *r, void *z, void *stream);
 int iga_solver_destroy(iga_solver *s);
 
 /* ---- device vector helpers (FP64 / FP32, used by drivers and tests) ----- */

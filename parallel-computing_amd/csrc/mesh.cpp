@@ -278,6 +278,65 @@ int iga_mesh_destroy(iga_mesh *m) {
   return IGA_OK;
 }
 
+int iga_mesh_eval(const iga_mesh *m, int64_t n, const double *uv, double *xy) {
+  if (!m || n < 0 || (n > 0 && (!uv || !xy))) return IGA_ERR_INVALID_ARGUMENT;
+  const int p = m->p, q = m->q, nfx = m->nex + p;
+  // element boundaries along each direction: partial sums of the sizes
+  std::vector<double> ex(m->nex + 1, 0.0), ey(m->ney + 1, 0.0);
+  for (int i = 0; i < m->nex; ++i) ex[i + 1] = ex[i] + m->h1[i];
+  for (int j = 0; j < m->ney; ++j) ey[j + 1] = ey[j] + m->h2[j];
+  // B-splines of one direction at parameter u: element e and the p+1
+  // functions e..e+p, N_a = sum_k C_e[a][k] B_k(t) with the Bernstein
+  // polynomials B_k of the local coordinate t (the extraction identity)
+  auto basis = [](const std::vector<double> &edge, const std::vector<double> &C, int deg,
+                  double u, int &e, double *N) {
+    const int ne = (int)edge.size() - 1;
+    // the summed element sizes may end a few ulps short of the nominal end
+    const double tol = 1e-12 * (edge[ne] - edge[0]);
+    if (!(u >= edge[0] - tol && u <= edge[ne] + tol)) return false;
+    u = std::min(std::max(u, edge[0]), edge[ne]);
+    e = (int)(std::upper_bound(edge.begin(), edge.end(), u) - edge.begin()) - 1;
+    if (e >= ne) e = ne - 1;
+    const double t = (u - edge[e]) / (edge[e + 1] - edge[e]);
+    double B[16];
+    B[0] = 1.0;  // de Casteljau-style degree raising of the Bernstein basis
+    for (int k = 1; k <= deg; ++k) {
+      double prev = 0.0;
+      for (int i = 0; i < k; ++i) {
+        const double b = B[i];
+        B[i] = prev + (1.0 - t) * b;
+        prev = t * b;
+      }
+      B[k] = prev;
+    }
+    const double *Ce = C.data() + (size_t)e * (deg + 1) * (deg + 1);
+    for (int a = 0; a <= deg; ++a) {
+      double s = 0.0;
+      for (int k = 0; k <= deg; ++k) s += Ce[a * (deg + 1) + k] * B[k];
+      N[a] = s;
+    }
+    return true;
+  };
+  for (int64_t k = 0; k < n; ++k) {
+    double Nx[16], Ny[16];
+    int e1 = 0, e2 = 0;
+    if (!basis(ex, m->C1, p, uv[2 * k], e1, Nx) || !basis(ey, m->C2, q, uv[2 * k + 1], e2, Ny))
+      return IGA_ERR_INVALID_ARGUMENT;
+    double X = 0.0, Y = 0.0, W = 0.0;
+    for (int b = 0; b <= q; ++b)
+      for (int a = 0; a <= p; ++a) {
+        const size_t A = (size_t)(e2 + b) * nfx + (e1 + a);
+        const double w = Nx[a] * Ny[b] * (m->has_w ? m->w[A] : 1.0);
+        X += w * m->cp[2 * A];
+        Y += w * m->cp[2 * A + 1];
+        W += w;
+      }
+    xy[2 * k] = X / W;
+    xy[2 * k + 1] = Y / W;
+  }
+  return IGA_OK;
+}
+
 }  // extern "C"
 
 namespace {

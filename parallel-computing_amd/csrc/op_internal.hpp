@@ -64,6 +64,19 @@ struct iga_op {
 };
 
 namespace igai {
+// geometric multigrid on the Q1 FEM matrix (mg.hip): rowptr / col / val a
+// host copy of the nx x ny grid's CSR in natural numbering
+struct MgHier;
+int mg_create(const int *rowptr, const int *col, const double *val, int nx, int ny, bool want64,
+              bool want32, MgHier **out);
+void mg_destroy(MgHier *h);
+int64_t mg_bytes(const MgHier *h);
+int mg_levels(const MgHier *h);
+// z = one V-cycle applied to r (an approximation of K^-1 r); every kernel
+// returns at once while *s1 or *s2 is set
+template <typename V>
+int mg_apply(MgHier *h, const V *r, V *z, hipStream_t st, const int *s1, const int *s2);
+
 int dev_alloc(iga_op *op, void **p, size_t bytes);
 int set_device(int dev);
 // CG fusion of an apply (see igak::ApplyArgs): per-tile partials of x.y into

@@ -1057,6 +1057,7 @@ struct iga_solver {
   double *rin = nullptr, *w = nullptr, *pp = nullptr, *q = nullptr;
   float *rin32 = nullptr, *w32 = nullptr, *pp32 = nullptr, *q32 = nullptr;
   double cheb_lmax = 0.0;  // Gershgorin bound of D^-1 K (FEM_CHEB)
+  igai::MgHier *mg = nullptr;  // multigrid hierarchy of the Q1 matrix (FEM_MG)
   // ILU(0) factors of K (FEM_ILU): values in K's pattern, diagonal
   // positions, level sets of the two triangular solves
   double *ilu64 = nullptr;
@@ -1166,6 +1167,8 @@ int cheb_pc(iga_solver *s, const V *r, V *z, V *d, V *q, const V *val, hipStream
 
 template <typename V>
 int apply_pc(iga_solver *s, bool f64, const V *r, V *z, hipStream_t st) {
+  if (s->o.pc == IGA_PC_FEM_MG)
+    return igai::mg_apply<V>(s->mg, r, z, st, s->ic + I_DONE, nullptr);
   if (s->o.pc == IGA_PC_FEM_ILU) {
     if (f64)
       hipLaunchKernelGGL((k_ilu_solve<double>), dim3(1), dim3(1024), 0, st, s->rowptr, s->col,
@@ -1200,6 +1203,9 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
   const bool loc = local_pc(s);
+  // preconditioners that are not a fixed linear operator (the inner CG) or
+  // not symmetric (ILU) take the flexible (Polak-Ribiere) beta
+  const bool flex = !loc && s->o.pc != IGA_PC_FEM_MG;
   const int dt = apply_dt(s, f64);
   // p.Ap accumulated by the apply kernel itself when it can, else k_cg_pap
   const bool fdot = s->o.fused == 1 && s->tdot && igai::cg_fusable_dt(s->op, dt, ebe);
@@ -1226,18 +1232,26 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   else if (loc)
     LAUNCH((k_cg_update<V, V, true, false, TU>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n, s->part,
            s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
-  else if (fseam)
+  else if (fseam && flex)
     LAUNCH((k_cg_update<V, V, false, true, TU, true>), nb, u, r, p, Ap, z, rold, (const V *)nullptr,
            0, n, s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
-  else
+  else if (flex)
     LAUNCH((k_cg_update<V, V, false, true, TU>), nb, u, r, p, Ap, z, rold, (const V *)nullptr, 0, n,
            s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
+  else if (fseam)
+    LAUNCH((k_cg_update<V, V, false, false, TU, true>), nb, u, r, p, Ap, z, (V *)nullptr,
+           (const V *)nullptr, 0, n, s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0,
+           tdot, ntiles, sg);
+  else
+    LAUNCH((k_cg_update<V, V, false, false, TU>), nb, u, r, p, Ap, z, (V *)nullptr,
+           (const V *)nullptr, 0, n, s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0,
+           tdot, ntiles, sg);
   if (!loc) {
     rc = apply_pc<V>(s, f64, r, z, st);
     if (rc) return rc;
-    LAUNCH((k_cg_rz<V>), nb, r, z, rold, n, s->part, s->ic);
+    LAUNCH((k_cg_rz<V>), nb, r, z, flex ? rold : (const V *)nullptr, n, s->part, s->ic);
   }
-  LAUNCH((k_cg_pupdate<V>), nb, p, z, n, s->part, s->sc, s->ic, nb, loc ? 0 : 1,
+  LAUNCH((k_cg_pupdate<V>), nb, p, z, n, s->part, s->sc, s->ic, nb, flex ? 1 : 0,
          (const int *)nullptr, (int64_t)0, gather ? (V *)nullptr : Ap);
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
@@ -1572,6 +1586,7 @@ int iga_solver_destroy(iga_solver *s) {
   for (hipEvent_t e : s->fev)
     if (e) (void)hipEventDestroy(e);
   if (s->hflag) (void)hipHostFree(s->hflag);
+  igai::mg_destroy(s->mg);
   for (void *p : s->allocs) (void)hipFree(p);
   for (void *p : s->ballocs) (void)hipFree(p);
   delete s;
@@ -1586,10 +1601,10 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   if (o_in) o = *o_in; else iga_solver_opts_default(&o);
   if (o.kind != IGA_SOLVE_PCG_F64 && o.kind != IGA_SOLVE_PCG_F32_REFINE) return IGA_ERR_INVALID_ARGUMENT;
   if (o.pc != IGA_PC_NONE && o.pc != IGA_PC_JACOBI && o.pc != IGA_PC_FEM_CG &&
-      o.pc != IGA_PC_FEM_CHEB && o.pc != IGA_PC_FEM_ILU)
+      o.pc != IGA_PC_FEM_CHEB && o.pc != IGA_PC_FEM_ILU && o.pc != IGA_PC_FEM_MG)
     return IGA_ERR_INVALID_ARGUMENT;
-  const bool fem_pc_kind =
-      o.pc == IGA_PC_FEM_CG || o.pc == IGA_PC_FEM_CHEB || o.pc == IGA_PC_FEM_ILU;
+  const bool fem_pc_kind = o.pc == IGA_PC_FEM_CG || o.pc == IGA_PC_FEM_CHEB ||
+                           o.pc == IGA_PC_FEM_ILU || o.pc == IGA_PC_FEM_MG;
   if (fem_pc_kind && !fem) return IGA_ERR_INVALID_ARGUMENT;
   if (o.pc == IGA_PC_FEM_CHEB && o.pc_cheb_degree < 1) return IGA_ERR_INVALID_ARGUMENT;
   if (o.kind == IGA_SOLVE_PCG_F32_REFINE && !op->G32) return IGA_ERR_UNSUPPORTED;
@@ -1660,6 +1675,24 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     s->rowptr = v.rowptr;
     s->col = v.col;
     s->val64 = v.val;
+    if (o.pc == IGA_PC_FEM_MG) {
+      // the hierarchy is built on the host from a copy of the CSR
+      std::vector<int> hrp(v.nrows + 1), hci(v.nnz);
+      std::vector<double> hv(v.nnz);
+      if (hipMemcpy(hrp.data(), v.rowptr, (v.nrows + 1) * 4, hipMemcpyDeviceToHost) !=
+              hipSuccess ||
+          hipMemcpy(hci.data(), v.col, v.nnz * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(hv.data(), v.val, v.nnz * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(IGA_ERR_HIP);
+      const bool f64 = o.kind == IGA_SOLVE_PCG_F64;
+      rc = igai::mg_create(hrp.data(), hci.data(), hv.data(), op->nfx, op->nfy, f64, !f64,
+                           &s->mg);
+      if (rc) return fail(rc);
+    }
+  }
+  if (fem_pc_kind && o.pc != IGA_PC_FEM_MG) {
+    iga_csr_view v;
+    if (iga_fem_view(fem, &v) != IGA_OK) return fail(IGA_ERR_INVALID_ARGUMENT);
     SA(s->fdinv, n * 8);
     SA(s->rin, n * 8);
     SA(s->w, n * 8);
@@ -1725,6 +1758,23 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   if (hipDeviceSynchronize() != hipSuccess) return fail(IGA_ERR_HIP);
   *out = s;
   return IGA_OK;
+}
+
+int iga_solver_apply_pc(iga_solver *s, int dt, const void *r, void *z, void *stream) {
+  if (!s || !r || !z || r == z || (dt != IGA_F64 && dt != IGA_F32))
+    return IGA_ERR_INVALID_ARGUMENT;
+  if (local_pc(s)) return IGA_ERR_UNSUPPORTED;
+  const bool f64 = dt == IGA_F64;
+  // the FP64 solve keeps the FP64 preconditioner, the FP32 inner solve the FP32 one
+  if (f64 != (s->o.kind == IGA_SOLVE_PCG_F64)) return IGA_ERR_UNSUPPORTED;
+  int rc = igai::set_device(s->device);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  // the preconditioner kernels return at once while the solver's done flag
+  // is set (it stays set after a solve): clear it
+  hipLaunchKernelGGL(k_ru_clear, dim3(1), dim3(64), 0, st, s->ic);
+  if (f64) return apply_pc<double>(s, true, (const double *)r, (double *)z, st);
+  return apply_pc<float>(s, false, (const float *)r, (float *)z, st);
 }
 
 int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,

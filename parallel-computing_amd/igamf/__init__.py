@@ -21,7 +21,7 @@ F64, F32, F32_G16 = 0, 1, 2  # F32_G16: FP32 arithmetic on FP16-stored G_q
 SF, EBE = 0, 1
 IEN_ATOMIC, INVLM_GATHER = 0, 1
 GEOM_UNIT_SQUARE, GEOM_QUARTER_ANNULUS, GEOM_PERTURBED_SQUARE = 0, 1, 2
-PC_NONE, PC_JACOBI, PC_FEM_CG, PC_FEM_CHEB, PC_FEM_ILU = 0, 1, 2, 3, 4
+PC_NONE, PC_JACOBI, PC_FEM_CG, PC_FEM_CHEB, PC_FEM_ILU, PC_FEM_MG = 0, 1, 2, 3, 4, 5
 SOLVE_PCG_F64, SOLVE_PCG_F32_REFINE = 0, 1
 LOAD_REF_POLY, LOAD_CONST, LOAD_SINSIN = 0, 1, 2
 IGA_ERR_INVALID_ARGUMENT, IGA_ERR_UNSUPPORTED = 1, 6
@@ -211,6 +211,7 @@ def _lib():
         L.iga_mesh_build.argtypes = [C.POINTER(MeshParams), C.POINTER(VP)]
         L.iga_mesh_describe.argtypes = [VP, C.POINTER(MeshDesc)]
         L.iga_mesh_destroy.argtypes = [VP]
+        L.iga_mesh_eval.argtypes = [VP, C.c_int64, VP, VP]
         L.iga_mesh_read_partition.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(VP)]
         L.iga_mesh_write_partition.argtypes = [VP, C.c_char_p]
         L.iga_read_preprocess_info.argtypes = [C.c_char_p, C.POINTER(PreprocessInfo)]
@@ -261,6 +262,7 @@ def _lib():
         L.iga_solver_solve_block.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP,
                                              C.POINTER(SolveStats)]
         L.iga_solver_destroy.argtypes = [VP]
+        L.iga_solver_apply_pc.argtypes = [VP, C.c_int, VP, VP, VP]
         L.iga_dot_f64.argtypes = [VP, VP, C.c_int64, C.POINTER(C.c_double), VP]
         L.iga_dot_f64_noalloc.argtypes = [VP, VP, C.c_int64, VP, VP]
         _LIB = L
@@ -364,6 +366,25 @@ class Mesh:
             raise ValueError("write_partition needs a library-owned mesh")
         check(_lib().iga_mesh_write_partition(self._h, os.fsencode(filename)),
               "mesh_write_partition")
+
+    def eval(self, uv):
+        """Geometry map at parametric points uv [n, 2] -> [n, 2]
+        (iga_mesh_eval; library-owned meshes)."""
+        if self._h is None:
+            raise ValueError("eval needs a library-owned mesh")
+        uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+        xy = np.empty_like(uv)
+        check(_lib().iga_mesh_eval(self._h, len(uv), uv.ctypes.data, xy.ctypes.data),
+              "mesh_eval")
+        return xy
+
+    def fem_nodes(self):
+        """The reference's FEM-preconditioner node grid (iga_fem_reference_nodes
+        on the unit parameter square) mapped through the geometry: on the
+        unit square it is the reference's grid itself."""
+        d = self.desc
+        uv = fem_reference_nodes(d.p, d.q, d.nelem_x, d.nelem_y)
+        return self.eval(uv).reshape(-1)
 
     @classmethod
     def from_arrays(cls, p, q, nex, ney, CP, W, IEN, ID, Dir, C1, C2, h1, h2, kappa=None):
@@ -704,6 +725,22 @@ class Solver:
         if rc != 0 and (check_status or rc not in (IGA_ERR_NOT_CONVERGED, IGA_ERR_DIVERGED)):
             raise IgaError(rc, "solver_solve")
         return u, st
+
+    def apply_pc(self, r, z=None, stream=None):
+        """z = M^-1 r, one application of the FEM preconditioner
+        (iga_solver_apply_pc): float64 for an FP64 solver, float32 for an
+        FP32-refinement one."""
+        import torch
+        dt = {torch.float64: F64, torch.float32: F32}.get(r.dtype)
+        if dt is None:
+            raise ValueError("r must be float64 or float32")
+        check_vec(r, r.dtype, self.op.nfunc, self.op.device, "r")
+        if z is None:
+            z = torch.empty_like(r)
+        check_vec(z, r.dtype, self.op.nfunc, self.op.device, "z")
+        check(_lib().iga_solver_apply_pc(self._h, dt, r.data_ptr(), z.data_ptr(),
+                                         _stream_ptr(stream)), "solver_apply_pc")
+        return z
 
     def solve_block(self, F, U=None, stream=None, check_status=True):
         """Solve A U[j] = F[j] for the k rows of F (FP64 [k, nfunc], rows

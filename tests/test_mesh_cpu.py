@@ -305,3 +305,28 @@ def test_fem_file_readers_reject_bad_input(tmp_path):
     with pytest.raises(igamf.IgaError) as e:  # IEN entry outside the partition
         igamf.read_fem_partition(str(short))
     assert e.value.status == 5
+
+
+def test_mesh_eval_reproduces_the_geometry():
+    """iga_mesh_eval: the identity on the unit square (the Greville control
+    points reproduce x(u) = u), exact circles on the quarter annulus
+    (|x| = r0 + u (r1 - r0), the angle increasing from 0 to pi/2), the
+    control points' own interpolation at the corners; outside: an error."""
+    rng = np.random.default_rng(0)
+    uv = rng.random((500, 2))
+    uv[:4] = [[0, 0], [1, 0], [0, 1], [1, 1]]
+    sq = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 11, 7)
+    assert np.abs(sq.eval(uv) - uv).max() < 1e-14
+    an = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 9, 13, r0=1.0, r1=2.0)
+    xy = an.eval(uv)
+    assert np.abs(np.hypot(xy[:, 0], xy[:, 1]) - (1.0 + uv[:, 0])).max() < 1e-13
+    th = np.arctan2(xy[:, 1], xy[:, 0])
+    order = np.argsort(uv[:, 1])
+    assert np.all(np.diff(th[order]) >= -1e-15)
+    assert np.abs(xy[:4] - [[1, 0], [2, 0], [0, 1], [0, 2]]).max() < 1e-14
+    # the FEM node grid of the reference, mapped: nodes inside the annulus
+    nodes = an.fem_nodes().reshape(-1, 2)
+    r = np.hypot(nodes[:, 0], nodes[:, 1])
+    assert r.min() >= 1 - 1e-14 and r.max() <= 2 + 1e-14
+    with pytest.raises(igamf.IgaError):
+        an.eval([[1.5, 0.5]])

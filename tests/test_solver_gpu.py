@@ -391,3 +391,90 @@ def test_refine_solve_repeatable_on_full_vector_grid(fused):
         assert torch.equal(u, u0) and st.iterations == st0.iterations
     d = ((u0 - u64).abs().max() / u64.abs().max()).item()
     assert d < 1e-7, d
+
+
+# --------------------------------------------------------------------------- multigrid
+def _fem_csr(nfx, nfy, nodes):
+    import scipy.sparse as sp
+    rows, cols, _, _, val = O.fem_assemble(nfx, nfy, nodes)
+    return sp.csr_matrix((val, (rows, cols)), shape=(nfx * nfy, nfx * nfy))
+
+
+def _annulus_with_fem(p, n, ney=None):
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, p, n, ney)
+    d = m.desc
+    return m, d.nelem_x + p, d.nelem_y + p, m.fem_nodes()
+
+
+@pytest.mark.parametrize("geom", ["square", "annulus", "rect"])
+def test_mg_vcycle_matches_cpu_model(geom):
+    """One V-cycle of IGA_PC_FEM_MG (FP64 and FP32) against the scipy model
+    of the same cycle, and symmetric: u.M^-1 v = v.M^-1 u.  The rectangle
+    (37 x 150 nodes) coarsens its long direction alone at the end."""
+    import mg_model
+    p = 3
+    if geom == "square":
+        m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, 61)
+        nfx = nfy = 64
+        nodes = igamf.fem_reference_nodes(p, p, 61, 61)
+    elif geom == "annulus":
+        m, nfx, nfy, nodes = _annulus_with_fem(p, 70)
+    else:
+        m, nfx, nfy, nodes = _annulus_with_fem(p, 34, 147)
+    op = igamf.Operator(m)
+    fem = igamf.FemPC(nfx, nfy, nodes)
+    model = mg_model.VCycle(_fem_csr(nfx, nfy, nodes), nfx, nfy)
+    interior = m.arrays()["ID"] >= 0
+    r = igamf.rng_normal(3, op.nfunc) * interior
+    v = igamf.rng_normal(4, op.nfunc) * interior
+    zref = model.apply(r)
+    for kind, dt, tol in ((igamf.SOLVE_PCG_F64, torch.float64, 1e-11),
+                          (igamf.SOLVE_PCG_F32_REFINE, torch.float32, 2e-5)):
+        s = igamf.Solver(op, fem, kind=kind, pc=igamf.PC_FEM_MG)
+        rt = torch.from_numpy(r).cuda().to(dt)
+        vt = torch.from_numpy(v).cuda().to(dt)
+        z = s.apply_pc(rt).double().cpu().numpy()
+        err = np.abs(z - zref).max() / np.abs(zref).max()
+        assert err < tol, (geom, dt, err)
+        assert np.all(z[~interior] == 0.0)
+        if dt == torch.float64:
+            zv = s.apply_pc(vt).cpu().numpy()
+            a, b = v @ z, r @ zv
+            assert abs(a - b) <= 1e-12 * abs(a), (a, b)
+            assert r @ z < 0  # K and so M^-1 are negative definite on the interior
+
+
+@pytest.mark.parametrize("geom", ["square", "annulus"])
+def test_mg_preconditioned_solve(geom):
+    """PCG with one V-cycle on the Q1 matrix: FP64 and FP32 + FP64 reliable
+    updates converge to the Jacobi-PCG solution, in an outer iteration count
+    that stays flat under refinement (the exact K^-1's) while Jacobi's
+    doubles."""
+    p = 3
+    its = {}
+    for n in (48, 96):
+        if geom == "square":
+            m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, n)
+            nodes = igamf.fem_reference_nodes(p, p, n, n)
+            kind, val = igamf.LOAD_REF_POLY, 0.0
+        else:
+            m, _, _, nodes = _annulus_with_fem(p, n)
+            kind, val = igamf.LOAD_CONST, 1.0
+        op = igamf.Operator(m)
+        fem = igamf.FemPC(n + p, n + p, nodes)
+        F = op.load(kind, val)
+        uj, stj = igamf.Solver(op, None, pc=igamf.PC_JACOBI, rtol=1e-10).solve(F)
+        its[("jacobi", n)] = stj.iterations
+        for sk in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+            s = igamf.Solver(op, fem, kind=sk, pc=igamf.PC_FEM_MG, rtol=1e-10)
+            u, st = s.solve(F)
+            assert st.status == 0 and st.rel_residual <= 1.0001e-10, (sk, st.rel_residual)
+            d = ((u - uj).abs().max() / uj.abs().max()).item()
+            assert d < 1e-8, (geom, sk, d)
+            its[(sk, n)] = st.iterations
+            u2, st2 = s.solve(F)  # graph replay: bitwise
+            assert torch.equal(u, u2) and st2.iterations == st.iterations
+    assert its[("jacobi", 96)] > 1.6 * its[("jacobi", 48)], its
+    for sk in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        assert its[(sk, 96)] <= its[(sk, 48)] + 4, its
+        assert its[(sk, 96)] < 0.5 * its[("jacobi", 96)], its
