@@ -18,7 +18,11 @@ assembly).  Alongside, in the same JSON line:
             per-function weights U[0.8, 1.2]: FP32 SF and FP64 EBE applies and
             the GPU FEM-preconditioner assembly, its indices compared bit for
             bit with the CPU's;
+  solve_fem_mg  the same solve preconditioned by the reference's Q1 FEM
+            matrix, applied as one multigrid V-cycle;
   c1        the plain method (p = 2, 32^2, FP64 EBE + FEM preconditioner);
+  fempc_iterations  the FEM-preconditioned outer iteration counts at p = 3,
+            32^2 / 64^2 / 128^2 beside the reference's logged ones;
   c4        the kappa adjoint (p = 3, 256^2, Fourier kappa of seed 1, u_obs
             of seed 2 plus noise) with the gradient against central
             differences along 16 directions;
@@ -318,7 +322,7 @@ def solves(out, op, only=(), reps=3):
         res[f"speedup_best_{name}_vs_classic_fp64"] = round(
             classic64 / res[best[name]]["solve_ms"], 3)
     res["preconditioner"] = "jacobi"
-    res["inner_rtol"] = "library default (0.1)"
+    res["inner_rtol"] = "library default (automatic: 0.1 with Jacobi)"
     out["solve"] = res
 
 
@@ -349,6 +353,103 @@ def c3_qp_values(p, ne, j):
     S = np.sin(np.pi * np.outer(xi, np.arange(1, 9)))
     T = S @ c3_coeffs(j) @ S.T
     return T.reshape(ne, nq, ne, nq).transpose(2, 0, 3, 1).reshape(-1)
+
+
+def solves_mg(out, mesh, op, reps=3):
+    """The same C2 solve (f = 1, rtol 1e-8) with the reference's
+    preconditioner, the Q1 FEM matrix on the reference's node grid mapped
+    onto the annulus, applied as one multigrid V-cycle (IGA_PC_FEM_MG): plain
+    FP64 PCG against FP32 PCG + FP64 reliable updates, both assemblies, p.Ap
+    by its own kernel (fused 0) or inside the apply (fused 1)."""
+    import torch
+    import igamf
+    t0 = time.perf_counter()
+    nodes = mesh.fem_nodes()
+    fem = igamf.FemPC(op.info.nfx, op.info.nfy, nodes)
+    torch.cuda.synchronize()
+    t_fem = time.perf_counter() - t0
+    F = op.load(igamf.LOAD_CONST, 1.0)
+    res, sols = {}, {}
+    for an, av in (("gather", igamf.INVLM_GATHER), ("atomic", igamf.IEN_ATOMIC)):
+        for fused in (0, 1):
+            for name, kind in (("fp64", igamf.SOLVE_PCG_F64),
+                               ("fp32_refine", igamf.SOLVE_PCG_F32_REFINE)):
+                key = f"{name}_{an}_fused{fused}"
+                t0 = time.perf_counter()
+                s = igamf.Solver(op, fem, kind=kind, pc=igamf.PC_FEM_MG, rtol=1e-8,
+                                 assembly=av, maxit=200000, fused=fused)
+                t_setup = time.perf_counter() - t0
+                s.solve(F)  # captures the iteration graph
+                ts = []
+                for _ in range(reps):
+                    u, st = s.solve(F)
+                    ts.append(st.solve_ms)
+                torch.cuda.synchronize()
+                res[key] = {"solve_ms": round(median(ts), 3), "iterations": st.iterations,
+                            "reliable_updates": st.outer_iterations,
+                            "us_per_iteration": round(median(ts) * 1e3 / st.iterations, 1),
+                            "rel_residual": st.rel_residual, "status": st.status,
+                            "setup_s": round(t_setup, 3)}
+                sols[key] = u
+                log(f"solve mg {key}: {median(ts):.2f} ms, {st.iterations} its")
+                del s
+    ref = sols["fp64_gather_fused0"]
+    for k in sols:
+        if k != "fp64_gather_fused0":
+            res[k]["solution_rel_diff"] = (sols[k] - ref).norm().item() / ref.norm().item()
+    best = {}
+    for name in ("fp64", "fp32_refine"):
+        best[name] = min((k for k in res if k.startswith(name + "_")),
+                         key=lambda k: res[k]["solve_ms"])
+        res[f"best_{name}"] = best[name]
+    res["speedup_best_fp32_refine_vs_best_fp64"] = round(
+        res[best["fp64"]]["solve_ms"] / res[best["fp32_refine"]]["solve_ms"], 3)
+    res["preconditioner"] = ("fem_mg: Q1 FEM matrix on the reference's node grid mapped onto the "
+                             "annulus, one V-cycle (Chebyshev-Jacobi degree 2, Galerkin)")
+    res["inner_rtol"] = "library default (automatic: 1e-4 with a FEM preconditioner)"
+    res["fem_assembly_s"] = round(t_fem, 3)
+    if "solve" in out and "best_fp64" in out["solve"]:
+        jb = out["solve"]
+        res["speedup_vs_best_jacobi_fp64"] = round(
+            jb[jb["best_fp64"]]["solve_ms"] / res[best["fp32_refine"]]["solve_ms"], 2)
+        res["speedup_fp64_vs_jacobi_fp64"] = round(
+            jb[jb["best_fp64"]]["solve_ms"] / res[best["fp64"]]["solve_ms"], 2)
+    out["solve_fem_mg"] = res
+
+
+def fempc_iterations(out):
+    """Outer iteration counts of the FEM-preconditioned solve on the unit
+    square, p = 3, f = x(1-x)y(1-y), rtol 1e-10, beside the reference's logged
+    counts (matrix_free/data/fempc_mf.txt:12,24,36: 35, 33, 28; its outer KSP
+    type is not recorded): the reference's preconditioner (inner CG + Jacobi
+    on the Q1 matrix to 1e-10, flexible outer CG) and one V-cycle."""
+    import igamf
+    ref = {32: 35, 64: 33, 128: 28}
+    rows = {}
+    for n in (32, 64, 128):
+        p = 3
+        mesh = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, n)
+        op = igamf.Operator(mesh)
+        fem = igamf.FemPC(n + p, n + p, igamf.fem_reference_nodes(p, p, n, n))
+        F = op.load(igamf.LOAD_REF_POLY, 0.0)
+        row = {"reference_logged": ref[n]}
+        # the inner CG is unrolled into pc_inner_maxit iterations per outer
+        # one (each returning at once after convergence): launched eagerly,
+        # not captured, at this length
+        for name, kw in (("inner_cg_1e-10", dict(pc=igamf.PC_FEM_CG, pc_inner_rtol=1e-10,
+                                                pc_inner_maxit=1500, use_graph=0)),
+                         ("mg_vcycle", dict(pc=igamf.PC_FEM_MG)),
+                         ("jacobi", dict(pc=igamf.PC_JACOBI))):
+            s = igamf.Solver(op, fem, rtol=1e-10, **kw)
+            s.solve(F)
+            _, st = s.solve(F)
+            row[name] = {"iterations": st.iterations, "solve_ms": round(st.solve_ms, 2),
+                         "rel_residual": st.rel_residual}
+        rows[f"{n}x{n}"] = row
+    out["fempc_iterations"] = rows
+    log("fem-pc iterations (ours inner-CG / V-cycle / jacobi vs reference): " + ", ".join(
+        f"{k}: {v['inner_cg_1e-10']['iterations']}/{v['mg_vcycle']['iterations']}/"
+        f"{v['jacobi']['iterations']} vs {v['reference_logged']}" for k, v in rows.items()))
 
 
 def block(out):
@@ -709,6 +810,8 @@ def main():
         fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, op.nfunc, op.nelem, op.nqp)
         if not a.no_solve:
             solves(out, op, tuple(x for x in a.solve_configs.split(",") if x), a.solve_reps)
+            if not a.solve_configs:
+                solves_mg(out, mesh, op, a.solve_reps)
         if not a.no_cpu:
             cpu_baseline(a, out, mesh, x64, op, kern, asm, n_int)
         del op
@@ -721,6 +824,7 @@ def main():
             del fem
         if not a.no_extra:
             c1(out)
+            fempc_iterations(out)
             c4(out)
     print(json.dumps(out), flush=True)
 

@@ -430,9 +430,13 @@ typedef struct {
   double rtol, abstol, divtol;  /* on ||f - A u||_2 (FP64) */
   int maxit;                    /* total inner iterations cap */
   double inner_rtol;            /* F32_REFINE: residual reduction between
-                                   reliable (FP64) updates (default 0.1:
-                                   fewer FP64 residuals than 0.5 at the same
-                                   inner iteration count) */
+                                   reliable (FP64) updates; 0 (default) =
+                                   automatic: 0.1 with Jacobi or none (fewer
+                                   FP64 residuals than 0.5 at the same inner
+                                   iteration count, fewer FP32 rounding
+                                   losses than longer recursions), 1e-4 with
+                                   a FEM preconditioner (few, costly
+                                   iterations) */
   int inner_maxit;
   int pc_inner_maxit;           /* FEM_CG: inner CG iteration cap */
   double pc_inner_rtol;         /* FEM_CG: inner CG tolerance */
@@ -465,7 +469,7 @@ typedef struct {
 
 typedef struct iga_solver iga_solver;
 /* Defaults: PCG_F64, Jacobi, SF kernel, INVLM gather; rtol 1e-8, abstol 0,
-   divtol 1e4, maxit 100000; inner_rtol 0.1, inner_maxit 100000;
+   divtol 1e4, maxit 100000; inner_rtol 0 (automatic), inner_maxit 100000;
    pc_inner_maxit 20, pc_inner_rtol 1e-2, pc_cheb_degree 8; use_graph 1,
    inner_f16_geometry 0, fused 1.  Host code only (no device call). */
 void iga_solver_opts_default(iga_solver_opts *o);

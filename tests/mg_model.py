@@ -4,14 +4,14 @@ scipy sparse matrices, written from its definition rather than its code:
   A_0      the Q1 matrix K with the couplings to Dirichlet (boundary) nodes
            removed and identity rows on them: M K M + (I - M), M the
            interior mask;
-  P_l      per direction with more than 5 nodes, coarse nodes = the even
+  P_l      per direction with more than 17 nodes, coarse nodes = the even
            nodes and the last one, a fine node between two coarse nodes
            takes 1/2 of each; P = Py (x) Px with the fine and coarse
            boundary rows / columns removed;
   A_{l+1}  Galerkin P^T A_l P (identity rows on the coarse boundary);
   smoother Chebyshev-Jacobi of degree 2 on [lmax/4, lmax], lmax the largest
            interior row sum of |D^-1 A|, pre and post;
-  coarsest both directions with at most 5 nodes: exact solve.
+  coarsest both directions with at most 17 nodes: exact solve.
 Used by the GPU tests as the reference of one V-cycle.
 """
 import numpy as np
@@ -49,10 +49,10 @@ class VCycle:
         self.levels = []
         while True:
             self.levels.append(dict(A=A, nx=nx, ny=ny, mask=mask))
-            if nx <= 5 and ny <= 5:
+            if nx <= 17 and ny <= 17:
                 break
-            Px, cnx = _interp1d(nx, nx > 5)
-            Py, cny = _interp1d(ny, ny > 5)
+            Px, cnx = _interp1d(nx, nx > 17)
+            Py, cny = _interp1d(ny, ny > 17)
             cm = _ring_mask(cnx, cny)
             P = (sp.diags(mask) @ sp.kron(Py, Px) @ sp.diags(cm)).tocsr()
             A = (sp.diags(cm) @ (P.T @ A @ P) @ sp.diags(cm) + sp.diags(1.0 - cm)).tocsr()

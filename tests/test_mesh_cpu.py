@@ -178,7 +178,7 @@ def test_solver_opts_defaults():
     assert (o.kind, o.pc, o.kernel, o.assembly) == (
         igamf.SOLVE_PCG_F64, igamf.PC_JACOBI, igamf.SF, igamf.INVLM_GATHER)
     assert (o.rtol, o.abstol, o.divtol, o.maxit) == (1e-8, 0.0, 1e4, 100000)
-    assert (o.inner_rtol, o.inner_maxit) == (0.1, 100000)
+    assert (o.inner_rtol, o.inner_maxit) == (0.0, 100000)  # 0: automatic
     assert (o.pc_inner_maxit, o.pc_inner_rtol, o.pc_cheb_degree) == (20, 1e-2, 8)
     assert (o.use_graph, o.inner_f16_geometry, o.fused) == (1, 0, 1)
 
