@@ -122,7 +122,8 @@ int iga_mesh_eval(const iga_mesh *m, int64_t n, const double *uv, double *xy);
  * are not in it (they are in the preprocess info file).  A file without the
  * element sizes or extraction operators (the short WritePartition form,
  * :71-112) gives IGA_ERR_BAD_MESH. */
-int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out);
+int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out);  /* reals
+   that are not finite: IGA_ERR_NOT_FINITE */
 /* Write m in the full form (ghostID empty: one partition).  The format has
  * no NURBS weights: IGA_ERR_UNSUPPORTED for a mesh that has them. */
 int iga_mesh_write_partition(const iga_mesh *m, const char *filename);
@@ -157,6 +158,9 @@ typedef struct {
   const int *ien;     /* [nlocalelemx * nlocalelemy][4] indices into id / cp */
   const double *cp;   /* [nid][2] */
 } iga_fem_partition_desc;
+/* IGA_ERR_BAD_MESH for missing sections or inconsistent sizes, IGA_ERR_NOT_FINITE
+ * for a non-finite coordinate (the reference's p = 2 FEM preprocessing
+ * writes "nan": AbscissaeGenerator averages over zero knots). */
 int iga_fem_read_partition(const char *filename, iga_fem_partition **out);
 int iga_fem_partition_view(const iga_fem_partition *fp, iga_fem_partition_desc *desc);
 int iga_fem_partition_destroy(iga_fem_partition *fp);

@@ -15,7 +15,9 @@
 // contribution), and values summed in contribution order inside each segment:
 // indices are bit-exact and values do not depend on thread timing.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <cmath>
 #include <new>
@@ -264,17 +266,21 @@ int iga_fem_assemble(int nfx, int nfy, const double *nodes_h, int device, iga_fe
   if (hipGetLastError() != hipSuccess) return cleanup(IGA_ERR_HIP);
   size_t tb = 0, tb2 = 0;
   void *temp = nullptr;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, keys_s, idx, perm, (int)n, 0, end_bit,
-                                         s) != hipSuccess ||
-      hipcub::DeviceScan::InclusiveSum(nullptr, tb2, head, seg, (int)n, s) != hipSuccess)
+  // rocPRIM's LSD radix sort is stable: equal keys keep their contribution
+  // order, so the sums below run in a fixed order (deterministic values)
+  if (rocprim::radix_sort_pairs(nullptr, tb, keys, keys_s, idx, perm, (size_t)n, 0u,
+                                (unsigned)end_bit, s) != hipSuccess ||
+      rocprim::inclusive_scan(nullptr, tb2, head, seg, (size_t)n, rocprim::plus<int>(), s) !=
+          hipSuccess)
     return cleanup(IGA_ERR_HIP);
   if (!tmp_alloc(&temp, tb > tb2 ? tb : tb2)) return cleanup(IGA_ERR_OUT_OF_MEMORY);
-  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, keys_s, idx, perm, (int)n, 0, end_bit,
-                                         s) != hipSuccess)
+  if (rocprim::radix_sort_pairs(temp, tb, keys, keys_s, idx, perm, (size_t)n, 0u,
+                                (unsigned)end_bit, s) != hipSuccess)
     return cleanup(IGA_ERR_HIP);
   hipLaunchKernelGGL(k_heads, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, keys_s, (int)n,
                      sentinel, head);
-  if (hipcub::DeviceScan::InclusiveSum(temp, tb2, head, seg, (int)n, s) != hipSuccess)
+  if (rocprim::inclusive_scan(temp, tb2, head, seg, (size_t)n, rocprim::plus<int>(), s) !=
+      hipSuccess)
     return cleanup(IGA_ERR_HIP);
   // nnz = last inclusive-scan value; valid entries precede the sentinels
   int last = 0;

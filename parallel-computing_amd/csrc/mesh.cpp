@@ -397,6 +397,12 @@ std::map<std::string, std::vector<double>> sections(const std::vector<std::strin
   return sec;
 }
 
+bool all_finite(const std::vector<double> &v) {
+  for (double d : v)
+    if (!std::isfinite(d)) return false;
+  return true;
+}
+
 void to_ints(const std::vector<double> &v, std::vector<int> &o) {
   o.resize(v.size());
   for (size_t i = 0; i < v.size(); ++i) o[i] = (int)v[i];
@@ -428,6 +434,8 @@ int iga_mesh_read_partition(const char *filename, int p, int q, iga_mesh **out) 
       sec["NURBSExtraction1"].size() != (size_t)nex * (p + 1) * (p + 1) ||
       sec["NURBSExtraction2"].size() != (size_t)ney * (q + 1) * (q + 1))
     return IGA_ERR_BAD_MESH;
+  for (const char *k : {"CP", "ElemSize1", "ElemSize2", "NURBSExtraction1", "NURBSExtraction2"})
+    if (!all_finite(sec[k])) return IGA_ERR_NOT_FINITE;
   iga_mesh *m = new (std::nothrow) iga_mesh();
   if (!m) return IGA_ERR_OUT_OF_MEMORY;
   m->p = p;
@@ -523,6 +531,9 @@ int iga_fem_read_partition(const char *filename, iga_fem_partition **out) {
   if (nid < (size_t)nfunc || sec["CP"].size() != 2 * nid ||
       sec["IEN"].size() != (size_t)nex * ney * 4)
     return IGA_ERR_BAD_MESH;
+  // the reference's p = 2 preprocessing writes "nan" coordinates (a
+  // degree-0 Greville average); its reader then stops at the first one
+  if (!all_finite(sec["CP"])) return IGA_ERR_NOT_FINITE;
   iga_fem_partition *fp = new (std::nothrow) iga_fem_partition();
   if (!fp) return IGA_ERR_OUT_OF_MEMORY;
   fp->nlocalfunc = (int)nfunc;

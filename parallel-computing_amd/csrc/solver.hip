@@ -43,6 +43,7 @@
 #include <vector>
 
 #include "op_internal.hpp"
+#include "outcome.hpp"
 
 struct iga_fem;  // defined in fem.hip; only its CSR view is used here
 
@@ -1815,7 +1816,21 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
     if (s->dinv32) LAUNCH((k_cast<float>), nb, s->dinv, s->dinv32, n);
     s->gen = op->gen;
   }
-  if (hipStreamSynchronize(st) != hipSuccess) return IGA_ERR_HIP;
+  // a right-hand side with a NaN or an infinity: an error, not a divergence
+  // (the reference would iterate on it until divtol or maxit)
+  {
+    LAUNCH(k_dot, nb, f, f, n, s->part);
+    hipLaunchKernelGGL(k_fold1, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc + 2 * S_NS + 8);
+    double ff = 0.0;
+    if (hipMemcpyAsync(&ff, s->sc + 2 * S_NS + 8, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return IGA_ERR_HIP;
+    if (!std::isfinite(ff)) {
+      stt.status = IGA_ERR_NOT_FINITE;
+      if (stats) *stats = stt;
+      return IGA_ERR_NOT_FINITE;
+    }
+  }
   const auto t0 = std::chrono::steady_clock::now();
   int status = IGA_OK;
   if (o.kind == IGA_SOLVE_PCG_F64) {
@@ -1881,9 +1896,8 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
         rc = outer_residual(s, f, s->x64, s->r, h, st);
         if (rc) return rc;
         rr = h[0];
-        if (!(rr == rr) || rr > o.divtol * o.divtol * ff) { status = IGA_ERR_DIVERGED; break; }
-        if (rr <= tol2) { status = IGA_OK; break; }
-        if (done == 2) break;
+        const int oc = igai::outer_outcome(rr, ff, tol2, o.divtol, done == 2);
+        if (oc != igai::OUTCOME_CONTINUE) { status = oc; break; }
         LAUNCH(k_cgf_ru_reset, nb, s->r, s->r32, s->z32, s->rold32, dinv32, n, s->cgs);
         hipLaunchKernelGGL(k_cgf_ru_fin, dim3(1), dim3(64), 0, st, s->sc, s->ic, RU2 * rr);
       }
@@ -1911,9 +1925,8 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
         rc = outer_residual(s, f, s->x64, s->r, h, st);
         if (rc) return rc;
         rr = h[0];
-        if (!(rr == rr) || rr > o.divtol * o.divtol * ff) { status = IGA_ERR_DIVERGED; break; }
-        if (rr <= tol2) { status = IGA_OK; break; }
-        if (done == 2) break;
+        const int oc = igai::outer_outcome(rr, ff, tol2, o.divtol, done == 2);
+        if (oc != igai::OUTCOME_CONTINUE) { status = oc; break; }
         LAUNCH(k_ru_reset, nb, s->r, s->r32, s->z32, dinv32, loc ? 1 : 0, n, s->part);
         if (!loc) {
           hipLaunchKernelGGL(k_ru_clear, dim3(1), dim3(64), 0, st, s->ic);
@@ -1989,11 +2002,15 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
            s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, s->bp32 + (size_t)j * n, dinv32, 1,
            n, colp(j));
     const double rr = h[0];
-    const bool done = !(rr == rr) || rr <= tol2[j];
+    const bool bad_f = !std::isfinite(ff[j]);
+    const bool done = bad_f || !(rr == rr) || rr <= tol2[j];
     hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, colp(j), nb, colsc(j), colic(j),
                        done ? 1.0 : tol2[j] / rr, 0.0, o.divtol * o.divtol * ff[j] / rr,
                        o.maxit, RU2);
-    if (!(rr == rr)) {
+    if (bad_f) {
+      status[j] = IGA_ERR_NOT_FINITE;
+      fin[j] = 1;
+    } else if (!(rr == rr)) {
       status[j] = IGA_ERR_DIVERGED;
       fin[j] = 1;
     } else if (rr <= tol2[j]) {
@@ -2035,14 +2052,9 @@ int iga_solver_solve_block(iga_solver *s, int k, int ld, const double *F, double
       rc = outer_residual(s, F + (size_t)j * ld, xj, s->r, h, st);
       if (rc) return rc;
       rrf[j] = h[0];
-      if (!(h[0] == h[0]) || h[0] > o.divtol * o.divtol * ff[j]) {
-        status[j] = IGA_ERR_DIVERGED;
-        fin[j] = 1;
-      } else if (h[0] <= tol2[j]) {
-        status[j] = IGA_OK;
-        fin[j] = 1;
-      } else if (code == 2) {
-        status[j] = IGA_ERR_NOT_CONVERGED;
+      const int oc = igai::outer_outcome(h[0], ff[j], tol2[j], o.divtol, code == 2);
+      if (oc != igai::OUTCOME_CONTINUE) {
+        status[j] = oc;
         fin[j] = 1;
       } else {
         LAUNCH(k_ru_reset, nb, s->r, s->br32 + (size_t)j * n, s->bz32 + (size_t)j * n, dinv32, 1,

@@ -692,6 +692,10 @@ def _copy_dev(dst_tensor, src_ptr, nbytes):
 
 
 # --------------------------------------------------------------------------- solver
+# statuses that describe the solve's outcome rather than a failed call
+_SOLVE_OUTCOMES = (IGA_ERR_NOT_CONVERGED, IGA_ERR_DIVERGED, IGA_ERR_NOT_FINITE)
+
+
 class Solver:
     def __init__(self, op, fem=None, **kw):
         o = SolverOpts()
@@ -710,8 +714,8 @@ class Solver:
 
     def solve(self, f, u=None, stream=None, check_status=True):
         """Solve A u = f; returns (u, stats).  A solve that does not converge
-        raises IgaError unless check_status is False (stats.status then
-        carries the code)."""
+        (or an f holding a NaN or an infinity) raises IgaError unless
+        check_status is False (stats.status then carries the code)."""
         import torch
         check_vec(f, torch.float64, self.op.nfunc, self.op.device, "f")
         if u is None:
@@ -722,7 +726,7 @@ class Solver:
         st = SolveStats()
         rc = _lib().iga_solver_solve(self._h, f.data_ptr(), u.data_ptr(), _stream_ptr(stream),
                                      C.byref(st))
-        if rc != 0 and (check_status or rc not in (IGA_ERR_NOT_CONVERGED, IGA_ERR_DIVERGED)):
+        if rc != 0 and (check_status or rc not in _SOLVE_OUTCOMES):
             raise IgaError(rc, "solver_solve")
         return u, st
 
@@ -761,7 +765,7 @@ class Solver:
         st = (SolveStats * k)()
         rc = _lib().iga_solver_solve_block(self._h, k, F.stride(0), F.data_ptr(), U.data_ptr(),
                                            _stream_ptr(stream), st)
-        if rc != 0 and (check_status or rc not in (IGA_ERR_NOT_CONVERGED, IGA_ERR_DIVERGED)):
+        if rc != 0 and (check_status or rc not in _SOLVE_OUTCOMES):
             raise IgaError(rc, "solver_solve_block")
         return U, list(st)
 

@@ -330,3 +330,26 @@ def test_mesh_eval_reproduces_the_geometry():
     assert r.min() >= 1 - 1e-14 and r.max() <= 2 + 1e-14
     with pytest.raises(igamf.IgaError):
         an.eval([[1.5, 0.5]])
+
+
+def test_partition_readers_reject_non_finite_values(tmp_path):
+    """NaN coordinates (what the reference's p = 2 FEM preprocessing writes)
+    are IGA_ERR_NOT_FINITE, in both partition readers."""
+    fem = tmp_path / "part_fem_0.txt"
+    fem.write_text("nlocalfunc: 4\nnlocalelemx: 1\nnlocalelemy: 1\nID\n-1 -1 -1 -1 \nDir\n0 1 2 3 \n"
+                   "IEN\n0 1 2 3 \nCP\nnan nan 1 0 0 1 1 1 \n")
+    with pytest.raises(igamf.IgaError) as e:
+        igamf.read_fem_partition(str(fem))
+    assert e.value.status == igamf.IGA_ERR_NOT_FINITE
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 2, 4)
+    fn = tmp_path / "part_0.txt"
+    m.write_partition(str(fn))
+    txt = fn.read_text().split("\n")
+    i = txt.index("CP")
+    vals = txt[i + 1].split()
+    vals[3] = "inf"
+    txt[i + 1] = " ".join(vals) + " "
+    fn.write_text("\n".join(txt))
+    with pytest.raises(igamf.IgaError) as e:
+        igamf.Mesh.read_partition(str(fn), 2, 2)
+    assert e.value.status == igamf.IGA_ERR_NOT_FINITE

@@ -483,3 +483,34 @@ def test_mg_preconditioned_solve(geom):
         assert its[(sk, 96)] <= its[(sk, 48)] + 4, its
         if geom == "annulus":
             assert its[(sk, 96)] < 0.6 * its[("jacobi", 96)], its
+
+
+def test_non_finite_rhs_and_divergence():
+    """Failure paths forced from the host: a right-hand side holding a NaN is
+    IGA_ERR_NOT_FINITE (single and block solves, before any iteration); a
+    divergence tolerance below 1 is IGA_ERR_DIVERGED at the first iteration
+    (the test ||r|| > divtol ||f|| of PETSc's KSPConvergedDefault, divtol 1e4
+    by default); an iteration cap below convergence is IGA_ERR_NOT_CONVERGED."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 24)
+    op = igamf.Operator(m)
+    F = op.load(igamf.LOAD_CONST, 1.0)
+    bad = F.clone()
+    bad[100] = float("nan")
+    for kind in (igamf.SOLVE_PCG_F64, igamf.SOLVE_PCG_F32_REFINE):
+        s = igamf.Solver(op, None, kind=kind)
+        with pytest.raises(igamf.IgaError) as e:
+            s.solve(bad)
+        assert e.value.status == igamf.IGA_ERR_NOT_FINITE
+        _, st = s.solve(bad, check_status=False)
+        assert st.status == igamf.IGA_ERR_NOT_FINITE and st.iterations == 0
+        # divtol below 1: the first iterate's residual already exceeds divtol ||f||
+        sd = igamf.Solver(op, None, kind=kind, divtol=1e-3)
+        _, st = sd.solve(F, check_status=False)
+        assert st.status == igamf.IGA_ERR_DIVERGED, st.status
+        sn = igamf.Solver(op, None, kind=kind, maxit=5)
+        _, st = sn.solve(F, check_status=False)
+        assert st.status == igamf.IGA_ERR_NOT_CONVERGED, st.status
+    sb = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F32_REFINE)
+    Fk = torch.stack([F, bad]).contiguous()
+    _, sts = sb.solve_block(Fk, check_status=False)
+    assert sts[0].status == 0 and sts[1].status == igamf.IGA_ERR_NOT_FINITE
