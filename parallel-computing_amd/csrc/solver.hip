@@ -58,6 +58,9 @@ constexpr int CHUNK = 16;  // iterations per enqueued chunk / graph
 // stopping chunk and the one enqueued ahead, each kernel returning at once
 // but still dispatched) few, and still hide the host's flag check.
 constexpr int CHUNK_PC = 4;
+// iterations per chunk near a predicted stop (run_chunks), enqueued as plain
+// launches
+constexpr int CHUNK_SHORT = 4;
 
 // scalar slots (doubles) of one CG instance
 enum { S_RZ = 0, S_RZ_OLD, S_RR, S_RR0, S_TOL2, S_DIV2, S_ALPHA, S_BETA, S_UPD2, S_NS };
@@ -1095,6 +1098,7 @@ struct iga_solver {
   // pinned host copies of the done / iteration flags after two chunks in
   // flight, and the events that mark them complete (run_chunks)
   int *hflag = nullptr;
+  double *hsc = nullptr;  // [2][S_NS] the main CG's scalars after those chunks
   hipEvent_t fev[2] = {nullptr, nullptr};
 };
 
@@ -1263,41 +1267,98 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
 }
 
-// Enqueue chunks (launch()) until the device's done flag is set.  The flags
-// after chunk k are copied to pinned memory and read while chunk k + 1 runs,
-// so the device does not idle for the host round trip between chunks; the
-// one chunk enqueued past the end finds the flag set and its kernels return
-// at once (later work on st stays behind it).
+// Enqueue chunks (launch(nit): nit iterations) until the device's done flag
+// is set.  The flags and the scalars after a chunk are copied to pinned
+// memory and read while the next chunk runs, so the device does not idle for
+// the host round trip between chunks.  Every iteration enqueued past the stop
+// still dispatches its kernels (they return at once, a few microseconds each
+// with the dispatch gaps), so near the stop the run switches to short
+// chunks: the squared residual's decay rate since the run's first chunk
+// predicts the iteration at which it meets the stop threshold (tolerance or
+// reliable-update level), and
+//  - a stop predicted inside the chunk in flight enqueues nothing more (the
+//    host waits for that chunk's flags),
+//  - a stop predicted inside the next chunk enqueues a short one.
+// A wrong prediction costs one host round trip or a few skipped iterations;
+// the iterations themselves are the same.
 template <typename L>
-int run_chunks(iga_solver *s, hipStream_t st, L &&launch, int *iters, int *done) {
-  auto issue = [&](int slot) -> int {
-    const int rc = launch();
+int run_chunks(iga_solver *s, hipStream_t st, L &&launch, int big, int small, int *iters,
+               int *done) {
+  int nit_of[2] = {0, 0};
+  auto issue = [&](int slot, int nit) -> int {
+    const int rc = launch(nit);
     if (rc) return rc;
     if (hipMemcpyAsync(s->hflag + 2 * slot, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
+        hipMemcpyAsync(s->hsc + S_NS * slot, s->sc, S_NS * sizeof(double), hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
         hipEventRecord(s->fev[slot], st) != hipSuccess)
       return IGA_ERR_HIP;
+    nit_of[slot] = nit;
     return IGA_OK;
   };
-  int rc = issue(0);
-  if (rc) return rc;
-  for (int k = 0;; ++k) {
-    const int cur = k & 1;
-    rc = issue(cur ^ 1);
+  // chunks in flight, oldest first (at most two)
+  int q[2], nq = 0, next_slot = 0;
+  auto push = [&](int nit) -> int {
+    const int slot = next_slot;
+    next_slot ^= 1;
+    const int rc = issue(slot, nit);
     if (rc) return rc;
+    q[nq++] = slot;
+    return IGA_OK;
+  };
+  int rc = push(big);
+  if (rc) return rc;
+  bool have0 = false, wait_only = false;
+  int it0 = 0, ahead = big;
+  double lrr0 = 0.0;
+  for (;;) {
+    if (nq == 1 && !wait_only) {
+      rc = push(ahead);
+      if (rc) return rc;
+    }
+    const int cur = q[0];
     if (hipEventSynchronize(s->fev[cur]) != hipSuccess) return IGA_ERR_HIP;
+    q[0] = q[1];
+    --nq;
     const int *h = s->hflag + 2 * cur;
     if (h[I_DONE]) {
       *iters = h[I_ITER];
       *done = h[I_DONE];
       return IGA_OK;
     }
+    const double *hs = s->hsc + S_NS * cur;
+    const double rr = hs[S_RR], stop = std::fmax(hs[S_TOL2], hs[S_UPD2]);
+    const int it = h[I_ITER];
+    const int inflight = nq ? nit_of[q[0]] : 0;
+    double need = 1e30;  // predicted iterations from it to the stop
+    if (rr > 0.0 && stop > 0.0 && std::isfinite(rr)) {
+      const double lrr = std::log(rr);
+      if (!have0) {
+        have0 = true;
+        it0 = it;
+        lrr0 = lrr;
+      } else if (it > it0 && lrr < lrr0) {
+        const double rate = (lrr - lrr0) / (it - it0);  // log decrease per iteration (< 0)
+        need = (std::log(stop) - lrr) / rate;
+      }
+    }
+    wait_only = need <= inflight;
+    ahead = need <= inflight + big ? small : big;
+    if (nq == 0) {  // nothing in flight: the next chunk goes now
+      rc = push(wait_only ? small : ahead);
+      if (rc) return rc;
+      wait_only = true;
+    }
   }
 }
 
+int chunk_big(const iga_solver *s) { return local_pc(s) ? CHUNK : CHUNK_PC; }
+int chunk_small(const iga_solver *s) { return local_pc(s) ? CHUNK_SHORT : 1; }
+
 template <typename V, typename TU>
-int cg_chunk(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st) {
-  const int nit = local_pc(s) ? CHUNK : CHUNK_PC;
+int cg_chunk(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
+             int nit) {
   for (int k = 0; k < nit; ++k) {
     int rc = cg_iteration<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
     if (rc) return rc;
@@ -1335,7 +1396,7 @@ int cg_run(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hip
       (void)hipStreamDestroy(cs);
       return IGA_ERR_HIP;
     }
-    int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, cs);
+    int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, cs, chunk_big(s));
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
@@ -1353,14 +1414,15 @@ int cg_run(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hip
 template <typename V, typename TU>
 int cg_resume(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hipStream_t st,
               hipGraphExec_t *gexec, int *iters, int *done) {
+  const int big = chunk_big(s);
   return run_chunks(
       s, st,
-      [&]() -> int {
-        if (s->o.use_graph && *gexec)
+      [&](int nit) -> int {
+        if (nit == big && s->o.use_graph && *gexec)
           return hipGraphLaunch(*gexec, st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
-        return cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st);
+        return cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st, nit);
       },
-      iters, done);
+      big, chunk_small(s), iters, done);
 }
 
 // ---- fused iteration (local preconditioners, single solves)
@@ -1404,8 +1466,9 @@ int cgf_iteration(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w,
 }
 
 template <typename V, typename TU>
-int cgf_chunk(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st) {
-  for (int k = 0; k < CHUNK; ++k) {
+int cgf_chunk(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipStream_t st,
+              int nit) {
+  for (int k = 0; k < nit; ++k) {
     int rc = cgf_iteration<V, TU>(s, f64, x, r, p, sv, u, w, st);
     if (rc) return rc;
   }
@@ -1428,7 +1491,7 @@ int cgf_run(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipSt
       (void)hipStreamDestroy(cs);
       return IGA_ERR_HIP;
     }
-    int rc = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, cs);
+    int rc = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, cs, CHUNK);
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
@@ -1441,12 +1504,12 @@ int cgf_run(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipSt
   }
   return run_chunks(
       s, st,
-      [&]() -> int {
-        if (s->o.use_graph)
+      [&](int nit) -> int {
+        if (nit == CHUNK && s->o.use_graph)
           return hipGraphLaunch(*gexec, st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
-        return cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, st);
+        return cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, st, nit);
       },
-      iters, done);
+      CHUNK, CHUNK_SHORT, iters, done);
 }
 
 // |f - A u|^2 and |f|^2 into h (host), r = f - A u
@@ -1594,6 +1657,7 @@ int iga_solver_destroy(iga_solver *s) {
   for (hipEvent_t e : s->fev)
     if (e) (void)hipEventDestroy(e);
   if (s->hflag) (void)hipHostFree(s->hflag);
+  if (s->hsc) (void)hipHostFree(s->hsc);
   igai::mg_destroy(s->mg);
   for (void *p : s->allocs) (void)hipFree(p);
   for (void *p : s->ballocs) (void)hipFree(p);
@@ -1667,6 +1731,8 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     SA(s->rold32, n * 4);
   }
   if (hipHostMalloc((void **)&s->hflag, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&s->hsc, 2 * S_NS * sizeof(double), hipHostMallocDefault) !=
+          hipSuccess ||
       hipEventCreateWithFlags(&s->fev[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&s->fev[1], hipEventDisableTiming) != hipSuccess)
     return fail(IGA_ERR_HIP);
