@@ -210,6 +210,8 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   a.mask_in = mask_in;
   a.mask_out = mask_out;
   a.skip = skip;
+  a.rev = op->rev_next;
+  op->rev_next ^= 1;
   if (!a.G) return IGA_ERR_UNSUPPORTED;
   const int ntiles = L.ntx * L.nty;
   const int tile = f64 ? op->tile64 : op->tile32;

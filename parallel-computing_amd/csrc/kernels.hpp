@@ -59,6 +59,7 @@ struct ApplyArgs {
   double *hdst;           // the CG state between the update kernels around this one)
   int hn;
   float gscale;           // FP16-stored G_q: the power of two they were divided by
+  int rev;                // k_apply: visit the tiles in reverse order
 };
 
 __device__ __forceinline__ bool is_dirichlet(const uint8_t *mask, int A, int gx,
@@ -677,7 +678,8 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   if (A.skip && *A.skip) return;
   cg_handoff(A, NT);
   const int t = threadIdx.x, h = t % SPLIT, el = t / SPLIT, lx = el % TX, ly = el / TX;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile0 = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = A.rev ? (int)gridDim.x - 1 - tile0 : tile0;
   const int tx = tile % A.ntx, ty = tile / A.ntx;
   const int ex0 = tx * TX, ey0 = ty * TY;
   const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);

@@ -20,6 +20,8 @@
   } while (0)
 
 struct iga_op {
+  // direction of the next apply's tile sweep (alternates: see k_apply)
+  int rev_next = 0;
   int device = 0;
   int p = 0, nq = 0, nex = 0, ney = 0, nfx = 0, nfy = 0;
   int nfunc = 0, nelem = 0, nqp = 0, nloc = 0, ndir = 0;
