@@ -73,6 +73,11 @@ def parse():
                     help="comma-separated solve keys to run (e.g. fp64_atomic_fused1); "
                          "default all")
     ap.add_argument("--solve-reps", type=int, default=3, help="timed solves per config")
+    ap.add_argument("--no-block-solve", action="store_true",
+                    help="block section: the applies only (for profiler runs)")
+    ap.add_argument("--only", default="",
+                    help="after the headline, run only these sections (comma list of "
+                         "variants, solve, mg, cpu, block, c5, extra)")
     return ap.parse_args()
 
 
@@ -452,7 +457,7 @@ def fempc_iterations(out):
         f"{v['jacobi']['iterations']} vs {v['reference_logged']}" for k, v in rows.items()))
 
 
-def block(out):
+def block(out, with_solve=True):
     """Multi-RHS apply, quarter annulus p = 4, 512^2, 5 x 5 Gauss, k = 32:
     the apply on f_j sampled at the Greville abscissae, the block solve on the
     load vectors of f_j (a_mn ~ N(0,1) from seed j)."""
@@ -472,6 +477,19 @@ def block(out):
     for kk in (8, 16, 32):
         cfg, ms = op.tune_block(X[:kk], Y[:kk], reps=5)
         tuned[kk] = {"cfg": list(cfgs[cfg]), "tuned_ms": round(ms, 4)}
+    best = op.block_config(k)
+    # every matrix-core configuration at k, against the tuned choice
+    mfma = {}
+    for ci, c in enumerate(cfgs):
+        if not igamf.block_config_matrix_core(ci):
+            continue
+        op.set_block_config(k, ci)
+        tm, _ = ev_time(lambda: op.apply_block(X, Y), 10, 2)
+        mfma[f"{c[0]}x{c[1]}x{c[2]}"] = {
+            "ms": round(tm, 4),
+            "tflops": round((16 * (p + 1) ** 3 + 6 * (p + 1) ** 2) * op.nelem * k
+                            / (tm * 1e-3) / 1e12, 2)}
+    op.set_block_config(k, best)
     t_block, _ = ev_time(lambda: op.apply_block(X, Y), 20, 3)
     single = torch.stack([op.apply(X[j].contiguous(), assembly=igamf.IEN_ATOMIC)
                           for j in range(k)])
@@ -486,10 +504,15 @@ def block(out):
         "tflops": round(tf, 2), "frac_fp32_peak": round(tf / FP32_PEAK_TFLOPS, 4),
         "flops_per_call": flops, "tuned": tuned,
         "config_used": list(cfgs[op.block_config(k)]),
+        "config_used_matrix_core": igamf.block_config_matrix_core(op.block_config(k)),
+        "matrix_core_configs": mfma,
         "max_rel_diff_vs_single": rel,
-        "note": "VALU (non-packed FP32) kernel; the flop count is the sum-factorised "
-                "16(p+1)^3 + 6(p+1)^2 per element and rhs"}
+        "note": "tuned over the VALU (non-packed FP32) and matrix-core "
+                "(v_mfma_f32_16x16x4f32, Kronecker form) kernels; tflops counts the "
+                "sum-factorised 16(p+1)^3 + 6(p+1)^2 per element and rhs for both"}
     log(f"block k={k}: {t_block * 1e3 / k:.1f} us/rhs, {tf:.1f} TFLOP/s")
+    if not with_solve:
+        return
     # block PCG: the k load vectors F_j of f_j in one block solve, against k
     # single FP32 + refinement solves and k plain FP64 solves, Jacobi, rtol 1e-8
     F = torch.stack([op.load_qp(torch.from_numpy(c3_qp_values(p, ne, j)).cuda())
@@ -806,23 +829,30 @@ def main():
                 op.time_kernel(x32, torch.empty_like(x32), a.steps, kern, asm, g16=True)
             elif form == "f64":
                 op.time_kernel(x64, torch.empty_like(x64), a.steps, kern, asm)
+    only = set(x for x in a.only.split(",") if x)
+
+    def want(sec, skip):
+        return sec in only if only else not skip
+
     if not a.only_apply:
-        fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, op.nfunc, op.nelem, op.nqp)
-        if not a.no_solve:
+        if want("variants", False):
+            fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, op.nfunc, op.nelem,
+                              op.nqp)
+        if want("solve", a.no_solve):
             solves(out, op, tuple(x for x in a.solve_configs.split(",") if x), a.solve_reps)
-            if not a.solve_configs:
-                solves_mg(out, mesh, op, a.solve_reps)
-        if not a.no_cpu:
+        if want("mg", a.no_solve or bool(a.solve_configs)):
+            solves_mg(out, mesh, op, a.solve_reps)
+        if want("cpu", a.no_cpu):
             cpu_baseline(a, out, mesh, x64, op, kern, asm, n_int)
         del op
-        if not a.no_block:
-            block(out)
-        if not a.no_c5:
+        if want("block", a.no_block):
+            block(out, not a.no_block_solve)
+        if want("c5", a.no_c5):
             fem, nodes, p5, ne5 = c5(out)
-            if not a.no_cpu:
+            if want("cpu", a.no_cpu):
                 cpu_check_fem(out, fem, nodes, p5, ne5)
             del fem
-        if not a.no_extra:
+        if want("extra", a.no_extra):
             c1(out)
             fempc_iterations(out)
             c4(out)

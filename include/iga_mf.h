@@ -356,6 +356,12 @@ int iga_op_tune_block(iga_op *op, int k, int ld, const float *X, float *Y,
  * built-in default), so a tuning result can be stored and restored. */
 int iga_block_config_count(void);
 int iga_block_config_desc(int cfg, int *tile_x, int *tile_y, int *rhs_chunk);
+/* *matrix_core = 1 when configuration cfg runs on the matrix cores
+ * (v_mfma_f32_16x16x4f32, the element operator as two dense products with
+ * the 16 right-hand sides of a chunk as columns; degrees up to 4), 0 for the
+ * VALU kernel.  No counterpart in the reference, which applies one vector
+ * per MatMult (matrix_free_gpu/matrixfreegpu.cu:52-66). */
+int iga_block_config_matrix_core(int cfg, int *matrix_core);
 int iga_op_set_block_config(iga_op *op, int k, int cfg);
 int iga_op_get_block_config(const iga_op *op, int k, int *cfg);
 

@@ -690,7 +690,7 @@ bool block_cfg_ok(const iga_op *op, int cfg, int k) {
   // compiled for the degree; a chunk wider than k leaves most threads idle
   if (!igak::block_cfg_compiled(op->p, cfg)) return false;
   const igak::BlockCfg c = igak::block_cfg(cfg);
-  return c.kr <= 8 || c.kr <= k;
+  return c.kr <= 8 || c.kr <= k;  // (the MFMA form has 16-wide chunks: k >= 16)
 }
 
 int block_default(const iga_op *op, int k) {
@@ -781,6 +781,12 @@ int iga_block_config_desc(int cfg, int *tile_x, int *tile_y, int *rhs_chunk) {
   if (tile_x) *tile_x = c.txb;
   if (tile_y) *tile_y = c.tyb;
   if (rhs_chunk) *rhs_chunk = c.kr;
+  return IGA_OK;
+}
+
+int iga_block_config_matrix_core(int cfg, int *matrix_core) {
+  if (cfg < 0 || cfg >= igak::NBLOCKCFG || !matrix_core) return IGA_ERR_INVALID_ARGUMENT;
+  *matrix_core = igak::block_cfg(cfg).mfma;
   return IGA_OK;
 }
 

@@ -245,6 +245,7 @@ def _lib():
         L.iga_op_apply_backward.argtypes = [VP, VP, VP, VP, VP, VP]
         L.iga_op_apply_block.argtypes = [VP, C.c_int, C.c_int, VP, VP, VP]
         L.iga_block_config_count.argtypes = []
+        L.iga_block_config_matrix_core.argtypes = [C.c_int, C.POINTER(C.c_int)]
         L.iga_block_config_desc.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                             C.POINTER(C.c_int)]
         L.iga_op_set_block_config.argtypes = [VP, C.c_int, C.c_int]
@@ -629,6 +630,13 @@ def block_configs():
         check(L.iga_block_config_desc(i, C.byref(a), C.byref(b), C.byref(c)), "block_config_desc")
         out.append((a.value, b.value, c.value))
     return out
+
+
+def block_config_matrix_core(cfg):
+    """Whether block configuration cfg runs on the matrix cores."""
+    v = C.c_int()
+    check(_lib().iga_block_config_matrix_core(cfg, C.byref(v)), "block_config_matrix_core")
+    return bool(v.value)
 
 
 # --------------------------------------------------------------------------- FEM-PC
