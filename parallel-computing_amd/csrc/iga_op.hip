@@ -207,6 +207,11 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   a.ntx = L.ntx;
   a.nty = L.nty;
   a.mode = gather ? 1 : 0;
+  if (gather && hook && hook->defer_seam) {
+    if (!hook->slots) return IGA_ERR_INVALID_ARGUMENT;
+    a.mode = 3;  // seam partials into the consumer's function slots
+    a.slots = hook->slots;
+  }
   a.mask_in = mask_in;
   a.mask_out = mask_out;
   a.skip = skip;
@@ -247,21 +252,15 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   return IGA_OK;
 }
 
-SeamGeom seam_geom(const iga_op *op, bool f64) {
+SeamGeom seam_geom(const iga_op *op, bool f64, const void *slots) {
   const igak::Layout &L = f64 ? op->L64 : op->L32;
   SeamGeom g{};
-  g.halo = f64 ? (const void *)op->halo64 : (const void *)op->halo32;
-  g.dirmask = op->dirmask;
+  g.slots = slots;
+  g.n = op->nfunc;
   g.nfx = op->nfx;
-  g.nfy = op->nfy;
-  g.nex = op->nex;
   g.ney = op->ney;
   g.p = op->p;
-  g.sx = __builtin_ctz((unsigned)L.tx);
   g.sy = __builtin_ctz((unsigned)L.ty);
-  g.fx = L.tx + op->p;
-  g.fxy = g.fx * (L.ty + op->p);
-  g.ntx = L.ntx;
   g.inv_nfx = 1.0 / op->nfx;
   return g;
 }

@@ -14,8 +14,12 @@
 //   4. functions touched by this tile only are stored; functions on a tile
 //      seam are either atomically added (IEN scatter-add, y pre-zeroed on the
 //      seams) or parked in a per-tile halo slot and summed in a fixed tile
-//      order by k_seam or by the CG update kernel that consumes y
-//      (inverse-LM gather, bitwise reproducible).
+//      order by k_seam (inverse-LM gather, bitwise reproducible); inside the
+//      CG iterations the gather parks them instead in function-indexed slot
+//      arrays, one per position of the tile among the (up to four) tiles
+//      touching the function, and the update kernel that consumes y sums
+//      y + slot1 + slot2 + slot3 in k_seam's order (slots no tile writes
+//      stay zero).
 // Dirichlet rows are written as zero; x is gathered for every function
 // (matrix_free/src/GlobalAssemblyMF.cpp:166-205).
 #pragma once
@@ -48,7 +52,9 @@ struct ApplyArgs {
   void *halo;             // [tile][(TX+P)*(TY+P)]
   const void *wf;         // per-function NURBS weights (general rational meshes) or nullptr
   int nex, ney, nfx, nfy, ntx, nty;
-  int mode;               // 0 scatter-add, 1 gather
+  int mode;               // 0 scatter-add, 1 gather (halo), 3 gather into function slots
+  void *slots;            // mode 3: [3][nfx*nfy] partials of the x-, y- and diagonal
+                          // neighbour of the lowest touching tile (the lowest writes y)
   int mask_in;            // treat x as P_int x
   int mask_out;           // zero Dirichlet rows of y
   const int *skip;        // device flag: nonzero = do nothing (converged solve)
@@ -463,11 +469,42 @@ struct TileStage {
 // --------------------------------------------------------------------------
 // footprint reduction and store
 // --------------------------------------------------------------------------
+// mode 3: the store of one footprint value v into y (functions this tile
+// alone touches, and the lowest touching tile's partial of a seam function)
+// or into the slot array of this tile's position among the touching tiles
+// (1 the x-, 2 the y-, 3 the diagonal neighbour of the lowest).  Every
+// touching tile writes its slot on every apply, Dirichlet rows included
+// (as zeros), so a slot holds either this apply's partial or the zero it
+// was created with.
+template <typename T, int P, int TX, int TY, bool RAT>
+__device__ __forceinline__ void put_footprint_slots(const ApplyArgs &A, T *y, const T *wf,
+                                                    int ex0, int ey0, int fx, int fy, T v,
+                                                    T xin, double &dacc, bool interior) {
+  const int gx = ex0 + fx, gy = ey0 + fy;
+  const int Ag = gy * A.nfx + gx;
+  int k;
+  if (interior) {  // first P columns / rows: this tile is the upper neighbour
+    k = (fx < P ? 1 : 0) + (fy < P ? 2 : 0);
+  } else {
+    if (gx >= A.nfx || gy >= A.nfy) return;
+    int xlo, xhi, ylo, yhi;
+    touching_tiles<P>(gx, A.nex, TX, xlo, xhi);
+    touching_tiles<P>(gy, A.ney, TY, ylo, yhi);
+    k = (ex0 / TX > xlo ? 1 : 0) + (ey0 / TY > ylo ? 2 : 0);
+    if (A.mask_out && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) v = T(0), xin = T(0);
+  }
+  if (A.dot) dacc += (double)xin * (double)v;
+  if (RAT) v *= wf[Ag];
+  T *const d = k ? static_cast<T *>(A.slots) + (size_t)(k - 1) * A.nfx * A.nfy : y;
+  d[Ag] = v;
+}
+
 // One footprint value v at (fx, fy) of the tile at (ex0, ey0): Dirichlet rows
 // get 0; functions only this tile touches are stored; seam functions are
 // atomically added (mode 0, after k_seam zeroed them) or written to the
-// tile's halo slot fy*FX+fx (mode 1, summed by k_seam).  General rational
-// meshes scale by the function's weight.
+// tile's halo slot fy*FX+fx (mode 1, summed by k_seam), or go to the
+// function slots (mode 3, put_footprint_slots).  General rational meshes
+// scale by the function's weight.
 // xin: the staged input at this footprint point (x, or w x for general
 // weights), accumulated into dacc as xin * v when A.dot is set.
 // interior: the tile has neighbours on all four sides and the Dirichlet set
@@ -478,6 +515,10 @@ template <typename T, int P, int TX, int TY, bool RAT>
 __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo, const T *wf,
                                               int tile, int ex0, int ey0, int fx, int fy, T v,
                                               T xin, double &dacc, bool interior) {
+  if (A.mode == 3) {
+    put_footprint_slots<T, P, TX, TY, RAT>(A, y, wf, ex0, ey0, fx, fy, v, xin, dacc, interior);
+    return;
+  }
   constexpr int FX = TX + P;
   const int gx = ex0 + fx, gy = ey0 + fy;
   if (interior) {
@@ -688,7 +729,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   // form the shorter store path measured slower, 123 against 85 us for the
   // FP64 apply of p = 3 on 1024 x 1024 elements)
   const bool full = ex0 + TX <= A.nex && ey0 + TY <= A.ney;
-  const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
+  const bool interior = (A.mode == 1 || A.mode == 3) && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
                         (!A.mask_out || !A.dirmask);
 
   // x footprint and tables first, then the geometry factors of this element
@@ -867,7 +908,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_mfma(ApplyArgs A) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % A.ntx, ty = tile / A.ntx;
   const int ex0 = tx * TX, ey0 = ty * TY;
-  const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
+  const bool interior = (A.mode == 1 || A.mode == 3) && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
                         (!A.mask_out || !A.dirmask);
 
   // x footprint and tables, then this lane's G_q values of its four rows:
@@ -1033,7 +1074,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
     const int tx = tile % A.ntx, ty = tile / A.ntx;
     const int ex0 = tx * TX, ey0 = ty * TY;
     const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
-    const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 &&
+    const bool interior = (A.mode == 1 || A.mode == 3) && tx > 0 && ty > 0 && tx < A.ntx - 1 &&
                           ty < A.nty - 1 && (!A.mask_out || !A.dirmask);
     __syncthreads();  // previous tile's LDS reads are done
     stn.store(A, ex0, ey0, t, xs, tbx, tby);

@@ -83,92 +83,65 @@ int dev_alloc(iga_op *op, void **p, size_t bytes);
 int set_device(int dev);
 // CG fusion of an apply (see igak::ApplyArgs): per-tile partials of x.y into
 // dot[ntiles], and the hand-off copy of hn doubles hsrc -> hdst
-// defer_seam (gather assembly): leave the seam functions of y unwritten;
-// the consumer of y sums them (seam_load, seam_value)
+// defer_seam (gather assembly): seam partials into function slots, summed
+// by the consumer of y (seam_load, seam_value)
 struct CgHook {
   double *dot;
   const double *hsrc;
   double *hdst;
   int hn;
+  // gather assembly: leave the seam partials in the function-indexed slot
+  // arrays `slots` ([3][nfunc], zero wherever no tile writes) for the
+  // consumer of y (seam_load, seam_value), with no seam kernel
   bool defer_seam;
+  void *slots = nullptr;
 };
 // What a consumer of a defer_seam gather apply needs to sum the seam
-// functions itself: the per-tile halo slots ([tile][(TX+P)*(TY+P)], tile
-// row-major), the Dirichlet set and the tile layout of that precision.
+// functions itself: the slot arrays and the tile rows of that precision.
 struct SeamGeom {
-  const void *halo;
-  const uint8_t *dirmask;  // nullptr: the boundary ring
-  int nfx, nfy, nex, ney, p;
-  int sx, sy;              // tiles of 2^sx x 2^sy elements
-  int fx, fxy, ntx;        // footprint row length, slots per tile, tiles per row
+  const void *slots;  // [3][n]: x-, y- and diagonal neighbour partials
+  int64_t n;          // functions
+  int nfx, ney, p;
+  int sy;             // tiles of 2^sy element rows
   double inv_nfx;
 };
-SeamGeom seam_geom(const iga_op *op, bool f64);
+SeamGeom seam_geom(const iga_op *op, bool f64, const void *slots);
 
 // The further partials of function i (see seam_load)
 template <typename V>
 struct SeamPart {
   V a1, a2, a3;
-  unsigned f;   // SP_SEAM, SP_X, SP_Y, SP_DIR
-  unsigned dm;  // the Dirichlet mask byte (explicit masks)
 };
-enum { SP_SEAM = 1, SP_X = 2, SP_Y = 4, SP_DIR = 8 };
 
-// Loads of y at function i after a defer_seam gather apply: y[i] for a
-// function one tile touches, else the halo slot of the lowest touching tile
-// (returned) and the slots of the others in s (tile row, then tile column).
-// The addresses come from i alone, so all loads of a step are in flight
-// together; within a wave the loads are unconditional (an absent partial
-// reloads the first address, which the same cache line serves), and a wave
-// with no function on a horizontal seam skips the two loads only those
-// need (wave vote: most waves of a row-major sweep).  32-bit function
-// indices.
+// Loads of y at function i after a defer_seam gather apply: y[i] (the whole
+// value, or the lowest touching tile's partial) and the three slot partials,
+// which are zero where the function has no such neighbour tile.  All loads
+// are coalesced; the y- and diagonal-neighbour slots are loaded only by a
+// wave holding a function on a horizontal tile seam (wave vote: most waves
+// of a row-major sweep hold none, and their slots are zero).
 template <typename V>
-__device__ __forceinline__ V seam_load(const V *__restrict__ y, const SeamGeom &g, int i,
+__device__ __forceinline__ V seam_load(const V *__restrict__ y, const SeamGeom &g, int64_t i,
                                        SeamPart<V> &s) {
+  const V *S = static_cast<const V *>(g.slots);
   int gy = (int)((double)i * g.inv_nfx);  // within one of i / nfx
-  int gx = i - gy * g.nfx;
-  const bool under = gx < 0, over = gx >= g.nfx;
-  gy += over ? 1 : (under ? -1 : 0);
-  gx += under ? g.nfx : (over ? -g.nfx : 0);
-  const int xlo = max(gx - g.p, 0) >> g.sx, xhi = min(gx, g.nex - 1) >> g.sx;
-  const int ylo = max(gy - g.p, 0) >> g.sy, yhi = min(gy, g.ney - 1) >> g.sy;
-  const bool shx = xlo != xhi, shy = ylo != yhi, sh = shx || shy;
-  const V *h = static_cast<const V *>(g.halo);
-  const int64_t base = (int64_t)(ylo * g.ntx + xlo) * g.fxy + (gy - (ylo << g.sy)) * g.fx +
-                       (gx - (xlo << g.sx));
-  const int64_t dx = g.fxy - (1 << g.sx);                                   // next tile in x
-  const int64_t dy = (int64_t)g.ntx * g.fxy - (int64_t)g.fx * (1 << g.sy);  // next in y
-  const V *p0 = sh ? h + base : y + i;
-  const V *p1 = shx ? h + base + dx : p0;
-  const V *p2 = shy ? h + base + dy : p0;
-  const V *p3 = (shx && shy) ? h + base + dx + dy : p0;
-  const V a0 = *p0;
-  s.a1 = *p1;
+  const int gx = (int)(i - (int64_t)gy * g.nfx);
+  gy += gx >= g.nfx ? 1 : (gx < 0 ? -1 : 0);
+  const bool shy = (max(gy - g.p, 0) >> g.sy) != (min(gy, g.ney - 1) >> g.sy);
+  const V a0 = y[i];
+  s.a1 = S[i];
   s.a2 = s.a3 = V(0);
   if (__any(shy)) {
-    s.a2 = *p2;
-    s.a3 = *p3;
+    s.a2 = S[g.n + i];
+    s.a3 = S[2 * g.n + i];
   }
-  // Dirichlet: the ring from the coordinates, an explicit mask by a load
-  // whose value is first needed in seam_value
-  const bool ring_dir = gx == 0 || gy == 0 || gx == g.nfx - 1 || gy == g.nfy - 1;
-  s.f = (sh ? SP_SEAM : 0u) | (shx ? SP_X : 0u) | (shy ? SP_Y : 0u) |
-        (!g.dirmask && ring_dir ? SP_DIR : 0u);
-  s.dm = g.dirmask ? (unsigned)g.dirmask[i] : 0u;
   return a0;
 }
 
-// y at function i from what seam_load returned: bitwise the value k_seam
-// stores (an absent partial adds +0, which leaves the sum unchanged)
+// y at function i from what seam_load returned: the sum in k_seam's order
+// (lowest tile, x-, y-, diagonal neighbour; an absent partial adds +0)
 template <typename V>
 __device__ __forceinline__ V seam_value(V a0, const SeamPart<V> &s) {
-  V acc = V(0) + a0;
-  acc += (s.f & SP_X) ? s.a1 : V(0);
-  acc += (s.f & SP_Y) ? s.a2 : V(0);
-  acc += ((s.f & SP_X) && (s.f & SP_Y)) ? s.a3 : V(0);
-  acc = ((s.f & SP_DIR) || s.dm) ? V(0) : acc;
-  return (s.f & SP_SEAM) ? acc : a0;
+  return ((a0 + s.a1) + s.a2) + s.a3;
 }
 // whether the apply kernel configured for this precision supports CgHook
 bool cg_fusable(const iga_op *op, bool f64, bool ebe);
