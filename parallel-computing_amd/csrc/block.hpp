@@ -116,9 +116,9 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
       }
     }
     double dnone = 0.0;
-    reduce_store_row16<float, P, TY, false>(oa, ye, red[r & 1], t, 0, ex0, ey0,
-                                            A.Y + (size_t)(j0 + r) * A.ld, nullptr, nullptr,
-                                            xs, dnone, interior);
+    reduce_store_row16<float, P, TY, false>(oa, ye, red[r & 1], t, ex0, ey0,
+                                            A.Y + (size_t)(j0 + r) * A.ld, nullptr, xs, dnone,
+                                            interior);
   }
 }
 
@@ -313,9 +313,9 @@ __global__ __launch_bounds__(256, 2) void k_apply_block_mfma(BlockArgs A) {
   for (int i = t; i < RC * NFP; i += NT) {
     const int jr = i / NFP, kk = i % NFP;
     if (j0 + jr >= A.k) continue;
-    put_footprint<float, P, TX, TY, false>(oa, A.Y + (size_t)(j0 + jr) * A.ld, nullptr, nullptr,
-                                           0, ex0, ey0, kk % FX, kk / FX, ys[jr][kk], 0.f,
-                                           dnone, interior);
+    put_footprint<float, P, TX, TY, false>(oa, A.Y + (size_t)(j0 + jr) * A.ld, nullptr, ex0,
+                                           ey0, kk % FX, kk / FX, ys[jr][kk], 0.f, dnone,
+                                           interior);
   }
 }
 

@@ -170,6 +170,33 @@ int apply_tiles(const iga_op *op, bool f64) {
   return L.ntx * L.nty;
 }
 
+namespace {
+constexpr int FOLD_U = 4;  // functions per thread of k_slot_fold
+// The gather's seam pass: y[i] = ((y[i] + s1[i]) + s2[i]) + s3[i] over the
+// function slots (seam_load, seam_value), stored only where a slot holds a
+// partial (the other functions' y is already whole).  Coalesced; out-of-range
+// lanes load the last function so that the wave votes in seam_load see them.
+template <typename T>
+__global__ __launch_bounds__(256) void k_slot_fold(T *__restrict__ y, SeamGeom g,
+                                                   const int *skip) {
+  if (skip && *skip) return;
+  const int64_t b = (int64_t)blockIdx.x * 256 * FOLD_U + threadIdx.x;
+  T a0[FOLD_U];
+  SeamPart<T> sp[FOLD_U];
+#pragma unroll
+  for (int k = 0; k < FOLD_U; ++k) {
+    const int64_t i = b + (int64_t)k * 256;
+    a0[k] = seam_load(y, g, i < g.n ? i : g.n - 1, sp[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < FOLD_U; ++k) {
+    const int64_t i = b + (int64_t)k * 256;
+    if (i < g.n && (sp[k].a1 != T(0) || sp[k].a2 != T(0) || sp[k].a3 != T(0)))
+      y[i] = seam_value(a0[k], sp[k]);
+  }
+}
+}  // namespace
+
 int apply_raw(iga_op *op, bool f64, bool ebe, bool gather, const void *x, void *y,
               int mask_in, int mask_out, hipStream_t s, const int *skip, bool y_zeroed,
               const CgHook *hook) {
@@ -198,7 +225,6 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   a.tabx = f64 ? (const void *)op->tabx64 : (const void *)op->tabx32;
   a.taby = f64 ? (const void *)op->taby64 : (const void *)op->taby32;
   a.dirmask = op->dirmask;
-  a.halo = f64 ? (void *)op->halo64 : (void *)op->halo32;
   a.wf = f64 ? (const void *)op->wts64 : (const void *)op->wts32;
   a.nex = op->nex;
   a.ney = op->ney;
@@ -207,11 +233,7 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   a.ntx = L.ntx;
   a.nty = L.nty;
   a.mode = gather ? 1 : 0;
-  if (gather && hook && hook->defer_seam) {
-    if (!hook->slots) return IGA_ERR_INVALID_ARGUMENT;
-    a.mode = 3;  // seam partials into the consumer's function slots
-    a.slots = hook->slots;
-  }
+  a.slots = f64 ? (void *)op->slots64 : (void *)op->slots32;
   a.mask_in = mask_in;
   a.mask_out = mask_out;
   a.skip = skip;
@@ -222,11 +244,9 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   const int tile = f64 ? op->tile64 : op->tile32;
   if (!gather && !y_zeroed) {
     if (f64)
-      IGA_HIP_TRY(op->ops->seam64((double *)y, nullptr, op->dirmask, op->nex, op->ney, op->nfx,
-                                  op->nfy, L, 0, mask_out, skip, s));
+      IGA_HIP_TRY(op->ops->seam64((double *)y, op->nex, op->nfx, op->nfy, L, skip, s));
     else
-      IGA_HIP_TRY(op->ops->seam32((float *)y, nullptr, op->dirmask, op->nex, op->ney, op->nfx,
-                                  op->nfy, L, 0, mask_out, skip, s));
+      IGA_HIP_TRY(op->ops->seam32((float *)y, op->nex, op->nfx, op->nfy, L, skip, s));
   }
   const bool prof = op->prof_on && op->prof_used < op->prof_cap;
   if (prof) IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used], s));
@@ -241,21 +261,23 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
     IGA_HIP_TRY(hipEventRecord(op->prof_ev[2 * op->prof_used + 1], s));
     ++op->prof_used;
   }
+  // the gather's seam sums, unless the consumer of y forms them (CG fusion)
   if (gather && !(hook && hook->defer_seam)) {
+    const SeamGeom g = seam_geom(op, f64);
+    const unsigned nb = (unsigned)((op->nfunc + FOLD_U * 256 - 1) / (FOLD_U * 256));
     if (f64)
-      IGA_HIP_TRY(op->ops->seam64((double *)y, op->halo64, op->dirmask, op->nex, op->ney,
-                                  op->nfx, op->nfy, L, 1, mask_out, skip, s));
+      hipLaunchKernelGGL(k_slot_fold<double>, dim3(nb), dim3(256), 0, s, (double *)y, g, skip);
     else
-      IGA_HIP_TRY(op->ops->seam32((float *)y, op->halo32, op->dirmask, op->nex, op->ney,
-                                  op->nfx, op->nfy, L, 1, mask_out, skip, s));
+      hipLaunchKernelGGL(k_slot_fold<float>, dim3(nb), dim3(256), 0, s, (float *)y, g, skip);
+    IGA_HIP_TRY(hipGetLastError());
   }
   return IGA_OK;
 }
 
-SeamGeom seam_geom(const iga_op *op, bool f64, const void *slots) {
+SeamGeom seam_geom(const iga_op *op, bool f64) {
   const igak::Layout &L = f64 ? op->L64 : op->L32;
   SeamGeom g{};
-  g.slots = slots;
+  g.slots = f64 ? (const void *)op->slots64 : (const void *)op->slots32;
   g.n = op->nfunc;
   g.nfx = op->nfx;
   g.ney = op->ney;
@@ -577,7 +599,10 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
     } else {
       op->G0 = op->G64;
     }
-    ALLOC(op->halo64, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 8);
+    // gather slots, zero where no tile ever writes
+    ALLOC(op->slots64, (size_t)3 * op->nfunc * 8);
+    if (hipMemsetAsync(op->slots64, 0, (size_t)3 * op->nfunc * 8, s) != hipSuccess)
+      return fail(IGA_ERR_HIP);
   }
   if (op->want32) {
     const igak::Layout &L = op->L32;
@@ -586,7 +611,9 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
     if (hipSuccess != ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
                                   op->kappa, nex, ney, nfx, L, s))
       return fail(IGA_ERR_HIP);
-    ALLOC(op->halo32, (size_t)L.ntx * L.nty * (L.tx + p) * (L.ty + p) * 4);
+    ALLOC(op->slots32, (size_t)3 * op->nfunc * 4);
+    if (hipMemsetAsync(op->slots32, 0, (size_t)3 * op->nfunc * 4, s) != hipSuccess)
+      return fail(IGA_ERR_HIP);
     if (o.f16_geometry) {
       if (rational == 2 || !igak::g16_compiled(op->tile32) || !ops->apply16)
         return fail(IGA_ERR_UNSUPPORTED);
@@ -831,7 +858,7 @@ int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, vo
   a.tabx = f64 ? (const void *)op->tabx64 : (const void *)op->tabx32;
   a.taby = f64 ? (const void *)op->taby64 : (const void *)op->taby32;
   a.dirmask = op->dirmask;
-  a.halo = f64 ? (void *)op->halo64 : (void *)op->halo32;
+  a.slots = f64 ? (void *)op->slots64 : (void *)op->slots32;
   a.wf = f64 ? (const void *)op->wts64 : (const void *)op->wts32;
   a.nex = op->nex;
   a.ney = op->ney;

@@ -13,13 +13,13 @@
 //      order (P+1 passes, one element-row offset per pass);
 //   4. functions touched by this tile only are stored; functions on a tile
 //      seam are either atomically added (IEN scatter-add, y pre-zeroed on the
-//      seams) or parked in a per-tile halo slot and summed in a fixed tile
-//      order by k_seam (inverse-LM gather, bitwise reproducible); inside the
-//      CG iterations the gather parks them instead in function-indexed slot
-//      arrays, one per position of the tile among the (up to four) tiles
-//      touching the function, and the update kernel that consumes y sums
-//      y + slot1 + slot2 + slot3 in k_seam's order (slots no tile writes
-//      stay zero).
+//      seams) or, in the inverse-LM gather, written to the slot of this
+//      tile's position among the (up to four) tiles touching the function:
+//      the lowest tile's partial into y, the others' into three
+//      function-indexed slot arrays, zero wherever no tile writes.  The sum
+//      y + slot1 + slot2 + slot3, in that fixed order, is formed by
+//      k_slot_fold or by the CG update kernel that consumes y (bitwise
+//      reproducible, no atomics).
 // Dirichlet rows are written as zero; x is gathered for every function
 // (matrix_free/src/GlobalAssemblyMF.cpp:166-205).
 #pragma once
@@ -49,11 +49,10 @@ struct ApplyArgs {
   const void *tabx;       // [nelem_x][2][NQ][P+1] : N, dN/dxi (reference coords)
   const void *taby;       // [nelem_y][2][NQ][P+1]
   const uint8_t *dirmask; // nullptr = Dirichlet on the boundary ring
-  void *halo;             // [tile][(TX+P)*(TY+P)]
   const void *wf;         // per-function NURBS weights (general rational meshes) or nullptr
   int nex, ney, nfx, nfy, ntx, nty;
-  int mode;               // 0 scatter-add, 1 gather (halo), 3 gather into function slots
-  void *slots;            // mode 3: [3][nfx*nfy] partials of the x-, y- and diagonal
+  int mode;               // 0 scatter-add, 1 gather into the function slots
+  void *slots;            // mode 1: [3][nfx*nfy] partials of the x-, y- and diagonal
                           // neighbour of the lowest touching tile (the lowest writes y)
   int mask_in;            // treat x as P_int x
   int mask_out;           // zero Dirichlet rows of y
@@ -469,13 +468,21 @@ struct TileStage {
 // --------------------------------------------------------------------------
 // footprint reduction and store
 // --------------------------------------------------------------------------
-// mode 3: the store of one footprint value v into y (functions this tile
-// alone touches, and the lowest touching tile's partial of a seam function)
-// or into the slot array of this tile's position among the touching tiles
-// (1 the x-, 2 the y-, 3 the diagonal neighbour of the lowest).  Every
-// touching tile writes its slot on every apply, Dirichlet rows included
-// (as zeros), so a slot holds either this apply's partial or the zero it
-// was created with.
+// One footprint value v at (fx, fy) of the tile at (ex0, ey0).  Functions
+// only this tile touches are stored.  Seam functions are atomically added
+// (mode 0, scatter-add, after k_seam zeroed them) or, in the gather (mode 1),
+// written to the slot of this tile's position among the touching tiles: 0
+// the lowest (into y), 1 its x-, 2 its y-, 3 its diagonal neighbour (into
+// slot array k - 1).  Every touching tile writes its slot on every apply,
+// Dirichlet rows included (as zeros), so a slot holds this apply's partial or
+// the zero it was created with.  Dirichlet rows get 0; general rational
+// meshes scale by the function's weight.
+// xin: the staged input at this footprint point (x, or w x for general
+// weights), accumulated into dacc as xin * v when A.dot is set.
+// interior: the tile has neighbours on all four sides and the Dirichlet set
+// is the boundary ring (or rows are not masked), so no footprint function
+// is Dirichlet or outside the mesh and a function is shared exactly when it
+// lies in the first P or the last P footprint columns / rows.
 template <typename T, int P, int TX, int TY, bool RAT>
 __device__ __forceinline__ void put_footprint_slots(const ApplyArgs &A, T *y, const T *wf,
                                                     int ex0, int ey0, int fx, int fy, T v,
@@ -499,27 +506,14 @@ __device__ __forceinline__ void put_footprint_slots(const ApplyArgs &A, T *y, co
   d[Ag] = v;
 }
 
-// One footprint value v at (fx, fy) of the tile at (ex0, ey0): Dirichlet rows
-// get 0; functions only this tile touches are stored; seam functions are
-// atomically added (mode 0, after k_seam zeroed them) or written to the
-// tile's halo slot fy*FX+fx (mode 1, summed by k_seam), or go to the
-// function slots (mode 3, put_footprint_slots).  General rational meshes
-// scale by the function's weight.
-// xin: the staged input at this footprint point (x, or w x for general
-// weights), accumulated into dacc as xin * v when A.dot is set.
-// interior: the tile has neighbours on all four sides and the Dirichlet set
-// is the boundary ring (or rows are not masked), so no footprint function
-// is Dirichlet or outside the mesh and a function is shared exactly when it
-// lies in the first P or the last P footprint columns / rows.
 template <typename T, int P, int TX, int TY, bool RAT>
-__device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo, const T *wf,
-                                              int tile, int ex0, int ey0, int fx, int fy, T v,
-                                              T xin, double &dacc, bool interior) {
-  if (A.mode == 3) {
+__device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, const T *wf, int ex0,
+                                              int ey0, int fx, int fy, T v, T xin, double &dacc,
+                                              bool interior) {
+  if (A.mode != 0) {
     put_footprint_slots<T, P, TX, TY, RAT>(A, y, wf, ex0, ey0, fx, fy, v, xin, dacc, interior);
     return;
   }
-  constexpr int FX = TX + P;
   const int gx = ex0 + fx, gy = ey0 + fy;
   if (interior) {
     const int Ag = gy * A.nfx + gx;
@@ -530,8 +524,6 @@ __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo,
       y[Ag] = v;
     else if (A.mode == 0)
       atomic_add(&y[Ag], v);
-    else
-      halo[(size_t)tile * (FX * (TY + P)) + fy * FX + fx] = v;
     return;
   }
   if (gx >= A.nfx || gy >= A.nfy) return;
@@ -550,8 +542,6 @@ __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, T *halo,
     y[Ag] = v;
   else if (A.mode == 0)
     atomic_add(&y[Ag], v);
-  else
-    halo[(size_t)tile * (FX * (TY + P)) + fy * FX + fx] = v;
 }
 
 // Cross-lane move inside 16-lane DPP rows (zero where the source lane falls
@@ -599,9 +589,9 @@ constexpr int row16_lds() {
 
 template <typename T, int P, int TY, bool RAT>
 __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (&ye)[P + 1][P + 1],
-                                                   T *lds, int t, int tile, int ex0, int ey0,
-                                                   T *y, T *halo, const T *wf, const T *xs,
-                                                   double &dacc, bool interior) {
+                                                   T *lds, int t, int ex0, int ey0, T *y,
+                                                   const T *wf, const T *xs, double &dacc,
+                                                   bool interior) {
   constexpr int FX = 16 + P;
   constexpr int NB = P + 1, NT = 16 * TY, R = (TY + 2 * P) * 16;  // values per b
   const int lx = t & 15, ly = t >> 4;
@@ -633,10 +623,10 @@ __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (
     w0 += bwr[b * R + (ly - b + P) * 16 + lx];
   }
   auto xin = [&](int fx, int fy) { return A.dot ? xs[fy * FX + fx] : T(0); };
-  put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, ly, v0, xin(lx, ly),
+  put_footprint<T, P, 16, TY, RAT>(A, y, wf, ex0, ey0, lx, ly, v0, xin(lx, ly),
                                    dacc, interior);
   if (lx < P)
-    put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, ly, w0,
+    put_footprint<T, P, 16, TY, RAT>(A, y, wf, ex0, ey0, 16 + lx, ly, w0,
                                      xin(16 + lx, ly), dacc, interior);
   if (ly < P) {
     T v1 = T(0), w1 = T(0);
@@ -645,10 +635,10 @@ __device__ __forceinline__ void reduce_store_row16(const ApplyArgs &A, const T (
       v1 += bin[b * R + (TY + ly - b + P) * 16 + lx];
       w1 += bwr[b * R + (TY + ly - b + P) * 16 + lx];
     }
-    put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, lx, TY + ly, v1,
+    put_footprint<T, P, 16, TY, RAT>(A, y, wf, ex0, ey0, lx, TY + ly, v1,
                                      xin(lx, TY + ly), dacc, interior);
     if (lx < P)
-      put_footprint<T, P, 16, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, 16 + lx, TY + ly, w1,
+      put_footprint<T, P, 16, TY, RAT>(A, y, wf, ex0, ey0, 16 + lx, TY + ly, w1,
                                        xin(16 + lx, TY + ly), dacc, interior);
   }
 }
@@ -729,7 +719,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   // form the shorter store path measured slower, 123 against 85 us for the
   // FP64 apply of p = 3 on 1024 x 1024 elements)
   const bool full = ex0 + TX <= A.nex && ey0 + TY <= A.ney;
-  const bool interior = (A.mode == 1 || A.mode == 3) && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
+  const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
                         (!A.mask_out || !A.dirmask);
 
   // x footprint and tables first, then the geometry factors of this element
@@ -741,7 +731,11 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   // whole element (one thread) or this thread's quadrature columns (split)
   constexpr bool PREF =
       IGAK_PREFETCH_G && MINW < 4 && SPLIT == 1 && NC * NQ2 * sizeof(T) <= 320;
-  constexpr bool PREFH = IGAK_PREFETCH_G && SPLIT > 1 && NC * NQ * NI * sizeof(T) <= 400;
+  // (general rational meshes: at most 320 bytes -- the degree-5 FP32 kernel
+  // at 360 bytes held 212 VGPRs, two waves per SIMD, and ran 862 us on the
+  // 2048^2 mesh against 812 us loading each row where it is used, 166 VGPRs)
+  constexpr bool PREFH =
+      IGAK_PREFETCH_G && SPLIT > 1 && NC * NQ * NI * sizeof(T) <= (RAT ? 320 : 400);
   T g[PREF ? NC : 1][PREF ? NQ2 : 1];
   T gh[PREFH ? NC : 1][PREFH ? NQ : 1][PREFH ? NI : 1];
   const TG *Gt = static_cast<const TG *>(A.G);
@@ -804,11 +798,9 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   }
 
   T *y = static_cast<T *>(A.y);
-  T *halo = static_cast<T *>(A.halo);
   double dacc = 0.0;
   if constexpr (ROW16) {
-    reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, tile, ex0, ey0, y, halo, wf, xs, dacc,
-                                      interior);
+    reduce_store_row16<T, P, TY, RAT>(A, ye, bufm, t, ex0, ey0, y, wf, xs, dacc, interior);
     if (A.dot) store_tile_dot<NT>(A, tile, dacc);
     return;
   }
@@ -845,7 +837,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   for (int o = 0; o < NOWN; ++o) {
     const int k = t + o * NT;
     if (k < NFP)
-      put_footprint<T, P, TX, TY, RAT>(A, y, halo, wf, tile, ex0, ey0, k % FX, k / FX, acc[o],
+      put_footprint<T, P, TX, TY, RAT>(A, y, wf, ex0, ey0, k % FX, k / FX, acc[o],
                                        A.dot ? xs[k] : T(0), dacc, interior);
   }
   if (A.dot) store_tile_dot<NT>(A, tile, dacc);
@@ -908,7 +900,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_mfma(ApplyArgs A) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % A.ntx, ty = tile / A.ntx;
   const int ex0 = tx * TX, ey0 = ty * TY;
-  const bool interior = (A.mode == 1 || A.mode == 3) && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
+  const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 && ty < A.nty - 1 &&
                         (!A.mask_out || !A.dirmask);
 
   // x footprint and tables, then this lane's G_q values of its four rows:
@@ -1011,7 +1003,6 @@ __global__ __launch_bounds__(256, 4) void k_apply_mfma(ApplyArgs A) {
   __syncthreads();
 
   float *y = static_cast<float *>(A.y);
-  float *halo = static_cast<float *>(A.halo);
   double dacc = 0.0;
 #pragma unroll
   for (int o = 0; o < NOWN; ++o) {
@@ -1030,7 +1021,7 @@ __global__ __launch_bounds__(256, 4) void k_apply_mfma(ApplyArgs A) {
           v += in ? u : 0.f;
         }
       }
-      put_footprint<float, P, TX, TY, false>(A, y, halo, nullptr, tile, ex0, ey0, fx, fy, v,
+      put_footprint<float, P, TX, TY, false>(A, y, nullptr, ex0, ey0, fx, fy, v,
                                              A.dot ? xs[k] : 0.f, dacc, interior);
     }
   }
@@ -1056,7 +1047,6 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
   const int t = threadIdx.x, lx = t % TX, ly = t / TX;
   const T *Gp = static_cast<const T *>(A.G);
   T *y = static_cast<T *>(A.y);
-  T *halo = static_cast<T *>(A.halo);
 
   // this workgroup's tiles: start, start + G, ... (start XCD-contiguous)
   const int G = gridDim.x;
@@ -1074,7 +1064,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
     const int tx = tile % A.ntx, ty = tile / A.ntx;
     const int ex0 = tx * TX, ey0 = ty * TY;
     const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
-    const bool interior = (A.mode == 1 || A.mode == 3) && tx > 0 && ty > 0 && tx < A.ntx - 1 &&
+    const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 &&
                           ty < A.nty - 1 && (!A.mask_out || !A.dirmask);
     __syncthreads();  // previous tile's LDS reads are done
     stn.store(A, ex0, ey0, t, xs, tbx, tby);
@@ -1105,8 +1095,8 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
     }
     double dnone = 0.0;  // A.dot is null here (no CG fusion in this variant)
     if constexpr (TX == 16) {
-      reduce_store_row16<T, P, TY, false>(A, ye, bufm, t, tile, ex0, ey0, y, halo, nullptr, xs,
-                                          dnone, interior);
+      reduce_store_row16<T, P, TY, false>(A, ye, bufm, t, ex0, ey0, y, nullptr, xs, dnone,
+                                          interior);
       continue;
     }
     T acc[NOWN];
@@ -1138,22 +1128,20 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
     for (int o = 0; o < NOWN; ++o) {
       const int k = t + o * NT;
       if (k < NFP)
-        put_footprint<T, P, TX, TY, false>(A, y, halo, nullptr, tile, ex0, ey0, k % FX, k / FX,
+        put_footprint<T, P, TX, TY, false>(A, y, nullptr, ex0, ey0, k % FX, k / FX,
                                            acc[o], T(0), dnone, interior);
     }
   }
 }
 
-// Seam functions (touched by more than one tile).  mode 0: zero them before
-// a scatter-add apply; mode 1: sum the halo slots of the touching tiles in
-// (tile row, tile column) order.  One thread per seam function, numbered
-// row by row within each band of P columns right of a vertical tile
-// boundary (so a wavefront covers short runs of a row), then the P rows
-// below each horizontal boundary (functions not on a vertical seam).  32-bit
-// indices; tile sizes are powers of two (shifts sx, sy).
+// Seam functions (touched by more than one tile): zero them before a
+// scatter-add apply.  One thread per seam function, numbered row by row
+// within each band of P columns right of a vertical tile boundary (so a
+// wavefront covers short runs of a row), then the P rows below each
+// horizontal boundary (functions not on a vertical seam).  32-bit indices;
+// tile sizes are powers of two (shifts sx, sy).
 template <typename T, int P>
-__global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex, int ney, int nfx,
-                       int nfy, int sx, int sy, int ntx, int nty, int mode, int mask_out,
+__global__ void k_seam(T *y, int nex, int nfx, int nfy, int sx, int sy, int ntx, int nty,
                        const int *skip) {
   if (skip && *skip) return;
   const int TX = 1 << sx, TY = 1 << sy;
@@ -1178,22 +1166,7 @@ __global__ void k_seam(T *y, const T *halo, const uint8_t *dirmask, int nex, int
     touching_tiles_sh<P>(gx, nex, sx, xlo, xhi);
     if (xlo != xhi) return;  // owned by the vertical-band pass
   }
-  const int Ag = gy * nfx + gx;
-  if (mode == 0 || (mask_out && is_dirichlet(dirmask, Ag, gx, gy, nfx, nfy))) {
-    y[Ag] = T(0);
-    return;
-  }
-  const int FX = TX + P, FY = TY + P;
-  int xlo, xhi, ylo, yhi;
-  touching_tiles_sh<P>(gx, nex, sx, xlo, xhi);
-  touching_tiles_sh<P>(gy, ney, sy, ylo, yhi);
-  T acc = T(0);
-  for (int ty = ylo; ty <= yhi; ++ty)
-    for (int tx = xlo; tx <= xhi; ++tx) {
-      const int k = (gy - ty * TY) * FX + (gx - tx * TX);
-      acc += halo[(size_t)(ty * ntx + tx) * FX * FY + k];
-    }
-  y[Ag] = acc;
+  y[gy * nfx + gx] = T(0);
 }
 
 // FP16 copy of FP32 G_q (same tile layout, 8 values per 16-byte vector):

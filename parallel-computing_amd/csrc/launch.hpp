@@ -147,15 +147,14 @@ struct Launch {
   }
 
   template <typename T>
-  static hipError_t seam(T *y, const T *halo, const uint8_t *mask, int nex, int ney, int nfx,
-                         int nfy, const Layout &L, int mode, int mask_out, const int *skip,
+  static hipError_t seam(T *y, int nex, int nfx, int nfy, const Layout &L, const int *skip,
                          hipStream_t s) {
     const int n = (L.ntx - 1) * P * nfy + (L.nty - 1) * P * nfx;
     if (n == 0) return hipSuccess;
     const int bs = 256;
     const int sx = __builtin_ctz((unsigned)L.tx), sy = __builtin_ctz((unsigned)L.ty);
-    hipLaunchKernelGGL((k_seam<T, P>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, y, halo, mask,
-                       nex, ney, nfx, nfy, sx, sy, L.ntx, L.nty, mode, mask_out, skip);
+    hipLaunchKernelGGL((k_seam<T, P>), dim3((n + bs - 1) / bs), dim3(bs), 0, s, y, nex, nfx,
+                       nfy, sx, sy, L.ntx, L.nty, skip);
     return hipGetLastError();
   }
 
@@ -249,10 +248,9 @@ struct Ops {
                         hipStream_t);
   hipError_t (*apply32)(const ApplyArgs &, bool ebe, int tile, int ntiles, int variant,
                         hipStream_t);
-  hipError_t (*seam64)(double *, const double *, const uint8_t *, int, int, int, int,
-                       const Layout &, int, int, const int *, hipStream_t);
-  hipError_t (*seam32)(float *, const float *, const uint8_t *, int, int, int, int,
-                       const Layout &, int, int, const int *, hipStream_t);
+  // zero the seam functions before a scatter-add apply
+  hipError_t (*seam64)(double *, int, int, int, const Layout &, const int *, hipStream_t);
+  hipError_t (*seam32)(float *, int, int, int, const Layout &, const int *, hipStream_t);
   hipError_t (*diag)(double *, const double *, const double *, const double *, const double *,
                      const uint8_t *, int, int, int, int, const Layout &, hipStream_t);
   hipError_t (*geom64)(double *, const double *, const double *, const double *,

@@ -46,8 +46,10 @@ struct iga_op {
   float g16_scale = 1.0f;
   unsigned *g16max = nullptr;       // device word: bits of max |G32| (FP16 scale)
   double *kappa = nullptr;          // device [E][nqp] or nullptr
-  double *halo64 = nullptr;
-  float *halo32 = nullptr;
+  // gather assembly: seam partials of the x-, y- and diagonal neighbour tile,
+  // [3][nfunc] per precision, zero wherever no tile writes (see k_apply)
+  double *slots64 = nullptr;
+  float *slots32 = nullptr;
   double *fJ = nullptr;             // load-vector scratch [E][nqp]
   double *tdot = nullptr;           // per-tile x.y partials (iga_op_apply_dot)
   bool want64 = true, want32 = true;
@@ -90,11 +92,9 @@ struct CgHook {
   const double *hsrc;
   double *hdst;
   int hn;
-  // gather assembly: leave the seam partials in the function-indexed slot
-  // arrays `slots` ([3][nfunc], zero wherever no tile writes) for the
-  // consumer of y (seam_load, seam_value), with no seam kernel
+  // gather assembly: leave the seam partials in the operator's slot arrays
+  // for the consumer of y (seam_load, seam_value), with no seam pass
   bool defer_seam;
-  void *slots = nullptr;
 };
 // What a consumer of a defer_seam gather apply needs to sum the seam
 // functions itself: the slot arrays and the tile rows of that precision.
@@ -105,7 +105,7 @@ struct SeamGeom {
   int sy;             // tiles of 2^sy element rows
   double inv_nfx;
 };
-SeamGeom seam_geom(const iga_op *op, bool f64, const void *slots);
+SeamGeom seam_geom(const iga_op *op, bool f64);
 
 // The further partials of function i (see seam_load)
 template <typename V>

@@ -1051,10 +1051,6 @@ struct iga_solver {
   double *Au = nullptr, *dinv = nullptr;  // Jacobi of A (inverse diagonal)
   float *dinv32 = nullptr;                // the same, rounded, for the FP32 inner PCG
   double *x64 = nullptr;                  // FP64 PCG iterate (owned: graphs capture it)
-  // gather assembly inside the iterations: the apply's seam partials by
-  // function ([3][n], zero wherever no tile writes; FP64 solve, FP32 inner)
-  double *slots64 = nullptr;
-  float *slots32 = nullptr;
   // FP32 inner state
   double *d64 = nullptr;  // inner iterate (FP64: see k_cg_init)
   float *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr, *z32 = nullptr, *rold32 = nullptr;
@@ -1226,10 +1222,8 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   const bool fdot = s->o.fused == 1 && s->tdot && igai::cg_fusable_dt(s->op, dt, ebe);
   // gather form with the fused p.Ap: k_cg_update sums the seam functions of
   // Ap itself (no seam kernel, no pass over y)
-  void *const slots = f64 ? (void *)s->slots64 : (void *)s->slots32;
-  const bool fseam = fdot && gather && slots;
-  igai::CgHook hook{s->tdot, nullptr, nullptr, 0, fseam};
-  hook.slots = slots;
+  const bool fseam = fdot && gather;
+  const igai::CgHook hook{s->tdot, nullptr, nullptr, 0, fseam};
   const double *tdot = fdot ? s->tdot : nullptr;
   const int ntiles = f64 ? s->ntiles64 : s->ntiles32;
   // atomic form: Ap was zeroed by the previous k_cg_pupdate (or cg_run)
@@ -1242,7 +1236,7 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
   const V *dinv = s->o.pc != IGA_PC_JACOBI
                       ? nullptr
                       : (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32);
-  const igai::SeamGeom sg = fseam ? igai::seam_geom(s->op, f64, slots) : igai::SeamGeom{};
+  const igai::SeamGeom sg = fseam ? igai::seam_geom(s->op, f64) : igai::SeamGeom{};
   if (loc && fseam)
     LAUNCH((k_cg_update<V, V, true, false, TU, true>), nb, u, r, p, Ap, z, (V *)nullptr, dinv, 0, n,
            s->part, s->sc, s->ic, nb, (const int *)nullptr, (int64_t)0, tdot, ntiles, sg);
@@ -1445,9 +1439,7 @@ int cgf_iteration(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w,
                   hipStream_t st) {
   const bool gather = s->o.assembly == IGA_ASSEMBLY_INVLM_GATHER;
   const bool ebe = s->o.kernel == IGA_KERNEL_EBE;
-  void *const slots = f64 ? (void *)s->slots64 : (void *)s->slots32;
   igai::CgHook hook{s->tdot, s->cgs + CG_L, s->cgs, CG_L, gather};
-  hook.slots = slots;
   // scatter-add form: w was cleared by the previous update (or cgf_run)
   int rc = igai::apply_raw_dt(s->op, apply_dt(s, f64), ebe, gather, u, w, 0, 1, st,
                               s->ic + I_DONE, /*y_zeroed=*/!gather, &hook);
@@ -1456,7 +1448,7 @@ int cgf_iteration(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w,
   const V *dinv = s->o.pc == IGA_PC_JACOBI
                       ? (f64 ? (const V *)(const void *)s->dinv : (const V *)(const void *)s->dinv32)
                       : nullptr;
-  const igai::SeamGeom sg = gather ? igai::seam_geom(s->op, f64, slots) : igai::SeamGeom{};
+  const igai::SeamGeom sg = gather ? igai::seam_geom(s->op, f64) : igai::SeamGeom{};
   const int zw = gather ? 0 : 1;
   if (dinv && gather)
     LAUNCH((k_cgf_update<V, V, true, TU, true>), s->nb, x, r, p, sv, u, w, dinv, zw, s->n, s->cgs,
@@ -1753,16 +1745,6 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     s->ntiles32 = igai::apply_tiles(op, false);
     SA(s->tdot, (size_t)std::max(s->ntiles64, s->ntiles32) * 8);
     if (o.fused == 2 && !fem_pc_kind) SA(s->cgs, (size_t)2 * CG_L * 8);
-    if (o.assembly == IGA_ASSEMBLY_INVLM_GATHER) {
-      // zero once: only the tiles at a function's slot position ever write it
-      if (o.kind == IGA_SOLVE_PCG_F64) {
-        SA(s->slots64, 3 * n * 8);
-        if (hipMemset(s->slots64, 0, 3 * n * 8) != hipSuccess) return fail(IGA_ERR_HIP);
-      } else {
-        SA(s->slots32, 3 * n * 4);
-        if (hipMemset(s->slots32, 0, 3 * n * 4) != hipSuccess) return fail(IGA_ERR_HIP);
-      }
-    }
   }
   if (hipMemset(s->ic, 0, 2 * I_NI * 4) != hipSuccess || hipMemset(s->sc, 0, 64 * 8) != hipSuccess)
     return fail(IGA_ERR_HIP);
