@@ -424,10 +424,18 @@ def solves_mg(out, mesh, op, reps=3):
 
 def fempc_iterations(out):
     """Outer iteration counts of the FEM-preconditioned solve on the unit
-    square, p = 3, f = x(1-x)y(1-y), rtol 1e-10, beside the reference's logged
-    counts (matrix_free/data/fempc_mf.txt:12,24,36: 35, 33, 28; its outer KSP
-    type is not recorded): the reference's preconditioner (inner CG + Jacobi
-    on the Q1 matrix to 1e-10, flexible outer CG) and one V-cycle."""
+    square, p = 3, f = x(1-x)y(1-y), beside the reference's logged counts
+    (matrix_free/data/fempc_mf.txt:12,24,36: 35, 33, 28).  First the
+    reference's own solver semantics, driven through this library's
+    MATSHELL and PCSHELL entry points: PETSc's default outer KSPGMRES
+    (igamf.ksp_gmres: left preconditioning, preconditioned norm, rtol =
+    abstol = 1e-10) around Operator.apply, with Solver.apply_pc running the
+    inner CG + Jacobi on the Q1 matrix to rtol = abstol = 1e-10.  Then this
+    library's solver (stop on the unpreconditioned relative residual 1e-10):
+    the reference's preconditioner (flexible outer CG), one V-cycle, and
+    Jacobi."""
+    import time as _t
+    import torch
     import igamf
     ref = {32: 35, 64: 33, 128: 28}
     rows = {}
@@ -438,6 +446,17 @@ def fempc_iterations(out):
         fem = igamf.FemPC(n + p, n + p, igamf.fem_reference_nodes(p, p, n, n))
         F = op.load(igamf.LOAD_REF_POLY, 0.0)
         row = {"reference_logged": ref[n]}
+        spc = igamf.Solver(op, fem, pc=igamf.PC_FEM_CG, pc_inner_rtol=1e-10,
+                           pc_inner_abstol=1e-10, pc_inner_maxit=400)
+        torch.cuda.synchronize()
+        t0 = _t.perf_counter()
+        u, its = igamf.ksp_gmres(lambda v: op.apply(v.contiguous()), F,
+                                 lambda v: spc.apply_pc(v.contiguous()))
+        torch.cuda.synchronize()
+        r = F - op.apply(u)
+        row["ksp_gmres_reference_semantics"] = {
+            "iterations": its, "solve_ms": round((_t.perf_counter() - t0) * 1e3, 2),
+            "rel_residual": (r.norm() / F.norm()).item()}
         # the inner CG is unrolled into pc_inner_maxit iterations per outer
         # one (each returning at once after convergence): launched eagerly,
         # not captured, at this length
@@ -452,9 +471,11 @@ def fempc_iterations(out):
                          "rel_residual": st.rel_residual}
         rows[f"{n}x{n}"] = row
     out["fempc_iterations"] = rows
-    log("fem-pc iterations (ours inner-CG / V-cycle / jacobi vs reference): " + ", ".join(
-        f"{k}: {v['inner_cg_1e-10']['iterations']}/{v['mg_vcycle']['iterations']}/"
-        f"{v['jacobi']['iterations']} vs {v['reference_logged']}" for k, v in rows.items()))
+    log("fem-pc iterations (reference KSP semantics / ours inner-CG / V-cycle / jacobi vs "
+        "reference log): " + ", ".join(
+            f"{k}: {v['ksp_gmres_reference_semantics']['iterations']}/"
+            f"{v['inner_cg_1e-10']['iterations']}/{v['mg_vcycle']['iterations']}/"
+            f"{v['jacobi']['iterations']} vs {v['reference_logged']}" for k, v in rows.items()))
 
 
 def block(out, with_solve=True):
@@ -676,13 +697,19 @@ def c4(out):
         u_obs = u_obs + 1e-3 * torch.from_numpy(igamf.rng_normal(9, op.nfunc)).cuda() * interior
     IA.solve_kappa(s, f, k1)  # graph capture outside the timed region
     torch.cuda.synchronize()
-    k = k1.clone().requires_grad_()
-    t0 = time.perf_counter()
-    u = IA.solve_kappa(s, f, k)
-    J = 0.5 * ((u - u_obs) ** 2).sum()
-    J.backward()
-    torch.cuda.synchronize()
-    t_grad = time.perf_counter() - t0
+    # three timed passes, the median reported: the first also pays torch's
+    # first use of its autograd and elementwise kernels
+    t_runs = []
+    for _ in range(3):
+        k = k1.clone().requires_grad_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u = IA.solve_kappa(s, f, k)
+        J = 0.5 * ((u - u_obs) ** 2).sum()
+        J.backward()
+        torch.cuda.synchronize()
+        t_runs.append(time.perf_counter() - t0)
+    t_grad = median(t_runs)
     grad = k.grad.clone()
     h = 1e-6
     errs = []

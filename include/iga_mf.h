@@ -467,6 +467,14 @@ typedef struct {
                                    u.s, u.Au inside the apply): the apply plus
                                    one vector kernel per iteration.
                                    Apply variant 1 cannot fuse: 0 is used. */
+  double pc_inner_abstol;       /* FEM_CG: absolute floor of the inner CG's
+                                   stopping test, which (like PETSc's KSPCG
+                                   default, KSP_NORM_PRECONDITIONED) compares
+                                   ||D^-1 r|| with max(pc_inner_rtol
+                                   ||D^-1 b||, pc_inner_abstol); the reference
+                                   drivers set 1e-10 for both
+                                   (matrix_free/iteration_number_fempc_mf.cpp:
+                                   214-220); default 0 */
 } iga_solver_opts;
 
 typedef struct {
@@ -481,7 +489,8 @@ typedef struct iga_solver iga_solver;
 /* Defaults: PCG_F64, Jacobi, SF kernel, INVLM gather; rtol 1e-8, abstol 0,
    divtol 1e4, maxit 100000; inner_rtol 0 (automatic), inner_maxit 100000;
    pc_inner_maxit 20, pc_inner_rtol 1e-2, pc_cheb_degree 8; use_graph 1,
-   inner_f16_geometry 0, fused 1.  Host code only (no device call). */
+   inner_f16_geometry 0, fused 1, pc_inner_abstol 0.  Host code only (no
+   device call). */
 void iga_solver_opts_default(iga_solver_opts *o);
 int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o,
                       iga_solver **out);

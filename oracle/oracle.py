@@ -487,3 +487,111 @@ def ref_read_partition(fn, cap):
     for k, n in zip(names, cnt[3:]):
         out[k] = (bufs.get(k) if k in bufs else ibufs[k])[:n].copy()
     return out
+
+
+# --------------------------------------------------------------------------- PETSc KSP semantics
+# The reference solves with PETSc 3.18.6 (matrix_free/CMakeLists.txt:55), a
+# dependency absent here.  Its FEM-preconditioned drivers
+# (matrix_free/iteration_number_fempc_mf.cpp:190-221,
+# matrix_free_gpu/matrixfreegpu.cu:264-301) call KSPSetFromOptions on the outer
+# KSP and never set its type, so it is PETSc's default KSPGMRES; the PCSHELL
+# runs an inner KSPCG + PCJACOBI on the Q1 matrix; both get rtol = abstol =
+# 1e-10.  The two functions below restate the published algorithms with the
+# defaults of that release: zero initial guess, KSP_NORM_PRECONDITIONED, and
+# KSPConvergedDefault (stop when the norm is <= max(rtol * norm_0, abstol));
+# GMRES restarts every 30 iterations, left preconditioning, classical
+# Gram-Schmidt without refinement, the residual norm estimated from the
+# Givens-rotated Hessenberg system.
+
+def petsc_cg(matvec, b, pc, rtol, atol, maxit=10000):
+    """KSPCG with a preconditioner pc(r): (x, iterations)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = pc(r)
+    n0 = np.linalg.norm(z)
+    if n0 <= max(rtol * n0, atol):
+        return x, 0
+    p = z.copy()
+    rz = r @ z
+    for it in range(1, maxit + 1):
+        q = matvec(p)
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        z = pc(r)
+        if np.linalg.norm(z) <= max(rtol * n0, atol):
+            return x, it
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, maxit
+
+
+def petsc_gmres(matvec, b, pc, rtol, atol, restart=30, maxit=10000):
+    """KSPGMRES(restart), left preconditioning: (x, iterations)."""
+    x = np.zeros_like(b)
+    its, norm0 = 0, None
+    while True:
+        r = pc(b - matvec(x))
+        beta = np.linalg.norm(r)
+        if norm0 is None:
+            norm0 = beta
+        if beta <= max(rtol * norm0, atol) or its >= maxit:
+            return x, its
+        V = np.zeros((restart + 1, len(b)))
+        V[0] = r / beta
+        H = np.zeros((restart + 1, restart))
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        k = 0
+        for j in range(restart):
+            w = pc(matvec(V[j]))
+            h = V[:j + 1] @ w              # classical Gram-Schmidt
+            w = w - V[:j + 1].T @ h
+            H[:j + 1, j] = h
+            H[j + 1, j] = np.linalg.norm(w)
+            for i in range(j):             # earlier rotations
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            d = np.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = H[j, j] / d, H[j + 1, j] / d
+            H[j, j], H[j + 1, j] = d, 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] *= cs[j]
+            its += 1
+            k = j + 1
+            if H[j, j] == 0.0 or abs(g[j + 1]) <= max(rtol * norm0, atol) or its >= maxit:
+                break
+            V[j + 1] = w / np.linalg.norm(w)
+        y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+        x = x + V[:k].T @ y
+        if abs(g[k]) <= max(rtol * norm0, atol) or its >= maxit:
+            return x, its
+
+
+def fempc_reference_iterations(p, n, exact_pc=False):
+    """The reference's FEM-preconditioned solve of the unit square (f =
+    x(1-x)y(1-y), matrix_free/include/LocalAssemblyMF.hpp:52-55) restated on
+    the oracle's matrices: outer GMRES on the interior IGA matrix, PCSHELL =
+    inner CG + Jacobi on the interior Q1 matrix of the reference's node grid,
+    all tolerances 1e-10.  Returns the outer iteration count (exact_pc: the
+    Q1 matrix factorised instead of the inner CG)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    m = mesh_unit_square(p, n)
+    idx = np.where(m.ID >= 0)[0]
+    A = assembled_matrix(m)[idx][:, idx].tocsr()
+    F = load(m, LOAD_REF_POLY)[idx]
+    nf = n + p
+    rows, cols, _, _, val = fem_assemble(nf, nf, fem_nodes(p, p, n, n))
+    K = sp.coo_matrix((val, (rows, cols)), shape=(nf * nf, nf * nf)).tocsr()[idx][:, idx].tocsr()
+    if exact_pc:
+        lu = spla.splu(K.tocsc())
+        pc = lu.solve
+    else:
+        dinv = 1.0 / K.diagonal()
+        pc = lambda v: petsc_cg(lambda q: K @ q, v, lambda r: dinv * r, 1e-10, 1e-10)[0]
+    _, its = petsc_gmres(lambda v: A @ v, F, pc, 1e-10, 1e-10)
+    return its

@@ -522,7 +522,7 @@ __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, const T 
     if (RAT) v *= wf[Ag];
     if (!shared)
       y[Ag] = v;
-    else if (A.mode == 0)
+    else
       atomic_add(&y[Ag], v);
     return;
   }
@@ -533,14 +533,14 @@ __device__ __forceinline__ void put_footprint(const ApplyArgs &A, T *y, const T 
   const bool shared = (xlo != xhi) || (ylo != yhi);
   const int Ag = gy * A.nfx + gx;
   if (A.mask_out && is_dirichlet(A.dirmask, Ag, gx, gy, A.nfx, A.nfy)) {
-    if (!shared || A.mode == 0) y[Ag] = T(0);
+    y[Ag] = T(0);
     return;
   }
   if (A.dot) dacc += (double)xin * (double)v;
   if (RAT) v *= wf[Ag];
   if (!shared)
     y[Ag] = v;
-  else if (A.mode == 0)
+  else
     atomic_add(&y[Ag], v);
 }
 

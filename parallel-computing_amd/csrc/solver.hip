@@ -947,14 +947,14 @@ __global__ void k_pc_init(const V *r, V *zz, V *rin, V *w, V *pp, const double *
 }
 
 __global__ void k_pc_init_fin(const double *part, int nb, double *sc, int *ic, double rtol2,
-                              int maxit, const int *outer_done) {
+                              double atol2, int maxit, const int *outer_done) {
   if (*outer_done) return;
   const double ww = fold(part + P_RR * RG, nb);
   const double rz = fold(part + P_RZ * RG, nb);
   if (threadIdx.x == 0) {
     sc[S_RZ] = rz;
     sc[S_RR0] = ww;
-    sc[S_TOL2] = rtol2 * ww;
+    sc[S_TOL2] = fmax(rtol2 * ww, atol2);
     sc[S_DIV2] = 1e300;
     ic[I_ITER] = 0;
     ic[I_NEXT] = 0;
@@ -1135,10 +1135,10 @@ int fem_pc(iga_solver *s, const V *r, V *z, V *rin, V *w, V *pp, V *q, const V *
   const int *od = s->ic + I_DONE;
   const int64_t n = s->n;
   const int nb = s->nb;
-  const double rt = s->o.pc_inner_rtol;
+  const double rt = s->o.pc_inner_rtol, at = s->o.pc_inner_abstol;
   LAUNCH((k_pc_init<V>), nb, r, z, rin, w, pp, s->fdinv, n, s->part_in, od);
   hipLaunchKernelGGL(k_pc_init_fin, dim3(1), dim3(RB), 0, st, s->part_in, nb, sc, ic, rt * rt,
-                     s->o.pc_inner_maxit, od);
+                     at * at, s->o.pc_inner_maxit, od);
   const int bs = 256;
   for (int it = 0; it < s->o.pc_inner_maxit; ++it) {
     hipLaunchKernelGGL((k_spmv<V>), dim3((s->fem_rows + bs - 1) / bs), dim3(bs), 0, st, s->rowptr,
@@ -1641,6 +1641,7 @@ void iga_solver_opts_default(iga_solver_opts *o) {
   o->inner_maxit = 100000;
   o->pc_inner_maxit = 20;
   o->pc_inner_rtol = 1e-2;
+  o->pc_inner_abstol = 0.0;
   o->pc_cheb_degree = 8;
   o->use_graph = 1;
   o->fused = 1;
@@ -1688,7 +1689,8 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
   // against 18.5 ms with the V-cycle, 63 iterations either way)
   if (o.inner_rtol == 0.0) o.inner_rtol = (o.pc == IGA_PC_NONE || o.pc == IGA_PC_JACOBI) ? 0.1 : 1e-4;
   if (!(o.rtol >= 0) || !(o.abstol >= 0) || o.maxit < 1 || !(o.inner_rtol > 0) ||
-      o.inner_maxit < 1 || (o.pc == IGA_PC_FEM_CG && o.pc_inner_maxit < 1))
+      o.inner_maxit < 1 || (o.pc == IGA_PC_FEM_CG && o.pc_inner_maxit < 1) ||
+      !(o.pc_inner_abstol >= 0))
     return IGA_ERR_INVALID_ARGUMENT;
   int rc = igai::set_device(op->device);
   if (rc) return rc;

@@ -73,7 +73,8 @@ class SolverOpts(C.Structure):
                 ("maxit", C.c_int), ("inner_rtol", C.c_double), ("inner_maxit", C.c_int),
                 ("pc_inner_maxit", C.c_int), ("pc_inner_rtol", C.c_double),
                 ("pc_cheb_degree", C.c_int), ("use_graph", C.c_int),
-                ("inner_f16_geometry", C.c_int), ("fused", C.c_int)]
+                ("inner_f16_geometry", C.c_int), ("fused", C.c_int),
+                ("pc_inner_abstol", C.c_double)]
 
 
 class PreprocessInfo(C.Structure):
@@ -781,3 +782,59 @@ class Solver:
         if getattr(self, "_h", None) is not None and _LIB is not None:
             _LIB.iga_solver_destroy(self._h)
             self._h = None
+
+
+def ksp_gmres(matvec, b, pc, rtol=1e-10, atol=1e-10, restart=30, maxit=10000):
+    """Host-driven GMRES on device vectors with PETSc 3.18's KSPGMRES
+    defaults -- the outer KSP the reference's drivers get from
+    KSPSetFromOptions (matrix_free/iteration_number_fempc_mf.cpp:190-221,
+    matrix_free_gpu/matrixfreegpu.cu:264-301): zero initial guess, left
+    preconditioning, the preconditioned residual norm, classical Gram-Schmidt
+    without refinement, restart 30, and the stop ||r_k|| <= max(rtol ||r_0||,
+    atol).  matvec(v) and pc(v) return new float64 tensors (e.g.
+    Operator.apply, the MATSHELL MatMult, and Solver.apply_pc, the PCSHELL
+    apply).  For users without PETSc: it drives the same entry points a
+    PETSc KSP would.  Returns (x, iterations)."""
+    import numpy as np
+    import torch
+    x = torch.zeros_like(b)
+    its, norm0 = 0, None
+    while True:
+        r = pc(b - matvec(x))
+        beta = torch.linalg.vector_norm(r).item()
+        if norm0 is None:
+            norm0 = beta
+        tol = max(rtol * norm0, atol)
+        if beta <= tol or its >= maxit:
+            return x, its
+        V = torch.zeros((restart + 1, b.numel()), dtype=b.dtype, device=b.device)
+        V[0] = r / beta
+        H = np.zeros((restart + 1, restart))
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        k = 0
+        for j in range(restart):
+            w = pc(matvec(V[j]))
+            h = V[:j + 1] @ w
+            w = w - V[:j + 1].T @ h
+            H[:j + 1, j] = h.cpu().numpy()
+            H[j + 1, j] = torch.linalg.vector_norm(w).item()
+            for i in range(j):
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            d = float(np.hypot(H[j, j], H[j + 1, j]))
+            cs[j], sn[j] = H[j, j] / d, H[j + 1, j] / d
+            H[j, j], H[j + 1, j] = d, 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] *= cs[j]
+            its += 1
+            k = j + 1
+            if H[j, j] == 0.0 or abs(g[j + 1]) <= tol or its >= maxit:
+                break
+            V[j + 1] = w / torch.linalg.vector_norm(w)
+        y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+        x = x + V[:k].T @ torch.from_numpy(y).to(b.device, b.dtype)
+        if abs(g[k]) <= tol or its >= maxit:
+            return x, its

@@ -174,6 +174,7 @@ def test_solver_opts_defaults():
     documents; reading them back through the ctypes mirror also checks that
     the mirror's field layout matches the C struct."""
     o = igamf.SolverOpts()
+    o.pc_inner_abstol = 7.0  # the last field: overwritten only if the layouts agree
     igamf.lib().iga_solver_opts_default(C.byref(o))
     assert (o.kind, o.pc, o.kernel, o.assembly) == (
         igamf.SOLVE_PCG_F64, igamf.PC_JACOBI, igamf.SF, igamf.INVLM_GATHER)
@@ -181,6 +182,7 @@ def test_solver_opts_defaults():
     assert (o.inner_rtol, o.inner_maxit) == (0.0, 100000)  # 0: automatic
     assert (o.pc_inner_maxit, o.pc_inner_rtol, o.pc_cheb_degree) == (20, 1e-2, 8)
     assert (o.use_graph, o.inner_f16_geometry, o.fused) == (1, 0, 1)
+    assert o.pc_inner_abstol == 0.0
 
 
 def test_binding_rejects_bad_vectors_before_any_device_call():

@@ -457,3 +457,70 @@ def test_ilu0_invariants():
     b = np.arange(1.0, m + 1)
     x = O.ilu0_solve(T.indptr, T.indices, lu2, diag2, b)
     assert np.abs(x - sla.lu_solve(sla.lu_factor(T.toarray()), b)).max() < 1e-13 * np.abs(x).max()
+
+
+# --------------------------------------------------------------------------- PETSc KSP semantics
+def test_petsc_krylov_restatements_against_direct_solves():
+    """The oracle's KSPGMRES and KSPCG restatements: on an SPD system with a
+    Jacobi preconditioner both reach the direct solution, GMRES's rotated
+    Hessenberg estimate of the preconditioned residual equals the true one
+    at every restart length, and a run capped at its own iteration count
+    returns the same iterate."""
+    rng = np.random.default_rng(5)
+    n = 120
+    Q = sp.random(n, n, density=0.05, random_state=6, format="csr")
+    A = (Q @ Q.T + sp.identity(n) * 4.0).tocsr()
+    b = rng.standard_normal(n)
+    x_ref = spla.spsolve(A.tocsc(), b)
+    dinv = 1.0 / A.diagonal()
+    pc = lambda r: dinv * r
+    xc, ic = O.petsc_cg(lambda v: A @ v, b, pc, 1e-12, 0.0)
+    assert np.abs(xc - x_ref).max() <= 1e-9 * np.abs(x_ref).max() and 0 < ic < n
+    for restart in (5, 30):
+        xg, ig = O.petsc_gmres(lambda v: A @ v, b, pc, 1e-12, 0.0, restart=restart)
+        assert np.abs(xg - x_ref).max() <= 1e-9 * np.abs(x_ref).max(), restart
+        # the estimate that stopped it is the true preconditioned residual
+        r_true = np.linalg.norm(pc(b - A @ xg))
+        assert r_true <= 1.0001e-12 * np.linalg.norm(pc(b)) * 10, (restart, r_true)
+        x5, i5 = O.petsc_gmres(lambda v: A @ v, b, pc, 0.0, 0.0, restart=restart, maxit=7)
+        est_stop = np.linalg.norm(pc(b - A @ x5))
+        x5b, _ = O.petsc_gmres(lambda v: A @ v, b, pc, 0.0, est_stop * (1 + 1e-9), restart=restart)
+        assert i5 == 7 and np.allclose(x5, x5b, rtol=0, atol=1e-12 * np.abs(x5).max())
+
+
+def test_fempc_counts_follow_the_reference_log():
+    """The reference's FEM-preconditioned iteration counts (p = 3, unit
+    square, matrix_free/data/fempc_mf.txt:12,24,36: 35, 33, 28) are those
+    of PETSc's default outer KSPGMRES with the reference's abstol 1e-10 on
+    the preconditioned norm, which the small load vector of f =
+    x(1-x)y(1-y) makes the binding test: the oracle's restatement gives
+    33 / 30 / 28 (31 / 30 / 28 with the Q1 matrix factorised), falling with
+    refinement as the log does, while a CG stopping on the unpreconditioned
+    relative residual (the library's own test) needs about twice as many."""
+    logged = {32: 35, 64: 33, 128: 28}
+    its = {n: O.fempc_reference_iterations(3, n) for n in (32, 64, 128)}
+    for n, ref in logged.items():
+        assert abs(its[n] - ref) <= 3, (n, its[n], ref)
+    assert its[32] >= its[64] >= its[128]
+    assert its[128] == logged[128]
+    # the unpreconditioned-norm CG on the same matrices (exact Q1 solves)
+    m = O.mesh_unit_square(3, 32)
+    idx = np.where(m.ID >= 0)[0]
+    A = O.assembled_matrix(m)[idx][:, idx].tocsr()
+    F = O.load(m, O.LOAD_REF_POLY)[idx]
+    rows, cols, _, _, val = O.fem_assemble(35, 35, O.fem_nodes(3, 3, 32, 32))
+    K = sp.coo_matrix((val, (rows, cols)), shape=(35 * 35, 35 * 35)).tocsr()[idx][:, idx]
+    lu = spla.splu(K.tocsc())
+    x, r, z = np.zeros_like(F), F.copy(), lu.solve(F)
+    p, rz, n0 = z.copy(), F @ z, np.linalg.norm(F)
+    for it in range(1, 500):
+        q = A @ p
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        if np.linalg.norm(r) <= 1e-10 * n0:
+            break
+        z = lu.solve(r)
+        rz, rz0 = r @ z, rz
+        p = z + (rz / rz0) * p
+    assert 55 <= it <= 75, it

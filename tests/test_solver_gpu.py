@@ -514,3 +514,31 @@ def test_non_finite_rhs_and_divergence():
     Fk = torch.stack([F, bad]).contiguous()
     _, sts = sb.solve_block(Fk, check_status=False)
     assert sts[0].status == 0 and sts[1].status == igamf.IGA_ERR_NOT_FINITE
+
+
+def test_fempc_iterations_under_reference_ksp_semantics():
+    """The reference's solver set-up driven through this library's entry
+    points: PETSc's default outer KSPGMRES (igamf.ksp_gmres) on the MATSHELL
+    apply (Operator.apply, FP64) with the PCSHELL apply (Solver.apply_pc:
+    inner CG + Jacobi on the Q1 matrix to rtol = abstol = 1e-10), unit
+    square p = 3, 32 x 32, f = x(1-x)y(1-y).  It takes the iteration count
+    of the oracle's restatement on the assembled matrices (33; the
+    reference logged 35, matrix_free/data/fempc_mf.txt:12) and reaches the
+    direct solution."""
+    p, n = 3, 32
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, p, n)
+    op = igamf.Operator(m)
+    fem = igamf.FemPC(n + p, n + p, igamf.fem_reference_nodes(p, p, n, n))
+    F = op.load(igamf.LOAD_REF_POLY, 0.0)
+    om = _oracle_mesh(m)
+    dirs = torch.from_numpy(om.Dir.astype(np.int64)).cuda()
+    assert F[dirs].abs().max().item() == 0.0
+    s = igamf.Solver(op, fem, pc=igamf.PC_FEM_CG, pc_inner_rtol=1e-10, pc_inner_abstol=1e-10,
+                     pc_inner_maxit=400)
+    u, its = igamf.ksp_gmres(lambda v: op.apply(v.contiguous()), F, lambda v: s.apply_pc(v.contiguous()))
+    its_oracle = O.fempc_reference_iterations(p, n)
+    assert abs(its - its_oracle) <= 1, (its, its_oracle)
+    assert abs(its - 35) <= 3, its
+    u_ref = _direct(m, om, F.cpu().numpy())
+    err = np.abs(u.cpu().numpy() - u_ref).max() / np.abs(u_ref).max()
+    assert err < 1e-7, err
