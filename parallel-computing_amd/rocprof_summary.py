@@ -106,6 +106,12 @@ def counters(a):
     print("\n".join(lines))
 
 
+def base(name):
+    """A kernel's name without its argument list (and without the
+    anonymous-namespace qualifier, whose parenthesis would cut it short)."""
+    return name.replace("(anonymous namespace)::", "").split("(")[0]
+
+
 def pick(names, vals, counter, pred):
     return [(d, vals[d][counter]) for d, n in names.items() if pred(n) and counter in vals[d]]
 
@@ -116,7 +122,7 @@ def traffic(a):
     for counter, db in (("FETCH_SIZE", a.fetch), ("WRITE_SIZE", a.write)):
         names, vals = counter_rows(db)
         copies = pick(names, vals, counter, lambda n: "copy" in n.lower())
-        kern = pick(names, vals, counter, lambda n: a.kernel in n.split("(")[0])
+        kern = pick(names, vals, counter, lambda n: a.kernel in base(n))
         if not copies or not kern:
             raise SystemExit(f"{counter}: no copy ({len(copies)}) or kernel ({len(kern)}) dispatch")
         copy_kib = max(v for _, v in copies)
@@ -124,7 +130,8 @@ def traffic(a):
         kib = sum(v for _, v in kern) / len(kern)
         per[counter] = {"kernel_kib_per_dispatch": kib, "copy_kib": copy_kib, "scale": scale,
                         "bytes_per_launch": kib * 1024.0 * scale, "dispatches": len(kern)}
-        res["kernel"] = short(names[kern[0][0]].split("(")[0].replace("void igak::", ""), 200)
+        res["kernel"] = short(base(names[kern[0][0]]).replace("void igak::", "").replace(
+            "void ", ""), 200)
     res.update(per)
     res["bytes_per_launch"] = per["FETCH_SIZE"]["bytes_per_launch"] + \
         per["WRITE_SIZE"]["bytes_per_launch"]
