@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--cpu-applies", type=int, default=10)
     ap.add_argument("--assembly", choices=["gather", "atomic"], default="gather")
     ap.add_argument("--kernel", choices=["sf", "ebe"], default="sf")
+    ap.add_argument("--variant-f32", type=int, default=-1,
+                    help="FP32 apply variant of the operator (-1: the library's default)")
     ap.add_argument("--detail", action="store_true", help="also time every apply variant")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB device copy first (PMC byte-counter calibration)")
@@ -126,7 +128,7 @@ def headline(a, out):
     p, n = a.p, a.nelem
     t0 = time.perf_counter()
     mesh = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, p, n, r0=1.0, r1=2.0)
-    op = igamf.Operator(mesh, f16_geometry=True)
+    op = igamf.Operator(mesh, f16_geometry=True, variant_f32=a.variant_f32)
     setup_s = time.perf_counter() - t0
     info = op.info
     nfunc = op.nfunc
@@ -168,6 +170,8 @@ def headline(a, out):
     tx, ty = info.tile_x_f32, info.tile_y_f32
     if variant == 1 and a.kernel == "sf":
         kname = f"k_apply_pf<float, {p}, {p + 1}, {tx}, {ty}>"
+    elif variant == 5 and a.kernel == "sf":
+        kname = f"k_apply_lds<float, {p}, {p + 1}, float>"
     else:
         minw = 4 if (variant == 2 and a.kernel == "sf") else 1
         split = 2 if (variant == 3 and a.kernel == "sf") else 1
@@ -262,6 +266,20 @@ def fp64_and_variants(a, out, op, x64, x32, kern, asm, n_int, nfunc, E, nqp):
             "g16_kernel_ms": round(k4g, 5)}
         del op4
         log(f"matrix-core apply kernel {k4 * 1e3:.1f} us (FP16 G_q {k4g * 1e3:.1f} us)")
+    # the LDS-staged persistent form (apply variant 5, degrees 1-3), unless
+    # it is the headline's own
+    if kern == igamf.SF and op.p <= 3 and op.info.rational != 2 and op.info.variant_f32 != 5:
+        op5 = igamf.Operator(op.mesh, variant_f32=5, f16_geometry=op.p == 3)
+        k5 = op5.time_kernel(x32, torch.empty_like(x32), a.steps, kern, asm)
+        alg_bytes32 = 4 * (2 * nfunc) + 4 * 3 * E * nqp
+        out["apply"]["lds_staged"] = {
+            "kernel_ms": round(k5, 5),
+            "kernel_hbm_frac": round(alg_bytes32 / (k5 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if op.p == 3:
+            out["apply"]["lds_staged"]["g16_kernel_ms"] = round(
+                op5.time_kernel(x32, torch.empty_like(x32), a.steps, kern, asm, g16=True), 5)
+        del op5
+        log(f"LDS-staged apply kernel {k5 * 1e3:.1f} us")
     if a.detail:
         var = {}
         for kn, kv in (("sf", igamf.SF), ("ebe", igamf.EBE)):

@@ -245,6 +245,11 @@ typedef struct {
                                    4 (FP32 only): the element operator on the
                                    matrix cores (SF, p = 3, 16x16, not general
                                    rational; EBE applies use 0);
+                                   5 (FP32 only): persistent workgroups that
+                                   stage the next tile's G_q in LDS by direct
+                                   global-to-LDS loads (SF, p <= 3, 16x16, not
+                                   general rational; its FP16-G_q form at
+                                   p = 3; EBE applies use 0);
                                    -1 (default): 3 for FP32 at p >= 5 and for
                                    FP64 at p = 3 (not general rational),
                                    else 0 */

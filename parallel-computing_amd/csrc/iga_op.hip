@@ -491,7 +491,7 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   };
   op->variant64 = o.variant_f64 < 0 ? dflt_variant(true) : o.variant_f64;
   op->variant32 = o.variant_f32 < 0 ? dflt_variant(false) : o.variant_f32;
-  if (o.variant_f64 < -1 || op->variant64 > 3 || o.variant_f32 < -1 || op->variant32 > 4) {
+  if (o.variant_f64 < -1 || op->variant64 > 3 || o.variant_f32 < -1 || op->variant32 > 5) {
     delete op;
     return IGA_ERR_INVALID_ARGUMENT;
   }
@@ -527,6 +527,7 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
       return fail(IGA_ERR_UNSUPPORTED);
   }
   if ((op->variant32 == 4 && (rational == 2 || !igak::mfma_compiled(p, op->tile32))) ||
+      (op->variant32 == 5 && (rational == 2 || !igak::lds_compiled(p, op->tile32, false))) ||
       (op->variant64 == 1 && !igak::pf_compiled(p, op->tile64)) ||
       (op->variant32 == 1 && !igak::pf_compiled(p, op->tile32)) || op->variant64 == 2 ||
       (op->variant32 == 2 && !igak::w4_compiled(p, op->tile32, false)))

@@ -1134,6 +1134,130 @@ __global__ __launch_bounds__(TX *TY) void k_apply_pf(ApplyArgs A, int ntiles) {
   }
 }
 
+// Direct global-to-LDS copy (gfx950 global_load_lds): lane l of the wave
+// moves SZ bytes (4 or 16) from src to lds_wave + l * SZ, where lds_wave is
+// the same address on every lane of the wave.  The data passes no register;
+// it is in LDS once the issuing wave's vmcnt has drained.
+template <int SZ>
+__device__ __forceinline__ void lds_dma(const void *src, void *lds_wave) {
+  static_assert(SZ == 4 || SZ == 16, "dword or dwordx4 copies");
+  const auto g = (const __attribute__((address_space(1))) void *)src;
+  const auto l = (__attribute__((address_space(3))) void *)lds_wave;
+  if constexpr (SZ == 16)
+    __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+  else
+    __builtin_amdgcn_global_load_lds(g, l, 4, 0, 0);
+}
+
+// Persistent sum-factorised apply on 16 x 16 tiles with the geometry factors
+// staged in LDS (variant 5).  While a workgroup computes and reduces one
+// tile, the next tile's G_q stream from HBM straight into LDS and its x
+// footprint and 1D tables into registers, so the G_q stream does not stop
+// during the compute, reduction and barrier phases and costs no registers.
+// Lane t of the copy moves exactly the G_q values that thread t reads back
+// (the storage order of gq_index), so the staging needs no barrier of its
+// own: each wave waits for its own copies (vmcnt) before reading them, and
+// for its reads (lgkmcnt) before it starts the next copy into the same bytes.
+// Polynomial and tensor-weight meshes; T = float on TG = float or _Float16.
+template <typename T, int P, int NQ, typename TG = T>
+__global__ __launch_bounds__(256, 2) void k_apply_lds(ApplyArgs A, int ntiles) {
+  constexpr int TX = 16, TY = 16, NB = P + 1, NT = TX * TY, FX = TX + P, FY = TY + P;
+  constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY;
+  constexpr int VW = 16 / sizeof(TG), NV = NQ2 / VW, NR = NQ2 - NV * VW;
+  static_assert(NR == 0 || sizeof(TG) == 4, "points past the vectors are staged as dwords");
+  constexpr int CB = NQ2 * NT * sizeof(TG);  // bytes of one component of one tile
+  __shared__ T xs[NFP];
+  __shared__ T tbx[2 * TAB][TX];
+  __shared__ T tby[2 * TAB][TY];
+  __shared__ T bufm[row16_lds<P, TY>()];
+  __shared__ __attribute__((aligned(16))) unsigned char gs[3 * CB];
+  typedef typename Vec16<TG>::type V;
+
+  if (A.skip && *A.skip) return;
+  cg_handoff(A, NT);
+  const int t = threadIdx.x, lx = t & 15, ly = t >> 4, w0 = t & ~63;
+  const unsigned char *Gb = static_cast<const unsigned char *>(A.G);
+  T *y = static_cast<T *>(A.y);
+  const int G = gridDim.x;
+  auto tile_of = [&](int s) { return A.rev ? ntiles - 1 - s : s; };
+  TileStage<T, P, NQ, TX, TY, false> stn;
+  auto prefetch = [&](int tl) {
+    stn.load(A, (tl % A.ntx) * TX, (tl / A.ntx) * TY, t);
+    const unsigned char *src = Gb + (size_t)tl * 3 * CB;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const int o = c * CB + k * NT * 16;
+        lds_dma<16>(src + o + t * 16, gs + o + w0 * 16);
+      }
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int o = c * CB + NV * NT * 16 + r * NT * 4;
+        lds_dma<4>(src + o + t * 4, gs + o + w0 * 4);
+      }
+    }
+  };
+  // this workgroup's steps: start, start + G, ... (starts XCD-contiguous)
+  int s = xcd_tile(blockIdx.x, G);
+  if (s < ntiles) prefetch(tile_of(s));
+  for (; s < ntiles; s += G) {
+    const int tile = tile_of(s);
+    const int tx = tile % A.ntx, ty = tile / A.ntx;
+    const int ex0 = tx * TX, ey0 = ty * TY;
+    const bool valid = (ex0 + lx < A.nex) && (ey0 + ly < A.ney);
+    const bool interior = A.mode == 1 && tx > 0 && ty > 0 && tx < A.ntx - 1 &&
+                          ty < A.nty - 1 && (!A.mask_out || !A.dirmask);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this tile's G_q copies and loads
+    __syncthreads();  // the previous tile's reads of xs, tables and bufm are done
+    stn.store(A, ex0, ey0, t, xs, tbx, tby);
+    T g[3][NQ2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const V v = *reinterpret_cast<const V *>(gs + c * CB + k * NT * 16 + t * 16);
+#pragma unroll
+        for (int l = 0; l < VW; ++l) g[c][k * VW + l] = (T)v[l];
+      }
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        g[c][NV * VW + r] =
+            (T) * reinterpret_cast<const TG *>(gs + c * CB + NV * NT * 16 + r * NT * 4 + t * 4);
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // g read before gs is refilled
+    if (s + G < ntiles) prefetch(tile_of(s + G));
+    __syncthreads();  // xs and tables written
+
+    T ye[NB][NB];
+    {
+      T X[NB][NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
+      auto gl = [&](int c, int q, int, int) { return g[c][q]; };
+      element_sf<T, P, NQ, TX, TY>(X, tbx, tby, lx, ly, gl, ye);
+      if constexpr (!std::is_same_v<TG, T>) {
+        const T gsc = (T)A.gscale;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int a = 0; a < NB; ++a) ye[b][a] *= gsc;
+      }
+      if (!valid) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int a = 0; a < NB; ++a) ye[b][a] = T(0);
+      }
+    }
+    double dacc = 0.0;
+    reduce_store_row16<T, P, TY, false>(A, ye, bufm, t, ex0, ey0, y, nullptr, xs, dacc, interior);
+    if (A.dot) store_tile_dot<NT>(A, tile, dacc);
+  }
+}
+
 // Seam functions (touched by more than one tile): zero them before a
 // scatter-add apply.  One thread per seam function, numbered row by row
 // within each band of P columns right of a vertical tile boundary (so a

@@ -39,6 +39,12 @@ inline bool w4_compiled(int p, int id, bool f64) { return p <= 4 && id == TILE_1
 inline bool g16_compiled(int id) { return id == TILE_16x16 || id == TILE_16x8; }
 // matrix-core variant (4): degree 3, 16x16 tiles, FP32 arithmetic
 inline bool mfma_compiled(int p, int id) { return p == 3 && id == TILE_16x16; }
+// LDS-staged persistent variant (5): 16x16 tiles, FP32 arithmetic; G_q of a
+// tile staged in LDS (FP32 G_q up to degree 3, FP16 G_q at degree 3)
+constexpr bool lds_compiled_c(int p, bool g16) { return g16 ? p == 3 : p >= 1 && p <= 3; }
+inline bool lds_compiled(int p, int id, bool g16) {
+  return id == TILE_16x16 && lds_compiled_c(p, g16);
+}
 inline bool rat_compiled(int p, int id) {
   return id == TILE_16x16 || id == TILE_16x8 || (p <= 4 && id == TILE_32x16);
 }
@@ -63,12 +69,37 @@ struct Launch {
     }
     return hipErrorInvalidValue;
   }
+  // LDS-staged persistent SF apply (variant 5): as many workgroups as fit
+  // at once, each walking every G-th tile
+  template <typename TG>
+  static hipError_t apply_lds_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
+    if constexpr (lds_compiled_c(P, sizeof(TG) == 2)) {
+      static int resident = 0;
+      if (!resident) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void *>(&k_apply_lds<float, P, NQ, TG>), 256, 0) !=
+                hipSuccess)
+          return hipErrorUnknown;
+        resident = cus * (per > 0 ? per : 1);
+      }
+      const int grid = ntiles < resident ? ntiles : resident;
+      hipLaunchKernelGGL((k_apply_lds<float, P, NQ, TG>), dim3(grid), dim3(256), 0, s, a, ntiles);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
   // FP32 SF apply on FP16-stored G_q (polynomial / tensor-weight meshes):
   // the 16x16 one-workgroup-per-tile kernel (its matrix-core form for
-  // variant 4), or the split-element one on 16x8 tiles
+  // variant 4, its LDS-staged persistent form for variant 5), or the
+  // split-element one on 16x8 tiles
   static hipError_t apply_g16(const ApplyArgs &a, int tile, int ntiles, int variant,
                               hipStream_t s) {
     if (variant == 4 && mfma_compiled(P, tile)) return apply_mfma_t<_Float16>(a, ntiles, s);
+    if (variant == 5 && lds_compiled(P, tile, true)) return apply_lds_t<_Float16>(a, ntiles, s);
     if (tile == TILE_16x16) return apply_t<float, 16, 16, false, false, 1, 1, _Float16>(a, ntiles, s);
     if (tile == TILE_16x8) return apply_t<float, 16, 8, false, false, 1, 2, _Float16>(a, ntiles, s);
     return hipErrorInvalidValue;
@@ -128,8 +159,10 @@ struct Launch {
   // variant: 0 one workgroup per tile, 1 persistent pipelined (SF only),
   // 2 one workgroup per tile at four waves per SIMD (FP32 SF, 16x16, p <= 4)
   // 3 two threads per element (SF; 16x8 tiles, which the EBE apply of such
-  // an operator also uses), 4 matrix-core SF (FP32, 16x16, p = 3).  One precision per call, so that FP32 and FP64
-  // kernels compile in separate translation units.
+  // an operator also uses), 4 matrix-core SF (FP32, 16x16, p = 3), 5
+  // persistent with G_q staged in LDS (FP32 SF, 16x16, p <= 3).  One
+  // precision per call, so that FP32 and FP64 kernels compile in separate
+  // translation units.
   template <typename T>
   static hipError_t apply_prec(const ApplyArgs &a, bool ebe, int tile, int ntiles, int variant,
                                hipStream_t s) {
@@ -139,9 +172,12 @@ struct Launch {
     if constexpr (P <= 4 && sizeof(T) == 4)
       if (variant == 2 && !ebe && tile == TILE_16x16)
         return apply_t<float, 16, 16, false, false, 4>(a, ntiles, s);
-    if constexpr (sizeof(T) == 4)
+    if constexpr (sizeof(T) == 4) {
       if (variant == 4 && !ebe && mfma_compiled(P, tile)) return apply_mfma_t<float>(a, ntiles, s);
-    if (variant == 2 || variant == 4) variant = 0;  // EBE applies of a variant-2 / 4 operator
+      if (variant == 5 && !ebe && lds_compiled(P, tile, false))
+        return apply_lds_t<float>(a, ntiles, s);
+    }
+    if (variant == 2 || variant >= 4) variant = 0;  // EBE applies of a variant-2 / 4 / 5 operator
     if (variant == 1 && !ebe) return apply_pf_tile<T>(a, tile, ntiles, s);
     return ebe ? apply_tile<T, true>(a, tile, ntiles, s) : apply_tile<T, false>(a, tile, ntiles, s);
   }

@@ -278,6 +278,56 @@ def test_matrix_core_variant_unsupported():
         igamf.Operator(m, variant_f64=4)
 
 
+LDS_CASES = [c for c in CASES if c[1]["p"] <= 3 and not c[0].startswith("rational")]
+
+
+@pytest.mark.parametrize("name,kw", LDS_CASES, ids=[c[0] for c in LDS_CASES])
+def test_lds_staged_variant(built, name, kw):
+    """Variant 5 (persistent workgroups, the next tile's G_q staged in LDS by
+    direct global-to-LDS loads) against the oracle, both assemblies, its x.Ax
+    product, the FP16-G_q form at degree 3, bitwise reproducibility; the
+    element arithmetic and reduction order are those of variant 0, so the
+    gather results agree with it to FP32 rounding of the same sums."""
+    m, om, op0 = built(name, kw)
+    op = igamf.Operator(m, variant_f32=5, f16_geometry=kw["p"] == 3)
+    assert op.info.variant_f32 == 5 and (op.info.tile_x_f32, op.info.tile_y_f32) == (16, 16)
+    x = igamf.rng_normal(11, op.nfunc)
+    y_ref = O.apply_mf(om, x)
+    xd = torch.from_numpy(x).cuda()
+    x32 = xd.float()
+    for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
+        y32 = op.apply(x32, assembly=asm).double().cpu().numpy()
+        assert rel(y32, y_ref) < TOL32, (name, asm, rel(y32, y_ref))
+        y, xy = op.apply_dot(x32, assembly=asm)
+        e = abs(xy.item() - float(x @ y_ref)) / abs(float(x @ y_ref))
+        assert e < 2e-6, (name, asm, e)
+        if kw["p"] == 3:
+            y16 = op.apply(x32, assembly=asm, g16=True).double().cpu().numpy()
+            assert rel(y16, y_ref) < TOL16, (name, asm, rel(y16, y_ref))
+        ye = op.apply(x32, kernel=igamf.EBE, assembly=asm).double().cpu().numpy()
+        assert rel(ye, y_ref) < TOL32, (name, asm, rel(ye, y_ref))
+    a = op.apply(x32)
+    b = op.apply(x32)
+    assert torch.equal(a, b)
+    y0 = op0.apply(x32)
+    assert rel(a.double().cpu().numpy(), y0.double().cpu().numpy()) < 1e-6
+    assert np.all(a.cpu().numpy()[om.Dir] == 0.0)
+
+
+def test_lds_staged_variant_unsupported():
+    """Variant 5 exists for FP32 at degrees 1-3 on 16x16 tiles, not general
+    rational meshes."""
+    for kw in (dict(kind=igamf.GEOM_UNIT_SQUARE, p=4, nex=20), RATIONAL[1][1]):
+        with pytest.raises(igamf.IgaError) as e:
+            igamf.Operator(igamf.Mesh.build(**kw), variant_f32=5)
+        assert e.value.status == 6, kw
+    m = igamf.Mesh.build(igamf.GEOM_UNIT_SQUARE, 3, 20)
+    with pytest.raises(igamf.IgaError):
+        igamf.Operator(m, variant_f32=5, tile_f32=(32, 16))
+    with pytest.raises(igamf.IgaError):
+        igamf.Operator(m, variant_f64=5)
+
+
 def test_general_rational_mode_and_tiles():
     """Non-tensor weights select the general rational kernels (5 geometry
     components per point); both of their tiles agree with the oracle; the

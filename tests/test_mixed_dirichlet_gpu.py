@@ -50,7 +50,7 @@ def test_mixed_dirichlet_apply(name, args):
     x = igamf.rng_normal(3, om.nfunc)
     y_ref = O.apply_mf(om, x)
     xd = torch.from_numpy(x).cuda()
-    variants = [0, 3] + ([4] if args[1] == 3 else [])
+    variants = [0, 3] + ([4] if args[1] == 3 else []) + ([5] if args[1] <= 3 else [])
     for v in variants:
         op = igamf.Operator(m, variant_f64=3 if v == 3 else 0, variant_f32=v)
         for kern in (igamf.SF, igamf.EBE):

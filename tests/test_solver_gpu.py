@@ -195,6 +195,34 @@ def test_matrix_core_inner_solve():
             assert abs(its[1] - its[0]) <= max(3, 0.12 * its[0]), (g16, asm, its)
 
 
+def test_lds_staged_inner_solve():
+    """FP32 inner solves on the LDS-staged persistent apply (variant 5), on
+    FP32 and FP16-stored G_q, both assemblies, every iteration form: the FP64
+    solution to 1e-8, and the default kernel's iteration count within 3 % with
+    the gather (the same element arithmetic and summation order) and 12 % with
+    the scatter-add (whose summation order varies from run to run)."""
+    m = igamf.Mesh.build(igamf.GEOM_QUARTER_ANNULUS, 3, 45, 38)
+    op0 = igamf.Operator(m, f16_geometry=True)
+    op5 = igamf.Operator(m, variant_f32=5, f16_geometry=True)
+    F = op0.load(igamf.LOAD_CONST, 1.0)
+    u64, st = igamf.Solver(op0, None, kind=igamf.SOLVE_PCG_F64, rtol=1e-10).solve(F)
+    assert st.status == 0
+    for g16 in (0, 1):
+        for asm in (igamf.INVLM_GATHER, igamf.IEN_ATOMIC):
+            for fused in (0, 1, 2):
+                its = []
+                for op in (op0, op5):
+                    s = igamf.Solver(op, None, kind=igamf.SOLVE_PCG_F32_REFINE, rtol=1e-10,
+                                     assembly=asm, inner_f16_geometry=g16, fused=fused)
+                    u, st = s.solve(F)
+                    assert st.status == 0 and st.rel_residual <= 1.0001e-10, (g16, asm, fused)
+                    d = ((u - u64).abs().max() / u64.abs().max()).item()
+                    assert d < 1e-8, (g16, asm, fused, d)
+                    its.append(st.iterations)
+                tol = 0.03 if asm == igamf.INVLM_GATHER else 0.12
+                assert abs(its[1] - its[0]) <= max(3, tol * its[0]), (g16, asm, fused, its)
+
+
 @pytest.mark.parametrize("asm", [igamf.INVLM_GATHER, igamf.IEN_ATOMIC])
 def test_f16_geometry_inner_solve(asm):
     """FP32 inner PCG on the FP16-geometry operator under FP64 reliable
