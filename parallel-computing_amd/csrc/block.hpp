@@ -52,7 +52,8 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
 
   // G_q of this element into registers
   float g[3][NQ2];
-  if (A.L.tx == TX && A.L.ty == TY) {
+  const bool split = gq_split(4, A.L.tx, A.L.ty, NQ);  // the operator's storage order
+  if (A.L.tx == TX && A.L.ty == TY && !split) {
     load_gq<float, NQ2>(A.G, by * A.L.ntx + bx, t, NT, g);
   } else {
     const int exc = valid ? ex : 0, eyc = valid ? ey : 0;
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(16 * TY) void k_apply_block(BlockArgs A) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int q = 0; q < NQ2; ++q) g[c][q] = A.G[gq_index<float, NQ2>(tile, c, q, tt, NTL)];
+      for (int q = 0; q < NQ2; ++q) g[c][q] = A.G[gq_at<float, NQ>(tile, c, q, tt, NTL, split)];
   }
   // X footprints of the chunk ([rhs][function]) and the 1D tables
   for (int i = t; i < RC * NFP; i += NT) {
@@ -206,6 +207,7 @@ __global__ __launch_bounds__(256, 2) void k_apply_block_mfma(BlockArgs A) {
   auto coff = [](int c) { return (c / NB) * FX + c % NB; };
 
   const int NTL = A.L.tx * A.L.ty;
+  const bool split = gq_split(4, A.L.tx, A.L.ty, NQ);  // the operator's storage order
   for (int ly = w; ly < TY; ly += NW) {
     const int ey = ey0 + ly;
     if (ey >= A.ney) break;
@@ -241,9 +243,9 @@ __global__ __launch_bounds__(256, 2) void k_apply_block_mfma(BlockArgs A) {
           const int q = 8 * s1 + 2 * g + h;
           const bool in = q < NQ2;
           const int qq = in ? q : 0;
-          g11[s1][h] = in ? A.G[gq_index<float, NQ2>(tileG, 0, qq, tt, NTL)] : 0.f;
-          g12[s1][h] = in ? A.G[gq_index<float, NQ2>(tileG, 1, qq, tt, NTL)] : 0.f;
-          g22[s1][h] = in ? A.G[gq_index<float, NQ2>(tileG, 2, qq, tt, NTL)] : 0.f;
+          g11[s1][h] = in ? A.G[gq_at<float, NQ>(tileG, 0, qq, tt, NTL, split)] : 0.f;
+          g12[s1][h] = in ? A.G[gq_at<float, NQ>(tileG, 1, qq, tt, NTL, split)] : 0.f;
+          g22[s1][h] = in ? A.G[gq_at<float, NQ>(tileG, 2, qq, tt, NTL, split)] : 0.f;
         }
       // X relative to the element's first coefficient (the operator
       // annihilates constants; see centre_element)

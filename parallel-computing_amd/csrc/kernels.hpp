@@ -125,6 +125,36 @@ __device__ __forceinline__ size_t gq_index(int tile, int c, int q, int t, int NT
   return base + (size_t)NV * VW * NT + (size_t)(q - NV * VW) * NT + t;
 }
 
+// The split-element kernels (two threads per element, 16 x 8 tiles) read
+// FP32 and FP16 factors in a thread-major order when the points per
+// direction are even: thread t = 2 el + h of the tile holds, for every row
+// j, the columns i = h NI + il (il < NI = NQ/2) of element el as NI
+// consecutive values ([tile][c][j][t][il]), so a thread reads one row of one
+// component with one load (with the element-major order the half rows
+// straddle the 16-byte vectors, and the reads become single strided dwords).
+// FP64 keeps the element-major order.
+constexpr bool gq_split(int tsize, int tx, int ty, int nq) {
+  return tsize <= 4 && tx == 16 && ty == 8 && nq % 2 == 0;
+}
+
+template <int NQ, int NI, int NC>
+__device__ __forceinline__ size_t gq_split_index(int tile, int c, int j, int il, int t, int NT) {
+  return (((size_t)tile * NC + c) * NQ + j) * (size_t)(NI * NT) + (size_t)t * NI + il;
+}
+
+// Index of point q = j NQ + i of element el (of NE) of the tile, component c.
+template <typename T, int NQ, int NC = 3>
+__device__ __forceinline__ size_t gq_at(int tile, int c, int q, int el, int NE, bool split) {
+  if constexpr (NQ % 2 == 0) {
+    if (split) {
+      constexpr int NI = NQ / 2;
+      const int j = q / NQ, i = q - j * NQ, h = i >= NI ? 1 : 0;
+      return gq_split_index<NQ, NI, NC>(tile, c, j, i - h * NI, 2 * el + h, 2 * NE);
+    }
+  }
+  return gq_index<T, NQ * NQ, NC>(tile, c, q, el, NE);
+}
+
 template <typename T>
 struct Vec16;
 template <>
@@ -739,7 +769,18 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   T g[PREF ? NC : 1][PREF ? NQ2 : 1];
   T gh[PREFH ? NC : 1][PREFH ? NQ : 1][PREFH ? NI : 1];
   const TG *Gt = static_cast<const TG *>(A.G);
-  if constexpr (PREF) load_gq<TG, NQ2, NC>(Gt, tile, el, NE, g);
+  // thread-major factors of the split kernels (gq_split): thread t's own
+  // point j NI + il of its columns; the 16x8 EBE companion reads the same
+  // storage through gq_at
+  constexpr bool GS = gq_split(sizeof(TG), TX, TY, NQ);
+  static_assert(!GS || SPLIT == 2 || EBE, "thread-major factors: split kernels");
+  if constexpr (PREF && !GS) load_gq<TG, NQ2, NC>(Gt, tile, el, NE, g);
+  if constexpr (PREF && GS) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int q = 0; q < NQ2; ++q) g[c][q] = (T)gload(Gt + gq_at<TG, NQ, NC>(tile, c, q, el, NE, true));
+  }
   if constexpr (PREFH) {
     const int i0 = h * NI;
 #pragma unroll
@@ -747,18 +788,25 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
 #pragma unroll
       for (int j = 0; j < NQ; ++j)
 #pragma unroll
-        for (int il = 0; il < NI; ++il)
-          gh[c][j][il] = (i0 + il < NQ)
-                             ? (T)gload(Gt + gq_index<TG, NQ2, NC>(tile, c, j * NQ + i0 + il, el, NE))
-                             : T(0);
+        for (int il = 0; il < NI; ++il) {
+          if constexpr (GS)
+            gh[c][j][il] = (T)gload(Gt + gq_split_index<NQ, NI, NC>(tile, c, j, il, t, NT));
+          else
+            gh[c][j][il] =
+                (i0 + il < NQ)
+                    ? (T)gload(Gt + gq_index<TG, NQ2, NC>(tile, c, j * NQ + i0 + il, el, NE))
+                    : T(0);
+        }
   }
   auto gl = [&](int c, int q, int j, int il) {
     if constexpr (PREF)
       return g[c][q];
     else if constexpr (PREFH)
       return gh[c][j][il];
+    else if constexpr (GS && SPLIT == 2)
+      return (T)gload(Gt + gq_split_index<NQ, NI, NC>(tile, c, j, il, t, NT));
     else
-      return (T)gload(Gt + gq_index<TG, NQ2, NC>(tile, c, q, el, NE));
+      return (T)gload(Gt + gq_at<TG, NQ, NC>(tile, c, q, el, NE, GS));
   };
   const T *wf = static_cast<const T *>(A.wf);
   stg.store(A, ex0, ey0, t, xs, tbx, tby);
@@ -1297,9 +1345,13 @@ __global__ void k_seam(T *y, int nex, int nfx, int nfy, int sx, int sy, int ntx,
 // G16 = G32 / scale, rounded to nearest
 template <int NQ2, int NC>
 __global__ void k_g16_from_g32(const float *__restrict__ G32, _Float16 *__restrict__ G16,
-                               int64_t n, int NT, float inv_scale) {
+                               int64_t n, int NT, float inv_scale, int linear) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
+    if (linear) {  // thread-major order (gq_split): the same index in both copies
+      G16[i] = (_Float16)(G32[i] * inv_scale);
+      continue;
+    }
     const int t = (int)(i % NT);
     const int64_t r = i / NT;
     const int q = (int)(r % NQ2);
@@ -1375,11 +1427,12 @@ __global__ void k_geometry(T *G, const double *tabx, const double *taby,
   constexpr int NQ2 = NQ * NQ, TAB = NQ * (P + 1);
   const int NT = TX * TY;
   const int tile = blockIdx.x;
+  const bool split = gq_split(sizeof(T), TX, TY, NQ);
   auto put = [&](int c, int q, int t, double v) {
     if (wts)
-      G[gq_index<T, NQ2, 5>(tile, c, q, t, NT)] = (T)v;
+      G[gq_at<T, NQ, 5>(tile, c, q, t, NT, split)] = (T)v;
     else
-      G[gq_index<T, NQ2, 3>(tile, c, q, t, NT)] = (T)v;
+      G[gq_at<T, NQ, 3>(tile, c, q, t, NT, split)] = (T)v;
   };
   const int nc = wts ? 5 : 3;
   for (int t = threadIdx.x; t < NT; t += blockDim.x) {

@@ -52,6 +52,12 @@ inline bool rat_compiled(int p, int id) {
 template <int P>
 struct Launch {
   static constexpr int NQ = P + 1;
+  // split-element kernels on thread-major geometry (gq_split) at three waves
+  // per SIMD (the register budget they then fit without spilling)
+  template <typename T>
+  static constexpr int split_minw() {
+    return gq_split(sizeof(T), 16, 8, NQ) ? 3 : 1;
+  }
 
   template <typename T, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1, int SPLIT = 1,
             typename TG = T>
@@ -101,7 +107,8 @@ struct Launch {
     if (variant == 4 && mfma_compiled(P, tile)) return apply_mfma_t<_Float16>(a, ntiles, s);
     if (variant == 5 && lds_compiled(P, tile, true)) return apply_lds_t<_Float16>(a, ntiles, s);
     if (tile == TILE_16x16) return apply_t<float, 16, 16, false, false, 1, 1, _Float16>(a, ntiles, s);
-    if (tile == TILE_16x8) return apply_t<float, 16, 8, false, false, 1, 2, _Float16>(a, ntiles, s);
+    if (tile == TILE_16x8)
+      return apply_t<float, 16, 8, false, false, split_minw<float>(), 2, _Float16>(a, ntiles, s);
     return hipErrorInvalidValue;
   }
   static hipError_t g16_from_g32(const float *G32, _Float16 *G16, const Layout &L,
@@ -111,7 +118,7 @@ struct Launch {
     int64_t nb = (n + 255) / 256;
     if (nb > 65536) nb = 65536;
     hipLaunchKernelGGL((k_g16_from_g32<NQ * NQ, 3>), dim3((unsigned)nb), dim3(256), 0, s, G32,
-                       G16, n, NT, inv_scale);
+                       G16, n, NT, inv_scale, gq_split(4, L.tx, L.ty, NQ) ? 1 : 0);
     return hipGetLastError();
   }
   template <typename T, bool EBE>
@@ -168,7 +175,7 @@ struct Launch {
                                hipStream_t s) {
     if (tile == TILE_16x8)
       return ebe ? apply_t<T, 16, 8, true>(a, ntiles, s)
-                 : apply_t<T, 16, 8, false, false, 1, 2>(a, ntiles, s);
+                 : apply_t<T, 16, 8, false, false, split_minw<T>(), 2>(a, ntiles, s);
     if constexpr (P <= 4 && sizeof(T) == 4)
       if (variant == 2 && !ebe && tile == TILE_16x16)
         return apply_t<float, 16, 16, false, false, 4>(a, ntiles, s);
@@ -254,9 +261,13 @@ struct Launch {
 // 16x16 tiles (and 32x16 for p <= 4); instantiated in kern_p<P>_rat.hip.
 template <int P>
 struct LaunchRat {
+  // the FP32 split-element kernel on thread-major geometry (gq_split: one
+  // load per row and component) at three waves per SIMD: it then fits 168
+  // VGPRs without spilling (182 VGPRs, two waves per SIMD, unbounded)
   template <typename T, int TX, int TY, bool EBE, int SPLIT = 1>
   static hipError_t run_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<T, P, P + 1, TX, TY, EBE, true, 1, SPLIT>), dim3(ntiles),
+    constexpr int MINW = (SPLIT == 2 && gq_split(sizeof(T), TX, TY, P + 1)) ? 3 : 1;
+    hipLaunchKernelGGL((k_apply<T, P, P + 1, TX, TY, EBE, true, MINW, SPLIT>), dim3(ntiles),
                        dim3(TX * TY * SPLIT), 0, s, a);
     return hipGetLastError();
   }
