@@ -109,6 +109,34 @@ def ev_time(fn, steps, warmup):
     return median(per), sum(per) / len(per)
 
 
+def stream_rates():
+    """Practical HBM rates of plain torch kernels on this device, as context
+    for the apply's fraction of the published peak: a sum over 1 GiB of FP32
+    (read-only) and a 512 MiB device copy (read + write), GB/s, best of 10
+    event-timed calls."""
+    import torch
+    x = torch.ones(256 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty(128 << 20, dtype=torch.float32, device="cuda")
+    out = {}
+    for name, fn, nbytes in (("read_gbs", lambda: x.sum(), x.numel() * 4),
+                             ("copy_gbs", lambda: y.copy_(x[:y.numel()]), 2 * y.numel() * 4)):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(10):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            best = min(best, a.elapsed_time(b))
+        out[name] = round(nbytes / (best * 1e-3) / 1e9, 1)
+    del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
 def kernel_avg_ms(op, x, y, steps, kern, asm):
     """Average duration of the apply kernel itself: HIP events recorded by
     the library around each k_apply launch on its stream."""
@@ -166,6 +194,8 @@ def headline(a, out):
     alg_bytes32 = 4 * (2 * nfunc) + 4 * 3 * E * nqp
     achieved = alg_bytes32 / (k_b2b * 1e-3) / 1e9
     gdofs = n_int / (wall32 * 1e-3) / 1e9
+    # (not in byte-counter runs: its copy would be taken for the calibration copy)
+    practical = None if a.calibrate_copy else stream_rates()
     variant = info.variant_f32
     tx, ty = info.tile_x_f32, info.tile_y_f32
     if variant == 1 and a.kernel == "sf":
@@ -211,7 +241,8 @@ def headline(a, out):
                      "kernel_avg_ms": round(k_b2b, 5),
                      "kernel_avg_ms_event_per_launch": round(k_avg, 5),
                      "kernel_min_ms_event_per_launch": round(k_min, 5),
-                     "algorithmic_bytes": alg_bytes32},
+                     "algorithmic_bytes": alg_bytes32,
+                     "practical_torch": practical},
         "apply": {"f32_wall_ms_per_step": round(wall32, 5),
                   "f32_event_median_ms": round(med32, 5), "f32_event_mean_ms": round(mean32, 5),
                   "setup_s": round(setup_s, 3), "op_setup_ms": round(info.setup_ms, 2),
@@ -219,7 +250,9 @@ def headline(a, out):
     })
     log(f"headline f32 apply {wall32 * 1e3:.1f} us/step, kernel {k_b2b * 1e3:.1f} us "
         f"({k_avg * 1e3:.1f} us with an event pair per launch), "
-        f"{achieved / HBM_PEAK_GBS:.3f} of HBM")
+        f"{achieved / HBM_PEAK_GBS:.3f} of HBM" +
+        (f"; torch sum {practical['read_gbs']:.0f} GB/s, copy {practical['copy_gbs']:.0f} GB/s"
+         if practical else ""))
     return mesh, op, x64, x32, kern, asm, n_int
 
 

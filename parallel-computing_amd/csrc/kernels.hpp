@@ -723,6 +723,18 @@ __device__ __forceinline__ void cg_handoff(const ApplyArgs &A, int nt) {
 // of high degrees and FP64.
 // TG: storage type of G_q (T, or _Float16 for the FP16-geometry FP32 apply:
 // values divided by A.gscale, the element result multiplied back by it).
+// Row stride of the x table of a split-element kernel: its two threads of an
+// element read rows NI*NB apart, which with a stride of 16 FP32 values land
+// in the same LDS banks (a two-way conflict on every table read); the
+// smallest stride >= TX that moves them 16 banks apart removes it.
+template <typename T, int NB, int NI, int TX, int SPLIT>
+constexpr int xtab_stride() {
+  if (SPLIT != 2 || sizeof(T) != 4) return TX;
+  for (int L = TX; L < TX + 32; ++L)
+    if ((NI * NB * L) % 32 == 16) return L;
+  return TX;
+}
+
 template <typename T, int P, int NQ, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1,
           int SPLIT = 1, typename TG = T>
 __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
@@ -730,8 +742,9 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   constexpr int NB = P + 1, NE = TX * TY, NT = NE * SPLIT, FX = TX + P, FY = TY + P;
   constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY;
   constexpr bool ROW16 = TX == 16 && SPLIT == 1;
+  constexpr int LXT = xtab_stride<T, NB, (NQ + SPLIT - 1) / SPLIT, TX, SPLIT>();
   __shared__ T xs[NFP];
-  __shared__ T tbx[2 * TAB][TX];
+  __shared__ T tbx[2 * TAB][LXT];
   __shared__ T tby[2 * TAB][TY];
   __shared__ T bufm[ROW16 ? row16_lds<P, TY>() : NB * NE];
   T(*buf)[NE] = reinterpret_cast<T(*)[NE]>(bufm);
@@ -809,7 +822,7 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
       return (T)gload(Gt + gq_at<TG, NQ, NC>(tile, c, q, el, NE, GS));
   };
   const T *wf = static_cast<const T *>(A.wf);
-  stg.store(A, ex0, ey0, t, xs, tbx, tby);
+  stg.template store<LXT, TY>(A, ex0, ey0, t, xs, tbx, tby);
   __syncthreads();
 
   T ye[NB][NB];
@@ -819,10 +832,10 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int a = 0; a < NB; ++a) X[b][a] = xs[(ly + b) * FX + lx + a];
-    if (EBE)
-      element_ebe<T, P, NQ, TX, TY, RAT>(X, tbx, tby, lx, ly, gl, ye);
+    if constexpr (EBE)
+      element_ebe<T, P, NQ, LXT, TY, RAT>(X, tbx, tby, lx, ly, gl, ye);
     else
-      element_sf<T, P, NQ, TX, TY, RAT, NI>(X, tbx, tby, lx, ly, gl, ye, h * NI);
+      element_sf<T, P, NQ, LXT, TY, RAT, NI>(X, tbx, tby, lx, ly, gl, ye, h * NI);
     if constexpr (SPLIT == 2) {
       // the pair (lanes 2e, 2e+1) adds its halves: quad_perm [1,0,3,2]
 #pragma unroll
