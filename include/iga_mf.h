@@ -250,9 +250,11 @@ typedef struct {
                                    global-to-LDS loads (SF, p <= 3, 16x16, not
                                    general rational; its FP16-G_q form at
                                    p = 3; EBE applies use 0);
-                                   -1 (default): 3 for FP32 at p >= 5 and for
-                                   FP64 at p = 3 (not general rational),
-                                   else 0 */
+                                   -1 (default): 3 for FP32 at p = 3 (not
+                                   general rational) and p >= 5, and for FP64
+                                   at p = 3 (not general rational); 0 when a
+                                   tile other than 16x8 is given, and
+                                   otherwise */
   int f16_geometry;   /* also keep an FP16 copy of the FP32 G_q, divided by a
                          power of two that maps their largest magnitude to
                          2^14 (IGA_F32_G16 applies; default 0) */

@@ -483,11 +483,17 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   op->want64 = o.want_f64 != 0;
   op->want32 = o.want_f32 != 0;
   // defaults measured on MI355X (quarter annulus 1024^2 / 512^2, p = 2..5):
-  // one workgroup per 16x16 tile, except two threads per element where the
-  // per-thread state is large -- FP32 at p >= 5 and FP64 at p = 3
+  // one workgroup per 16x16 tile, except two threads per element on 16x8
+  // tiles where the per-thread state is large -- FP32 at p >= 5 and FP64 at
+  // p = 3 -- and for FP32 at p = 3 (the same kernel time as one thread per
+  // element, 34.9 against 35.0 us at 1024^2, and FP32 refinement solves 2 %
+  // faster, 94.7 against 96.5 ms); an explicitly chosen tile other than
+  // 16x8 keeps variant 0
   auto dflt_variant = [&](bool f64) {
+    const int tx = f64 ? o.tile_x_f64 : o.tile_x_f32, ty = f64 ? o.tile_y_f64 : o.tile_y_f32;
+    if ((tx || ty) && !(tx == 16 && ty == 8)) return 0;
     if (f64) return (p == 3 && rational != 2) ? 3 : 0;
-    return p >= 5 ? 3 : 0;
+    return (p >= 5 || (p == 3 && rational != 2)) ? 3 : 0;
   };
   op->variant64 = o.variant_f64 < 0 ? dflt_variant(true) : o.variant_f64;
   op->variant32 = o.variant_f32 < 0 ? dflt_variant(false) : o.variant_f32;
