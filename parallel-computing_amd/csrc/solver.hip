@@ -135,14 +135,22 @@ __device__ __forceinline__ double fold(const double *slot, int nb) {
 // This thread's share of the sum of n partials, eight loads in flight at a
 // time (a serial chain of dependent loads would expose the L2 latency once
 // per partial); fixed order.
+template <int NL = 8>
 __device__ __forceinline__ double fold_share(const double *__restrict__ a, int n) {
-  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  int i = threadIdx.x;
-  for (; i + 7 * RB < n; i += 8 * RB) {
+  static_assert(NL == 8 || NL == 16, "8 or 16 loads in flight");
+  double acc[NL];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] += a[i + k * RB];
+  for (int k = 0; k < NL; ++k) acc[k] = 0.0;
+  int i = threadIdx.x;
+  for (; i + (NL - 1) * RB < n; i += NL * RB) {
+#pragma unroll
+    for (int k = 0; k < NL; ++k) acc[k] += a[i + k * RB];
   }
   for (; i < n; i += RB) acc[0] += a[i];
+  if constexpr (NL == 16) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] += acc[k + 8];
+  }
   return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
@@ -305,7 +313,10 @@ __global__ __launch_bounds__(RB) void k_cg_update(TU *__restrict__ u, V *__restr
   SP sp[U];
   if (b0 < n) load(b0, uu, pv, rv, av, dv, sp);
   // p.Ap: partials of k_cg_pap, or the apply kernel's per-tile partials
-  const double pap = tdot ? block_sum_all(fold_share(tdot, ntiles)) : fold(part + P_PAP * RG, nb);
+  // (FP32: the split-element apply's 8,192 tile partials at 1024^2, sixteen
+  // loads in flight)
+  const double pap = tdot ? block_sum_all(fold_share<sizeof(V) == 4 ? 16 : 8>(tdot, ntiles))
+                          : fold(part + P_PAP * RG, nb);
   const double a = sc[S_RZ] / pap;
   if (!finite(a) || pap == 0.0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ic[I_DONE] = 3;
