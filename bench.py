@@ -109,6 +109,20 @@ def ev_time(fn, steps, warmup):
     return median(per), sum(per) / len(per)
 
 
+def settle(fn, seconds=0.3):
+    """Untimed calls of fn for about `seconds`, so that the timing that
+    follows sees the device's steady clocks: after seconds of host-side work
+    (mesh generation, set-up) the device has idled and its clocks have
+    dropped, and the first few hundred calls of a hot loop run several per
+    cent slower."""
+    import torch
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(20):
+            fn()
+        torch.cuda.synchronize()
+
+
 def stream_rates():
     """Practical HBM rates of plain torch kernels on this device, as context
     for the apply's fraction of the published peak: a sum over 1 GiB of FP32
@@ -174,15 +188,10 @@ def headline(a, out):
         torch.cuda.synchronize()
         del src, dst
     y32 = torch.empty_like(x32)
-    # the GPU's clocks rise over the first few hundred microseconds of load:
     # 0.3 s of untimed applies before the W warmup steps, so that the timed
-    # steps see the steady state (the first 200 applies of a fresh process
-    # otherwise ran about 7 % slower per step)
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < 0.3:
-        for _ in range(50):
-            op.apply(x32, y32, kernel=kern, assembly=asm)
-        torch.cuda.synchronize()
+    # steps see the device's steady clocks (the first 200 applies of a fresh
+    # process otherwise ran about 7 % slower per step)
+    settle(lambda: op.apply(x32, y32, kernel=kern, assembly=asm))
     # the contract's timing: W untimed steps, then K steps between two
     # synchronisations, wall clock
     for _ in range(a.warmup):
@@ -657,8 +666,9 @@ def c5(out):
     x32 = x.float()
     y32 = torch.empty_like(x32)
     y64 = torch.empty_like(x)
-    t32, _ = ev_time(lambda: op.apply(x32, y32), 10, 2)
-    k32 = op.time_kernel(x32, torch.empty_like(x32), 10)
+    settle(lambda: op.apply(x32, y32))  # after seconds of host-side mesh set-up
+    t32, _ = ev_time(lambda: op.apply(x32, y32), 20, 2)
+    k32 = op.time_kernel(x32, torch.empty_like(x32), 30)
     t64e, _ = ev_time(lambda: op.apply(x, y64, kernel=igamf.EBE), 5, 1)
     nc = 5 if info.rational == 2 else 3
     b32 = 4 * (2 * op.nfunc) + 4 * nc * op.nelem * op.nqp + 4 * op.nfunc

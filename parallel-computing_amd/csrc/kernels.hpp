@@ -743,10 +743,13 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   constexpr int NQ2 = NQ * NQ, TAB = NQ * NB, NFP = FX * FY;
   constexpr bool ROW16 = TX == 16 && SPLIT == 1;
   constexpr int LXT = xtab_stride<T, NB, (NQ + SPLIT - 1) / SPLIT, TX, SPLIT>();
+  // split-element tiles reduce the footprint in one pass: every element
+  // vector to LDS at once ([b][a][element]), one barrier
+  constexpr bool ONEPASS = SPLIT == 2;
   __shared__ T xs[NFP];
   __shared__ T tbx[2 * TAB][LXT];
   __shared__ T tby[2 * TAB][TY];
-  __shared__ T bufm[ROW16 ? row16_lds<P, TY>() : NB * NE];
+  __shared__ T bufm[ROW16 ? row16_lds<P, TY>() : (ONEPASS ? NB * NB * NE : NB * NE)];
   T(*buf)[NE] = reinterpret_cast<T(*)[NE]>(bufm);
 
   if (A.skip && *A.skip) return;
@@ -870,8 +873,38 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   T acc[NOWN];
 #pragma unroll
   for (int o = 0; o < NOWN; ++o) acc[o] = T(0);
+  if constexpr (ONEPASS) {
+    // both threads of an element hold its summed vector: half h writes the
+    // rows b of parity h; the sums then run in the order of the passes below
+    T(*bb)[NB][NE] = reinterpret_cast<T(*)[NB][NE]>(bufm);
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
+    for (int b = 0; b < NB; ++b)
+      if ((b & 1) == h) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a) bb[b][a][el] = ye[b][a];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int k = t + o * NT;
+      if (k < NFP) {
+        const int fx = k % FX, fy = k / FX;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int el_y = fy - b;
+          if (el_y >= 0 && el_y < TY) {
+#pragma unroll
+            for (int a = 0; a < NB; ++a) {
+              const int el_x = fx - a;
+              if (el_x >= 0 && el_x < TX) acc[o] += bb[b][a][el_y * TX + el_x];
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < (ONEPASS ? 0 : NB); ++b) {
     if (b > 0) __syncthreads();
     if (h == 0) {
 #pragma unroll
