@@ -683,6 +683,7 @@ def c5(out):
         "workload": f"unit square p={p}, {ne}x{ne}, perturbed, weights U[0.8,1.2] "
                     "(general rational kernels)",
         "interior_dofs": n_int, "rational_mode": info.rational,
+        "f32_streamed_geometry_loads": bool(info.stream_g_f32),
         "f32_sf_ms": round(t32, 4), "f32_sf_gdofs": round(n_int / (t32 * 1e-3) / 1e9, 4),
         "f32_kernel_ms": round(k32, 4),
         "f32_hbm_frac": round(b32 / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

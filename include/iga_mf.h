@@ -258,6 +258,12 @@ typedef struct {
   int f16_geometry;   /* also keep an FP16 copy of the FP32 G_q, divided by a
                          power of two that maps their largest magnitude to
                          2^14 (IGA_F32_G16 applies; default 0) */
+  int stream_g_f32;   /* FP32 SF applies on 16x8 tiles (variant 3): 1 load
+                         G_q non-temporally, 0 with normal loads, -1
+                         (default) non-temporally when the FP32 G_q take
+                         1 GiB or more (four times the device's last-level
+                         cache, so that none of them stays cached from one
+                         apply to the next); the result is the same */
 } iga_op_opts;
 
 typedef struct {
@@ -269,6 +275,7 @@ typedef struct {
   int variant_f64, variant_f32; /* apply variant in use (see iga_op_opts) */
   int f16_geometry;           /* the FP16 copy of G_q exists */
   double f16_scale;           /* FP16 G_q = G_q / f16_scale (a power of two) */
+  int stream_g_f32;           /* FP32 SF applies load G_q non-temporally */
 } iga_op_info;
 
 void iga_op_opts_default(iga_op_opts *o);

@@ -238,6 +238,7 @@ int apply_raw_dt(iga_op *op, int dt, bool ebe, bool gather, const void *x, void 
   a.mask_out = mask_out;
   a.skip = skip;
   a.rev = op->rev_next;
+  a.ntg = dt == IGA_F32 && op->stream32;
   op->rev_next ^= 1;
   if (!a.G) return IGA_ERR_UNSUPPORTED;
   const int ntiles = L.ntx * L.nty;
@@ -356,6 +357,7 @@ void iga_op_opts_default(iga_op_opts *o) {
   o->want_f32 = 1;
   o->variant_f64 = -1;
   o->variant_f32 = -1;
+  o->stream_g_f32 = -1;
 }
 
 int iga_op_destroy(iga_op *op) {
@@ -497,7 +499,8 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
   };
   op->variant64 = o.variant_f64 < 0 ? dflt_variant(true) : o.variant_f64;
   op->variant32 = o.variant_f32 < 0 ? dflt_variant(false) : o.variant_f32;
-  if (o.variant_f64 < -1 || op->variant64 > 3 || o.variant_f32 < -1 || op->variant32 > 5) {
+  if (o.variant_f64 < -1 || op->variant64 > 3 || o.variant_f32 < -1 || op->variant32 > 5 ||
+      o.stream_g_f32 < -1 || o.stream_g_f32 > 1) {
     delete op;
     return IGA_ERR_INVALID_ARGUMENT;
   }
@@ -615,6 +618,14 @@ int iga_op_create(const iga_mesh_desc *m, const iga_op_opts *o_in, iga_op **out)
     const igak::Layout &L = op->L32;
     const size_t n = (size_t)L.ntx * L.nty * op->nc * nqp * L.tx * L.ty;
     ALLOC(op->G32, n * 4);
+    // streamed G_q loads once the factors exceed four times the device's
+    // 256 MB last-level cache: then no part of them survives from one apply
+    // to the next, and normal loads only evict x and the slots (2048^2
+    // elements, p = 5 general rational, 3.0 GB: 650 against 670 us per
+    // apply; at 0.4-0.9 GB normal loads are as fast or faster)
+    constexpr size_t kStreamBytes = (size_t)1 << 30;
+    op->stream32 = op->tile32 == igak::TILE_16x8 &&
+                   (o.stream_g_f32 < 0 ? n * 4 >= kStreamBytes : o.stream_g_f32 != 0);
     if (hipSuccess != ops->geom32(op->G32, op->tabx64, op->taby64, op->cp, op->wts64, op->wq,
                                   op->kappa, nex, ney, nfx, L, s))
       return fail(IGA_ERR_HIP);
@@ -660,6 +671,7 @@ int iga_op_get_info(const iga_op *op, iga_op_info *info) {
   info->variant_f32 = op->variant32;
   info->f16_geometry = op->G16 != nullptr;
   info->f16_scale = op->g16_scale;
+  info->stream_g_f32 = op->stream32;
   return IGA_OK;
 }
 
@@ -877,6 +889,7 @@ int iga_op_time_kernel(iga_op *op, int dt, int kern, int asmb, const void *x, vo
   a.mask_in = 0;
   a.mask_out = 1;
   a.skip = nullptr;
+  a.ntg = dt == IGA_F32 && op->stream32;
   const bool ebe = kern == IGA_KERNEL_EBE;
   const int ntiles = L.ntx * L.nty;
   const int tile = f64 ? op->tile64 : op->tile32;

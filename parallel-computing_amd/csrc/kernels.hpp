@@ -65,6 +65,8 @@ struct ApplyArgs {
   int hn;
   float gscale;           // FP16-stored G_q: the power of two they were divided by
   int rev;                // k_apply: visit the tiles in reverse order
+  int ntg;                // host side: the FP32 split-element kernels' streaming
+                          // instantiation (NTG) loads G_q
 };
 
 __device__ __forceinline__ bool is_dirichlet(const uint8_t *mask, int A, int gx,
@@ -175,8 +177,12 @@ struct Vec16<_Float16> {
 // loaded non-temporally: the stream, read once per apply, then does not
 // evict the vectors the surrounding solver kernels re-read every iteration
 // (the FP64 CG solve of that mesh measured 7% faster).  FP32 factors use
-// normal loads: part of them stays in that cache from one apply to the next,
-// and streaming them measured slower (45.7 against 54.7 us per apply).
+// normal loads by default: part of them stays in that cache from one apply
+// to the next (consecutive applies visit the tiles in opposite orders), and
+// streaming them measured slower (45.7 against 54.7 us per apply).  The
+// split-element FP32 kernels also have a streaming instantiation (NTG), for
+// factors several times larger than that cache, where nothing survives
+// from one apply to the next and the stream only evicts x and the slots.
 template <bool NT, typename TG>
 __device__ __forceinline__ TG gload_t(const TG *p) {
   if constexpr (NT)
@@ -193,7 +199,7 @@ __device__ __forceinline__ TG gload(const TG *p) {
 }
 
 // G_q of element t of a tile into registers of type T (TG: storage type)
-template <typename TG, int NQ2, int NC = 3, typename T>
+template <typename TG, int NQ2, int NC = 3, bool NTG = kStreamG<TG>, typename T>
 __device__ __forceinline__ void load_gq(const TG *G, int tile, int t, int NT, T (&g)[NC][NQ2]) {
   constexpr int VW = 16 / sizeof(TG);
   constexpr int NV = NQ2 / VW;
@@ -203,13 +209,13 @@ __device__ __forceinline__ void load_gq(const TG *G, int tile, int t, int NT, T 
     const TG *Gc = G + ((size_t)tile * NC + c) * NQ2 * NT;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      const V v = gload(reinterpret_cast<const V *>(Gc + ((size_t)k * NT + t) * VW));
+      const V v = gload_t<NTG>(reinterpret_cast<const V *>(Gc + ((size_t)k * NT + t) * VW));
 #pragma unroll
       for (int l = 0; l < VW; ++l) g[c][k * VW + l] = (T)v[l];
     }
 #pragma unroll
     for (int r = NV * VW; r < NQ2; ++r)
-      g[c][r] = (T)gload(Gc + (size_t)NV * VW * NT + (size_t)(r - NV * VW) * NT + t);
+      g[c][r] = (T)gload_t<NTG>(Gc + (size_t)NV * VW * NT + (size_t)(r - NV * VW) * NT + t);
   }
 }
 
@@ -736,7 +742,7 @@ constexpr int xtab_stride() {
 }
 
 template <typename T, int P, int NQ, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1,
-          int SPLIT = 1, typename TG = T>
+          int SPLIT = 1, typename TG = T, bool NTG = kStreamG<TG>>
 __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   static_assert(SPLIT == 1 || !EBE, "split elements are sum-factorised");
   constexpr int NB = P + 1, NE = TX * TY, NT = NE * SPLIT, FX = TX + P, FY = TY + P;
@@ -790,12 +796,12 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
   // storage through gq_at
   constexpr bool GS = gq_split(sizeof(TG), TX, TY, NQ);
   static_assert(!GS || SPLIT == 2 || EBE, "thread-major factors: split kernels");
-  if constexpr (PREF && !GS) load_gq<TG, NQ2, NC>(Gt, tile, el, NE, g);
+  if constexpr (PREF && !GS) load_gq<TG, NQ2, NC, NTG>(Gt, tile, el, NE, g);
   if constexpr (PREF && GS) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int q = 0; q < NQ2; ++q) g[c][q] = (T)gload(Gt + gq_at<TG, NQ, NC>(tile, c, q, el, NE, true));
+      for (int q = 0; q < NQ2; ++q) g[c][q] = (T)gload_t<NTG>(Gt + gq_at<TG, NQ, NC>(tile, c, q, el, NE, true));
   }
   if constexpr (PREFH) {
     const int i0 = h * NI;
@@ -806,11 +812,11 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
 #pragma unroll
         for (int il = 0; il < NI; ++il) {
           if constexpr (GS)
-            gh[c][j][il] = (T)gload(Gt + gq_split_index<NQ, NI, NC>(tile, c, j, il, t, NT));
+            gh[c][j][il] = (T)gload_t<NTG>(Gt + gq_split_index<NQ, NI, NC>(tile, c, j, il, t, NT));
           else
             gh[c][j][il] =
                 (i0 + il < NQ)
-                    ? (T)gload(Gt + gq_index<TG, NQ2, NC>(tile, c, j * NQ + i0 + il, el, NE))
+                    ? (T)gload_t<NTG>(Gt + gq_index<TG, NQ2, NC>(tile, c, j * NQ + i0 + il, el, NE))
                     : T(0);
         }
   }
@@ -820,9 +826,9 @@ __global__ __launch_bounds__(TX *TY *SPLIT, MINW) void k_apply(ApplyArgs A) {
     else if constexpr (PREFH)
       return gh[c][j][il];
     else if constexpr (GS && SPLIT == 2)
-      return (T)gload(Gt + gq_split_index<NQ, NI, NC>(tile, c, j, il, t, NT));
+      return (T)gload_t<NTG>(Gt + gq_split_index<NQ, NI, NC>(tile, c, j, il, t, NT));
     else
-      return (T)gload(Gt + gq_at<TG, NQ, NC>(tile, c, q, el, NE, GS));
+      return (T)gload_t<NTG>(Gt + gq_at<TG, NQ, NC>(tile, c, q, el, NE, GS));
   };
   const T *wf = static_cast<const T *>(A.wf);
   stg.template store<LXT, TY>(A, ex0, ey0, t, xs, tbx, tby);

@@ -60,9 +60,9 @@ struct Launch {
   }
 
   template <typename T, int TX, int TY, bool EBE, bool RAT = false, int MINW = 1, int SPLIT = 1,
-            typename TG = T>
+            typename TG = T, bool NTG = kStreamG<TG>>
   static hipError_t apply_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
-    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE, RAT, MINW, SPLIT, TG>), dim3(ntiles),
+    hipLaunchKernelGGL((k_apply<T, P, NQ, TX, TY, EBE, RAT, MINW, SPLIT, TG, NTG>), dim3(ntiles),
                        dim3(TX * TY * SPLIT), 0, s, a);
     return hipGetLastError();
   }
@@ -173,9 +173,13 @@ struct Launch {
   template <typename T>
   static hipError_t apply_prec(const ApplyArgs &a, bool ebe, int tile, int ntiles, int variant,
                                hipStream_t s) {
-    if (tile == TILE_16x8)
+    if (tile == TILE_16x8) {
+      if constexpr (sizeof(T) == 4)
+        if (!ebe && a.ntg)
+          return apply_t<T, 16, 8, false, false, split_minw<T>(), 2, T, true>(a, ntiles, s);
       return ebe ? apply_t<T, 16, 8, true>(a, ntiles, s)
                  : apply_t<T, 16, 8, false, false, split_minw<T>(), 2>(a, ntiles, s);
+    }
     if constexpr (P <= 4 && sizeof(T) == 4)
       if (variant == 2 && !ebe && tile == TILE_16x16)
         return apply_t<float, 16, 16, false, false, 4>(a, ntiles, s);
@@ -267,6 +271,12 @@ struct LaunchRat {
   template <typename T, int TX, int TY, bool EBE, int SPLIT = 1>
   static hipError_t run_t(const ApplyArgs &a, int ntiles, hipStream_t s) {
     constexpr int MINW = (SPLIT == 2 && gq_split(sizeof(T), TX, TY, P + 1)) ? 3 : 1;
+    if constexpr (SPLIT == 2 && sizeof(T) == 4)
+      if (a.ntg) {
+        hipLaunchKernelGGL((k_apply<T, P, P + 1, TX, TY, EBE, true, MINW, SPLIT, T, true>),
+                           dim3(ntiles), dim3(TX * TY * SPLIT), 0, s, a);
+        return hipGetLastError();
+      }
     hipLaunchKernelGGL((k_apply<T, P, P + 1, TX, TY, EBE, true, MINW, SPLIT>), dim3(ntiles),
                        dim3(TX * TY * SPLIT), 0, s, a);
     return hipGetLastError();

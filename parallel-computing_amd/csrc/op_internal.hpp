@@ -42,6 +42,9 @@ struct iga_op {
   double *G64 = nullptr;            // kappa-weighted
   double *G0 = nullptr;             // geometry only (== G64 when kappa unset)
   float *G32 = nullptr;
+  // the FP32 split-element SF kernels load G32 non-temporally (their
+  // streaming instantiation; see iga_op_opts.stream_g_f32)
+  bool stream32 = false;
   _Float16 *G16 = nullptr;          // FP16 copy of G32 / g16_scale (f16_geometry)
   float g16_scale = 1.0f;
   unsigned *g16max = nullptr;       // device word: bits of max |G32| (FP16 scale)

@@ -53,7 +53,8 @@ class MeshParams(C.Structure):
 class OpOpts(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("device", "want_f64", "want_f32", "tile_x_f64",
                                        "tile_y_f64", "tile_x_f32", "tile_y_f32",
-                                       "variant_f64", "variant_f32", "f16_geometry")]
+                                       "variant_f64", "variant_f32", "f16_geometry",
+                                       "stream_g_f32")]
 
 
 class OpInfo(C.Structure):
@@ -64,7 +65,8 @@ class OpInfo(C.Structure):
                                                          ("variant_f64", C.c_int),
                                                          ("variant_f32", C.c_int),
                                                          ("f16_geometry", C.c_int),
-                                                         ("f16_scale", C.c_double)]
+                                                         ("f16_scale", C.c_double),
+                                                         ("stream_g_f32", C.c_int)]
 
 
 class SolverOpts(C.Structure):
@@ -433,7 +435,8 @@ class Operator:
     """The matrix-free operator y = A x on one GPU (iga_op)."""
 
     def __init__(self, mesh, device=0, want_f64=True, want_f32=True, tile_f64=None,
-                 tile_f32=None, kappa=None, variant_f64=-1, variant_f32=-1, f16_geometry=False):
+                 tile_f32=None, kappa=None, variant_f64=-1, variant_f32=-1, f16_geometry=False,
+                 stream_g_f32=-1):
         import torch
         self.mesh = mesh
         o = OpOpts()
@@ -445,6 +448,7 @@ class Operator:
         if tile_f32:
             o.tile_x_f32, o.tile_y_f32 = tile_f32
         o.variant_f64, o.variant_f32 = variant_f64, variant_f32
+        o.stream_g_f32 = stream_g_f32
         desc = mesh.desc
         keep = None
         if kappa is not None:

@@ -328,6 +328,40 @@ def test_lds_staged_variant_unsupported():
         igamf.Operator(m, variant_f64=5)
 
 
+STREAM_CASES = [c for c in CASES if c[0] in ("annulus_p3_n24", "annulus_p5_n12",
+                                             "rational_p5_n12", "square_p6_n9")]
+
+
+@pytest.mark.parametrize("name,kw", STREAM_CASES, ids=[c[0] for c in STREAM_CASES])
+def test_streamed_geometry_loads(built, name, kw):
+    """The FP32 split-element SF kernels' streaming instantiation (G_q loaded
+    non-temporally; the default from 1 GiB of FP32 G_q) gives bitwise the
+    results of normal loads, both assemblies, x.Ax included; small meshes
+    default to normal loads; EBE applies keep their one kernel."""
+    m, om, op0 = built(name, kw)
+    assert op0.info.stream_g_f32 == 0
+    ops = {k: igamf.Operator(m, stream_g_f32=k) for k in (0, 1)}
+    assert (ops[0].info.stream_g_f32, ops[1].info.stream_g_f32) == (0, 1)
+    assert (ops[1].info.tile_x_f32, ops[1].info.tile_y_f32) == (16, 8)
+    x32 = torch.from_numpy(igamf.rng_normal(12, op0.nfunc)).cuda().float()
+    y_ref = O.apply_mf(om, x32.double().cpu().numpy())
+    for asm in (igamf.IEN_ATOMIC, igamf.INVLM_GATHER):
+        for kern in (igamf.SF, igamf.EBE):
+            y1 = ops[1].apply(x32, kernel=kern, assembly=asm)
+            assert rel(y1.double().cpu().numpy(), y_ref) < TOL32, (name, asm, kern)
+            if asm == igamf.INVLM_GATHER:
+                assert torch.equal(y1, ops[0].apply(x32, kernel=kern, assembly=asm))
+        y1, d1 = ops[1].apply_dot(x32, assembly=asm)
+        y0, d0 = ops[0].apply_dot(x32, assembly=asm)
+        if asm == igamf.INVLM_GATHER:
+            assert torch.equal(y1, y0) and d1.item() == d0.item()
+    # no split kernel on another tile: the option changes nothing there
+    op16 = igamf.Operator(m, stream_g_f32=1, tile_f32=(16, 16))
+    assert op16.info.stream_g_f32 == 0
+    with pytest.raises(igamf.IgaError):
+        igamf.Operator(m, stream_g_f32=2)
+
+
 def test_general_rational_mode_and_tiles():
     """Non-tensor weights select the general rational kernels (5 geometry
     components per point); both of their tiles agree with the oracle; the

@@ -185,6 +185,18 @@ def test_solver_opts_defaults():
     assert o.pc_inner_abstol == 0.0
 
 
+def test_op_opts_defaults():
+    """iga_op_opts_default (host code only): automatic variants and
+    automatic streamed FP32 G_q loads; the last field doubles as the layout
+    check of the ctypes mirror."""
+    o = igamf.OpOpts()
+    o.stream_g_f32 = 7
+    igamf.lib().iga_op_opts_default(C.byref(o))
+    assert (o.want_f64, o.want_f32, o.variant_f64, o.variant_f32) == (1, 1, -1, -1)
+    assert (o.tile_x_f32, o.tile_y_f32, o.f16_geometry) == (0, 0, 0)
+    assert o.stream_g_f32 == -1
+
+
 def test_binding_rejects_bad_vectors_before_any_device_call():
     """The C ABI trusts bare device pointers: the binding checks dtype,
     device, contiguity and size first (no library call is reached here)."""
