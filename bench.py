@@ -227,7 +227,10 @@ def headline(a, out):
         minw = 4 if (variant == 2 and a.kernel == "sf") else (
             3 if (split == 2 and (tx, ty) == (16, 8) and (p + 1) % 2 == 0) else 1)
         ebe = "true" if a.kernel == "ebe" else "false"
-        kname = f"k_apply<float, {p}, {p + 1}, {tx}, {ty}, {ebe}, false, {minw}, {split}, float>"
+        # last argument: G_q loaded non-temporally (large factors, split kernels)
+        ntg = "true" if (split == 2 and info.stream_g_f32) else "false"
+        kname = (f"k_apply<float, {p}, {p + 1}, {tx}, {ty}, {ebe}, false, {minw}, {split}, "
+                 f"float, {ntg}>")
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
