@@ -154,6 +154,39 @@ __device__ __forceinline__ double fold_share(const double *__restrict__ a, int n
   return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
+// The same with 16-byte loads of consecutive pairs, NL in flight: the
+// 8,192 per-tile partials of the FP32 split-element apply at 1024^2 take
+// one round of loads per thread instead of two (each round waits for the
+// last-level cache, where the apply left them); fixed order.  a: 16-byte
+// aligned.
+template <int NL>
+__device__ __forceinline__ double fold_pairs(const double *__restrict__ a, int n) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 *a2 = reinterpret_cast<const d2 *>(a);
+  const int m = n >> 1;
+  double acc[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) acc[k] = 0.0;
+  int i = threadIdx.x;
+  for (; i + (NL - 1) * RB < m; i += NL * RB) {
+    d2 v[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) v[k] = a2[i + k * RB];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) acc[k] += v[k].x + v[k].y;
+  }
+  for (; i < m; i += RB) {
+    const d2 v = a2[i];
+    acc[0] += v.x + v.y;
+  }
+  if ((n & 1) && threadIdx.x == 0) acc[0] += a[n - 1];
+#pragma unroll
+  for (int w = NL / 2; w > 0; w >>= 1)
+#pragma unroll
+    for (int k = 0; k < w; ++k) acc[k] += acc[k + w];
+  return acc[0];
+}
+
 __device__ __forceinline__ bool finite(double x) { return x == x && !isinf(x); }
 
 #define GRID_LOOP(i, n) \
@@ -314,8 +347,9 @@ __global__ __launch_bounds__(RB) void k_cg_update(TU *__restrict__ u, V *__restr
   if (b0 < n) load(b0, uu, pv, rv, av, dv, sp);
   // p.Ap: partials of k_cg_pap, or the apply kernel's per-tile partials
   // (FP32: the split-element apply's 8,192 tile partials at 1024^2, sixteen
-  // loads in flight)
-  const double pap = tdot ? block_sum_all(fold_share<sizeof(V) == 4 ? 16 : 8>(tdot, ntiles))
+  // 16-byte loads in flight)
+  const double pap = tdot ? block_sum_all(sizeof(V) == 4 ? fold_pairs<16>(tdot, ntiles)
+                                                          : fold_share<8>(tdot, ntiles))
                           : fold(part + P_PAP * RG, nb);
   const double a = sc[S_RZ] / pap;
   if (!finite(a) || pap == 0.0) {
