@@ -1123,12 +1123,12 @@ struct iga_solver {
   std::vector<void *> allocs;
   int gen = -1;  // operator generation the Jacobi diagonal was built for
   // graphs of one chunk of iterations (FP64 PCG, FP32 inner PCG)
-  hipGraphExec_t g64 = nullptr, g32 = nullptr;
+  hipGraphExec_t g64[2] = {}, g32[2] = {};  // one per flag slot (run_chunks)
   // fused iteration: double-buffered recursion state, per-tile u.Au partials,
   // graphs of one chunk
   double *cgs = nullptr, *tdot = nullptr;
   int ntiles64 = 0, ntiles32 = 0;
-  hipGraphExec_t gf64 = nullptr, gf32 = nullptr;
+  hipGraphExec_t gf64[2] = {}, gf32[2] = {};
   // block solve (k columns of nfunc): FP64 iterates, FP32 inner vectors,
   // per-column scalars / flags / partial slots, graph of one chunk
   int bk = 0;
@@ -1144,6 +1144,8 @@ struct iga_solver {
   // flight, and the events that mark them complete (run_chunks)
   int *hflag = nullptr;
   double *hsc = nullptr;  // [2][S_NS] the main CG's scalars after those chunks
+  int *dflag = nullptr;   // the same two buffers' device addresses (k_chunk_flags)
+  double *dsc = nullptr;
   hipEvent_t fev[2] = {nullptr, nullptr};
 };
 
@@ -1313,8 +1315,9 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
 }
 
 // Enqueue chunks (launch(nit): nit iterations) until the device's done flag
-// is set.  The flags and the scalars after a chunk are copied to pinned
-// memory and read while the next chunk runs, so the device does not idle for
+// is set.  The flags and the scalars after a chunk are written to pinned
+// memory (k_chunk_flags, the chunk's last kernel: launch(nit, slot) enqueues
+// it) and read while the next chunk runs, so the device does not idle for
 // the host round trip between chunks.  Every iteration enqueued past the stop
 // still dispatches its kernels (they return at once, a few microseconds each
 // with the dispatch gaps), so near the stop the run switches to short
@@ -1326,19 +1329,59 @@ int cg_iteration(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rol
 //  - a stop predicted inside the next chunk enqueues a short one.
 // A wrong prediction costs one host round trip or a few skipped iterations;
 // the iterations themselves are the same.
+// The done / iteration flags and the scalars after a chunk, written straight
+// into the pinned host copies (coherent host memory: the stores cross to the
+// host, and the system-scope fence orders them before the kernel's end,
+// which the chunk's event follows).  One kernel instead of two
+// device-to-host copies, each a runtime copy kernel with its own dispatch;
+// inside a chunk's graph it follows the last iteration without the idle
+// time a copy after the graph waits.
+__global__ void k_chunk_flags(const int *ic, const double *sc, int *hflag, double *hsc) {
+  const int t = threadIdx.x;
+  if (t < 2) hflag[t] = ic[t];
+  if (t < S_NS) hsc[t] = sc[t];
+  __threadfence_system();
+}
+
+int chunk_flags(iga_solver *s, int slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_chunk_flags, dim3(1), dim3(64), 0, st, s->ic, s->sc, s->dflag + 2 * slot,
+                     s->dsc + S_NS * slot);
+  return hipGetLastError() == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+}
+
+// Capture body(stream, slot) once per flag slot into gexec[slot].
+template <typename F>
+int capture_chunks(hipGraphExec_t *gexec, F &&body) {
+  for (int slot = 0; slot < 2; ++slot) {
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return IGA_ERR_HIP;
+    hipGraph_t g;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return IGA_ERR_HIP;
+    }
+    const int rc = body(cs, slot);
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
+    e = hipGraphInstantiate(&gexec[slot], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      gexec[slot] = nullptr;
+      return IGA_ERR_HIP;
+    }
+  }
+  return IGA_OK;
+}
+
 template <typename L>
 int run_chunks(iga_solver *s, hipStream_t st, L &&launch, int big, int small, int *iters,
                int *done) {
   int nit_of[2] = {0, 0};
   auto issue = [&](int slot, int nit) -> int {
-    const int rc = launch(nit);
+    const int rc = launch(nit, slot);
     if (rc) return rc;
-    if (hipMemcpyAsync(s->hflag + 2 * slot, s->ic, 2 * sizeof(int), hipMemcpyDeviceToHost, st) !=
-            hipSuccess ||
-        hipMemcpyAsync(s->hsc + S_NS * slot, s->sc, S_NS * sizeof(double), hipMemcpyDeviceToHost,
-                       st) != hipSuccess ||
-        hipEventRecord(s->fev[slot], st) != hipSuccess)
-      return IGA_ERR_HIP;
+    if (hipEventRecord(s->fev[slot], st) != hipSuccess) return IGA_ERR_HIP;
     nit_of[slot] = nit;
     return IGA_OK;
   };
@@ -1432,25 +1475,14 @@ int cg_run(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, hip
   if (s->o.assembly != IGA_ASSEMBLY_INVLM_GATHER &&
       hipMemsetAsync(Ap, 0, (size_t)n * sizeof(V), st) != hipSuccess)
     return IGA_ERR_HIP;
-  if (s->o.use_graph && !*gexec) {
-    // capture one chunk; the graph reads every scalar from device memory
-    hipStream_t cs;
-    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return IGA_ERR_HIP;
-    hipGraph_t g;
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      (void)hipStreamDestroy(cs);
-      return IGA_ERR_HIP;
-    }
-    int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, cs, chunk_big(s));
-    hipError_t e = hipStreamEndCapture(cs, &g);
-    (void)hipStreamDestroy(cs);
-    if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
-    e = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      *gexec = nullptr;
-      return IGA_ERR_HIP;
-    }
+  if (s->o.use_graph && !gexec[0]) {
+    // capture one chunk and its flags per slot; the graphs read every
+    // scalar from device memory
+    const int rc = capture_chunks(gexec, [&](hipStream_t cs, int slot) {
+      const int r2 = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, cs, chunk_big(s));
+      return r2 ? r2 : chunk_flags(s, slot, cs);
+    });
+    if (rc) return rc;
   }
   return cg_resume<V, TU>(s, f64, u, r, p, Ap, z, rold, st, gexec, iters, done);
 }
@@ -1462,10 +1494,11 @@ int cg_resume(iga_solver *s, bool f64, TU *u, V *r, V *p, V *Ap, V *z, V *rold, 
   const int big = chunk_big(s);
   return run_chunks(
       s, st,
-      [&](int nit) -> int {
-        if (nit == big && s->o.use_graph && *gexec)
-          return hipGraphLaunch(*gexec, st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
-        return cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st, nit);
+      [&](int nit, int slot) -> int {
+        if (nit == big && s->o.use_graph && gexec[slot])
+          return hipGraphLaunch(gexec[slot], st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+        const int rc = cg_chunk<V, TU>(s, f64, u, r, p, Ap, z, rold, st, nit);
+        return rc ? rc : chunk_flags(s, slot, st);
       },
       big, chunk_small(s), iters, done);
 }
@@ -1528,31 +1561,20 @@ int cgf_run(iga_solver *s, bool f64, TU *x, V *r, V *p, V *sv, V *u, V *w, hipSt
   if (s->o.assembly != IGA_ASSEMBLY_INVLM_GATHER &&
       hipMemsetAsync(w, 0, (size_t)s->n * sizeof(V), st) != hipSuccess)
     return IGA_ERR_HIP;
-  if (s->o.use_graph && !*gexec) {
-    hipStream_t cs;
-    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return IGA_ERR_HIP;
-    hipGraph_t g;
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      (void)hipStreamDestroy(cs);
-      return IGA_ERR_HIP;
-    }
-    int rc = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, cs, CHUNK);
-    hipError_t e = hipStreamEndCapture(cs, &g);
-    (void)hipStreamDestroy(cs);
-    if (rc || e != hipSuccess) return rc ? rc : IGA_ERR_HIP;
-    e = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      *gexec = nullptr;
-      return IGA_ERR_HIP;
-    }
+  if (s->o.use_graph && !gexec[0]) {
+    const int rc = capture_chunks(gexec, [&](hipStream_t cs, int slot) {
+      const int r2 = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, cs, CHUNK);
+      return r2 ? r2 : chunk_flags(s, slot, cs);
+    });
+    if (rc) return rc;
   }
   return run_chunks(
       s, st,
-      [&](int nit) -> int {
+      [&](int nit, int slot) -> int {
         if (nit == CHUNK && s->o.use_graph)
-          return hipGraphLaunch(*gexec, st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
-        return cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, st, nit);
+          return hipGraphLaunch(gexec[slot], st) == hipSuccess ? IGA_OK : IGA_ERR_HIP;
+        const int rc = cgf_chunk<V, TU>(s, f64, x, r, p, sv, u, w, st, nit);
+        return rc ? rc : chunk_flags(s, slot, st);
       },
       CHUNK, CHUNK_SHORT, iters, done);
 }
@@ -1695,10 +1717,12 @@ void iga_solver_opts_default(iga_solver_opts *o) {
 int iga_solver_destroy(iga_solver *s) {
   if (!s) return IGA_OK;
   (void)hipSetDevice(s->device);
-  if (s->g64) (void)hipGraphExecDestroy(s->g64);
-  if (s->g32) (void)hipGraphExecDestroy(s->g32);
-  if (s->gf64) (void)hipGraphExecDestroy(s->gf64);
-  if (s->gf32) (void)hipGraphExecDestroy(s->gf32);
+  for (int k = 0; k < 2; ++k) {
+    if (s->g64[k]) (void)hipGraphExecDestroy(s->g64[k]);
+    if (s->g32[k]) (void)hipGraphExecDestroy(s->g32[k]);
+    if (s->gf64[k]) (void)hipGraphExecDestroy(s->gf64[k]);
+    if (s->gf32[k]) (void)hipGraphExecDestroy(s->gf32[k]);
+  }
   if (s->gblk) (void)hipGraphExecDestroy(s->gblk);
   for (hipEvent_t e : s->fev)
     if (e) (void)hipEventDestroy(e);
@@ -1777,9 +1801,11 @@ int iga_solver_create(iga_op *op, const iga_fem *fem, const iga_solver_opts *o_i
     SA(s->z32, n * 4);
     SA(s->rold32, n * 4);
   }
-  if (hipHostMalloc((void **)&s->hflag, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&s->hsc, 2 * S_NS * sizeof(double), hipHostMallocDefault) !=
+  if (hipHostMalloc((void **)&s->hflag, 4 * sizeof(int), hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void **)&s->hsc, 2 * S_NS * sizeof(double), hipHostMallocCoherent) !=
           hipSuccess ||
+      hipHostGetDevicePointer((void **)&s->dflag, s->hflag, 0) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&s->dsc, s->hsc, 0) != hipSuccess ||
       hipEventCreateWithFlags(&s->fev[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&s->fev[1], hipEventDisableTiming) != hipSuccess)
     return fail(IGA_ERR_HIP);
@@ -1954,7 +1980,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
              dinv, n, s->cgs);
       hipLaunchKernelGGL(k_cgf_init_fin, dim3(1), dim3(RB), 0, st, s->cgs, nb, s->sc, s->ic,
                          o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit, 0.0);
-      rc = cgf_run<double, double>(s, true, s->x64, s->r, s->p, s->rold, s->z, s->Ap, st, &s->gf64,
+      rc = cgf_run<double, double>(s, true, s->x64, s->r, s->p, s->rold, s->z, s->Ap, st, s->gf64,
                            &iters, &done);
     } else {
       LAUNCH((k_cg_init<double, double, double>), nb, f, 1.0, s->x64, s->r, s->z, s->p, dinv,
@@ -1962,7 +1988,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
       hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(RB), 0, st, s->part, nb, s->sc, s->ic,
                          o.rtol * o.rtol, o.abstol * o.abstol, o.divtol * o.divtol, o.maxit,
                          0.0);
-      rc = cg_run<double, double>(s, true, s->x64, s->r, s->p, s->Ap, s->z, s->rold, st, &s->g64, &iters,
+      rc = cg_run<double, double>(s, true, s->x64, s->r, s->p, s->Ap, s->z, s->rold, st, s->g64, &iters,
                           &done);
     }
     if (rc) return rc;
@@ -2000,7 +2026,7 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
                          tol2 / rr, 0.0, o.divtol * o.divtol * ff / rr, o.maxit, RU2);
       for (;;) {
         rc = cgf_run<float, double>(s, false, s->d64, s->r32, s->p32, s->rold32, s->z32, s->Ap32, st,
-                            &s->gf32, &iters, &done);
+                            s->gf32, &iters, &done);
         if (rc) return rc;
         if (done == 3) { status = IGA_ERR_DIVERGED; break; }
         // reliable update
@@ -2024,11 +2050,11 @@ int iga_solver_solve(iga_solver *s, const double *f, double *u, void *stream,
       for (;;) {
         if (first) {
           rc = cg_run<float, double>(s, false, s->d64, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
-                             &s->g32, &iters, &done);
+                             s->g32, &iters, &done);
           first = false;
         } else {
           rc = cg_resume<float, double>(s, false, s->d64, s->r32, s->p32, s->Ap32, s->z32, s->rold32, st,
-                                &s->g32, &iters, &done);
+                                s->g32, &iters, &done);
         }
         if (rc) return rc;
         if (done == 3) { status = IGA_ERR_DIVERGED; break; }
