@@ -464,11 +464,25 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
   if (b0 < n) load(b0, zv, pv);
   double f[3];
   {
+    // partials in 16-byte pairs: the update kernel's 512 in one round of
+    // loads per thread (the slots are RG-aligned)
+    typedef double d2 __attribute__((ext_vector_type(2)));
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int i = threadIdx.x; i < nb; i += RB) {
-      a0 += part[P_RR * RG + i];
-      a1 += part[P_RZ * RG + i];
-      if (flexible) a2 += part[P_RZO * RG + i];
+    const int np = nb >> 1;
+    for (int i = threadIdx.x; i < np; i += RB) {
+      const d2 v0 = reinterpret_cast<const d2 *>(part + P_RR * RG)[i];
+      const d2 v1 = reinterpret_cast<const d2 *>(part + P_RZ * RG)[i];
+      a0 += v0.x + v0.y;
+      a1 += v1.x + v1.y;
+      if (flexible) {
+        const d2 v2 = reinterpret_cast<const d2 *>(part + P_RZO * RG)[i];
+        a2 += v2.x + v2.y;
+      }
+    }
+    if ((nb & 1) && threadIdx.x == 0) {
+      a0 += part[P_RR * RG + nb - 1];
+      a1 += part[P_RZ * RG + nb - 1];
+      if (flexible) a2 += part[P_RZO * RG + nb - 1];
     }
     f[0] = a0;
     f[1] = a1;
