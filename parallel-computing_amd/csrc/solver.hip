@@ -299,11 +299,10 @@ __global__ __launch_bounds__(RB) void k_cg_update(TU *__restrict__ u, V *__restr
   Ap += voff_;
   z += voff_;
   if (ROLD) rold += voff_;
-  if (ic[I_DONE] || (outer_done && *outer_done)) return;
   // Four elements per thread and step, every load of a step issued before
-  // any store; the first step's loads are issued before the p.Ap fold and
-  // each next step's before the current one is processed, so their latency
-  // overlaps the fold and the stores.
+  // any store; the first step's loads are issued before the done flags are
+  // read and the p.Ap fold, and each next step's before the current one is
+  // processed, so their latency overlaps the flags, the fold and the stores.
   constexpr int U = 4;
   const int64_t step = (int64_t)gridDim.x * RB * U;
   using SP = igai::SeamPart<V>;
@@ -345,13 +344,25 @@ __global__ __launch_bounds__(RB) void k_cg_update(TU *__restrict__ u, V *__restr
   D dv[U];
   SP sp[U];
   if (b0 < n) load(b0, uu, pv, rv, av, dv, sp);
+  const bool stop = ic[I_DONE] || (outer_done && *outer_done);
+  const double rz_old = sc[S_RZ];
   // p.Ap: partials of k_cg_pap, or the apply kernel's per-tile partials
   // (FP32: the split-element apply's 8,192 tile partials at 1024^2, sixteen
-  // 16-byte loads in flight)
-  const double pap = tdot ? block_sum_all(sizeof(V) == 4 ? fold_pairs<16>(tdot, ntiles)
-                                                          : fold_share<8>(tdot, ntiles))
-                          : fold(part + P_PAP * RG, nb);
-  const double a = sc[S_RZ] / pap;
+  // 16-byte loads in flight), summed over the workgroup together with the
+  // stop flag, so that every thread returns or none
+  double red[2];
+  if (tdot) {
+    red[0] = sizeof(V) == 4 ? fold_pairs<16>(tdot, ntiles) : fold_share<8>(tdot, ntiles);
+  } else {
+    double sp0 = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) sp0 += part[P_PAP * RG + i];
+    red[0] = sp0;
+  }
+  red[1] = stop ? 1.0 : 0.0;
+  block_sum_n<2>(red);
+  if (red[1] != 0.0) return;
+  const double pap = red[0];
+  const double a = rz_old / pap;
   if (!finite(a) || pap == 0.0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ic[I_DONE] = 3;
     return;
@@ -398,7 +409,7 @@ __global__ __launch_bounds__(RB) void k_cg_update(TU *__restrict__ u, V *__restr
   // bookkeeping for K3 (slots no workgroup of this kernel reads)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[S_ALPHA] = a;
-    sc[S_RZ_OLD] = sc[S_RZ];
+    sc[S_RZ_OLD] = rz_old;
     ic[I_NEXT] = ic[I_ITER] + 1;
   }
 }
@@ -436,9 +447,6 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
   p += voff_;
   z += voff_;
   if (Apz) Apz += voff_;
-  if (outer_done && *outer_done) return;
-  const int it = ic[I_NEXT];
-  if (ic[I_TOK + blockIdx.x] == it) return;
   constexpr int U = 4;
   const int64_t step = (int64_t)gridDim.x * RB * U;
   auto load = [&](int64_t bq, V (&zv)[U], V (&pv)[U]) {
@@ -457,11 +465,17 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
       }
     }
   };
-  // first step's loads before the folds, each next step's before the current
-  // one is processed (latency overlap, as in k_cg_update)
+  // first step's loads before the flags, scalars and folds, each next
+  // step's before the current one is processed (latency overlap, as in
+  // k_cg_update); the scalars the test below reads are loaded with the flags
   int64_t b0 = (int64_t)blockIdx.x * RB * U + threadIdx.x;
   V zv[U], pv[U];
   if (b0 < n) load(b0, zv, pv);
+  if (outer_done && *outer_done) return;
+  const int it = ic[I_NEXT];
+  if (ic[I_TOK + blockIdx.x] == it) return;
+  const double tol2 = sc[S_TOL2], div2 = sc[S_DIV2], upd2 = sc[S_UPD2], rzo = sc[S_RZ_OLD];
+  const int maxit = ic[I_MAXIT];
   double f[3];
   {
     // partials in 16-byte pairs: the update kernel's 512 in one round of
@@ -492,11 +506,11 @@ __global__ void k_cg_pupdate(V *__restrict__ p, const V *__restrict__ z, int64_t
   if (threadIdx.x == 0) ic[I_TOK + blockIdx.x] = it;
   const double rr = f[0], rz = f[1], ro = f[2];
   int st = 0;
-  if (rr <= sc[S_TOL2]) st = 1;
-  else if (!finite(rr) || rr > sc[S_DIV2]) st = 3;
-  else if (it >= ic[I_MAXIT]) st = 2;
-  else if (rr <= sc[S_UPD2]) st = 4;
-  const double b = (rz - ro) / sc[S_RZ_OLD];
+  if (rr <= tol2) st = 1;
+  else if (!finite(rr) || rr > div2) st = 3;
+  else if (it >= maxit) st = 2;
+  else if (rr <= upd2) st = 4;
+  const double b = (rz - ro) / rzo;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[S_RR] = rr;
     sc[S_RZ] = rz;
