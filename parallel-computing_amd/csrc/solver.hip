@@ -404,8 +404,17 @@ __global__ __launch_bounds__(RB) void k_cg_update(TU *__restrict__ u, V *__restr
     }
     b0 = bn;
   }
-  put_partial(part + P_RR * RG, rr);
-  if (LOCAL) put_partial(part + P_RZ * RG, rz);
+  if constexpr (LOCAL) {
+    // both partials through one set of barriers (the sums of put_partial)
+    double v[2] = {rr, rz};
+    block_sum_n<2>(v);
+    if (threadIdx.x == 0) {
+      part[P_RR * RG + blockIdx.x] = v[0];
+      part[P_RZ * RG + blockIdx.x] = v[1];
+    }
+  } else {
+    put_partial(part + P_RR * RG, rr);
+  }
   // bookkeeping for K3 (slots no workgroup of this kernel reads)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[S_ALPHA] = a;
